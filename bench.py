@@ -1,0 +1,181 @@
+"""Benchmark: TDCCSD (T + Lambda) RK4 steps/s on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--method CCSD] [--o 20 --v 160]
+
+Workload (defaults): real-time TDCCSD at o = 20, v = 160 spin orbitals, the
+reference's partial-RK4 propagator (four decoupled chains, 16 right-hand sides
+per step, rounding to 15 decimals, dipole after every step), dt = 0.001,
+z field with w0 = 0.923, A = 0.002, FP64 on the GPU.  Inputs are synthetic
+(seeded Philox generator; the TEI is generated on the device), the initial
+state is the MP2 guess for t2 with t1 = 0 and lam = t (the ground-state solve
+is not part of the timed path).
+
+Prints one JSON line.  `value` is steps/s of the timed region (inputs resident
+in HBM, graph-replayed steps, synchronised on both sides).  `roofline` comes
+from one eager profiled step after the timed region (HIP events around every
+launch): the dominant kernel is the kernel name with the largest summed time.
+`cpu_baseline` times the oracle (numpy complex128, the reference's einsum
+strings) on a bounded sample and scales it to the workload by F_ref.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "TDCCSD RK4 steps/s (T+Λ) o=20,v=160: FP32-split vs FP64 at dipole err≤1e-8"
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def initial_state(sysm, o, v):
+    from python_code_to_check_c_amd import synthetic
+    eps = np.linalg.eigh(sysm.F)[0]
+    eo, ev = eps[:o], eps[o:]
+    D = eo.reshape(-1, 1, 1, 1) + eo.reshape(-1, 1, 1) - ev.reshape(-1, 1) - ev
+    oovv = synthetic.tei_block(sysm.config_id, o, v, "oovv", sysm.tei_sigma)
+    t2 = (oovv / D).astype(complex)
+    t1 = np.zeros((o, v), complex)
+    return t1, t2, t1.copy(), t2.copy()
+
+
+def cpu_baseline(method, o_full, v_full, budget_s=20.0):
+    """Time the oracle's RK4 step at a reduced shape and scale by F_ref."""
+    from oracle import rk4
+    from oracle.cc_equations import CCEquations
+    from oracle.flops import step_flops
+    from python_code_to_check_c_amd import synthetic
+    o, v = min(o_full, 20), 24
+    s = synthetic.SyntheticSystem(90, o, v, tei_sigma=0.02)
+    eq = CCEquations(s.TEI, o)
+    t1, t2, l1, l2 = initial_state(s, o, v)
+    Vt = lambda x: rk4.field(s.mu[2], 0.002, 0.923, x)
+    steps, t0 = 0, time.perf_counter()
+    t = 0.0
+    state = (t1, t2, l1, l2)
+    while True:
+        t += 0.001
+        state, _ = rk4.rk4_step(eq, method, s.F, *state, t, 0.001, Vt, 15)
+        eq.dipole_moment(*state, s.mu)
+        steps += 1
+        if time.perf_counter() - t0 > budget_s or steps >= 50:
+            break
+    el = time.perf_counter() - t0
+    f_small = step_flops(o, v, method)[0]
+    f_full = step_flops(o_full, v_full, method)[0]
+    sps = steps / el * f_small / f_full
+    cores = len(os.sched_getaffinity(0))
+    return {"value": sps, "unit": "steps/s", "cores": cores, "kind": "port",
+            "sample": f"oracle (numpy complex128 einsum, reference strings) {steps} {method} RK4 steps at "
+                      f"o={o}, v={v} in {el:.2f} s on {cores} host threads, scaled to o={o_full}, v={v_full} "
+                      f"by F_ref {f_small:.3e} -> {f_full:.3e} flop/step"}
+
+
+def roofline(records):
+    groups = {}
+    for name, fl, by, ms in records:
+        g = groups.setdefault(name, [0.0, 0.0, 0.0, 0])
+        g[0] += fl
+        g[1] += by
+        g[2] += ms
+        g[3] += 1
+    name, (fl, by, ms, calls) = max(groups.items(), key=lambda kv: kv[1][2])
+    avg_ms = ms / calls
+    if fl > 0:
+        achieved = fl / (ms * 1e-3) / 1e12
+        return name, {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                      "kernel": name, "calls_per_step": calls, "avg_ms": avg_ms,
+                      "flops_per_launch": fl / calls}, groups
+    achieved = by / (ms * 1e-3) / 1e9
+    return name, {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                  "frac": achieved / PEAK_HBM_GBS, "traffic": None, "kernel": name,
+                  "calls_per_step": calls, "avg_ms": avg_ms}, groups
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--method", default="CCSD", choices=["CCSD", "CC2"])
+    ap.add_argument("--o", type=int, default=20)
+    ap.add_argument("--v", type=int, default=160)
+    ap.add_argument("--config-id", type=int, default=3)
+    ap.add_argument("--tei-sigma", type=float, default=0.005)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-profile", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding of this "
+              "path is a design, not a run.", file=sys.stderr)
+        sys.exit(2)
+
+    from python_code_to_check_c_amd import _lib, synthetic
+    from oracle.flops import step_flops
+
+    o, v = args.o, args.v
+    dt, w0, A, pol, precs = 0.001, 0.923, 0.002, 2, 15
+    t_setup = time.perf_counter()
+    sysm = synthetic.SyntheticSystem(args.config_id, o, v, tei_sigma=args.tei_sigma, tei_on_device=True)
+    ctx = _lib.Context(o, v)
+    ctx.set_system(None, sysm.F, sysm.mu)
+    ctx.synth_tei(args.config_id, args.tei_sigma)
+    ctx.set_state(*initial_state(sysm, o, v))
+    flops_exec, arena = ctx.step_info(args.method, dt)
+    print(f"[bench] setup {time.perf_counter() - t_setup:.1f} s; step arena {arena / 2**30:.2f} GiB; "
+          f"executed contraction flops/step {flops_exec:.4e}", file=sys.stderr, flush=True)
+
+    t = 0.0
+    if args.warmup:
+        _, ts = ctx.rk4_run(args.method, args.warmup, t, dt, w0, A, pol, precs)
+        t = float(ts[-1])
+    print("[bench] warmup done", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    mu, ts = ctx.rk4_run(args.method, args.steps, t, dt, w0, A, pol, precs)
+    el = time.perf_counter() - t0
+    t = float(ts[-1])
+    sps = args.steps / el
+    print(f"[bench] timed {args.steps} steps in {el:.3f} s", file=sys.stderr, flush=True)
+    if not np.all(np.isfinite(mu)):
+        raise RuntimeError("non-finite dipole in the timed region")
+
+    f_ref, f_alg = step_flops(o, v, args.method)
+    out = {
+        "metric": METRIC, "value": sps, "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"T{args.method} (T+Lambda) partial-RK4 propagation, o={o}, v={v} spin orbitals, "
+                               f"dt={dt}, z field w0={w0} A={A}, precs={precs}",
+                   "method": args.method, "precision": "fp64", "o": o, "v": v,
+                   "tei_sigma": args.tei_sigma, "config_id": args.config_id,
+                   "initial_state": "MP2 t2, t1=0, lam=t",
+                   "F_ref_flop_per_step": f_ref, "F_alg_flop_per_step": f_alg,
+                   "F_exec_contraction_flop_per_step": flops_exec,
+                   "ref_equiv_TFLOPs": f_alg * sps / 1e12,
+                   "exec_TFLOPs": flops_exec * sps / 1e12},
+    }
+    if not args.no_profile:
+        recs = ctx.profile_step(args.method, t, dt, w0, A, pol, precs)
+        name, roof, groups = roofline(recs)
+        out["roofline"] = roof
+        top = sorted(groups.items(), key=lambda kv: -kv[1][2])[:6]
+        out["config"]["profile_top"] = {k: {"ms_per_step": g[2], "calls": g[3],
+                                            "TFLOPs": (g[0] / (g[2] * 1e-3) / 1e12) if g[0] else None}
+                                        for k, g in top}
+        out["config"]["profiled_step_ms"] = sum(r[3] for r in recs)
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.method, o, v)
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,115 @@
+/*
+ * tdcc.h -- C ABI of the MI355X real-time TDCCSD / TDCC2 propagation library
+ * (libtdcc_hip.so, built for gfx950).
+ *
+ * The reference (yramis/python_code_to_check_C) is pure Python with no FFI;
+ * its boundary for this path is the Python surface of CCSD_Helper /
+ * CC2_Helper / CC_Calculator.  Each entry point below replaces one piece of
+ * that surface and cites it.  The Python mirror of the reference surface
+ * (python_code_to_check_c_amd/CCSD_Helper.py, CC2_Helper.py, CC_Calculator.py)
+ * binds these symbols through ctypes.
+ *
+ * Conventions
+ *   - status: 0 = ok, nonzero = error; tdcc_last_error() gives the message.
+ *   - complex arrays are interleaved (re, im) doubles in C order, i.e. the
+ *     memory of a numpy complex128 array; real arrays are C-order doubles.
+ *   - shapes: o = occupied, v = virtual spin orbitals, n = o + v;
+ *     t1/lam1 (o,v), t2/lam2 (o,o,v,v), F (n,n), TEI (n,n,n,n) = <pq||rs>
+ *     occupied first, mu (3,n,n).
+ *   - host buffers are borrowed for the duration of a call only; the context
+ *     owns all device memory; calls are synchronous for the caller.
+ *   - a context is not thread-safe; separate contexts may be used concurrently.
+ */
+#ifndef TDCC_H
+#define TDCC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tdcc_ctx tdcc_ctx;
+
+enum tdcc_method { TDCC_CCSD = 0, TDCC_CC2 = 1 };
+
+enum tdcc_rhs_kind {
+  TDCC_RHS_T1 = 0,      /* CCSD_Helper.T1eq_rhs(t1, t2, F)             CCSD_Helper.py:177 */
+  TDCC_RHS_T2 = 1,      /* CCSD_Helper.T2eq_rhs(t1, t2, F)             CCSD_Helper.py:268 */
+  TDCC_RHS_L1 = 2,      /* CCSD_Helper.lam_1eq_rhs(t1,t2,lam1,lam2,F)  CCSD_Helper.py:555 */
+  TDCC_RHS_L2 = 3,      /* CCSD_Helper.lam2eq_rhs(t1,t2,lam1,lam2,F)   CCSD_Helper.py:610 */
+  TDCC_RHS_T2_CC2 = 4,  /* CC2_Helper.T2eq_rhs_CC2(t1, t2, F)          CC2_Helper.py:12   */
+  TDCC_RHS_L1_CC2 = 5,  /* CC2_Helper.L1eq_rhs_CC2(...)                CC2_Helper.py:50   */
+  TDCC_RHS_L2_CC2 = 6   /* CC2_Helper.L2eq_rhs_CC2(...)                CC2_Helper.py:128  */
+};
+
+/* Create a context for one system of o occupied / v virtual spin orbitals on
+ * HIP device `device`.  Replaces the psi4-backed CCSD_Helper.__init__
+ * (CCSD_Helper.py:26-53) as far as this path needs it (sizes, slices). */
+int tdcc_create(int nocc, int nvir, int device, tdcc_ctx** out);
+void tdcc_destroy(tdcc_ctx* ctx);
+const char* tdcc_last_error(const tdcc_ctx* ctx);
+
+/* Integrals, Fock and dipoles (CCSD_Helper.py:53 self.TEI, :82-93 F_MO,
+ * :970-980 Defd_dipole).  tei may be NULL when tdcc_synth_tei() provides it. */
+int tdcc_set_system(tdcc_ctx* ctx, const double* tei, const double* fock, const double* mu);
+
+/* Generate the antisymmetrized synthetic TEI on the device (seeded
+ * Philox4x32-10 generator, identical to python_code_to_check_c_amd.synthetic). */
+int tdcc_synth_tei(tdcc_ctx* ctx, uint32_t config_id, double sigma);
+
+/* Propagated state t1, t2, lam1, lam2 (complex, interleaved). */
+int tdcc_set_state(tdcc_ctx* ctx, const double* t1, const double* t2, const double* l1, const double* l2);
+/* which = 0: current state; which = 1: rounded base of the last step taken
+ * (the t1min/t2min/L1min/L2min the reference saves on a stop,
+ * CCSD_Helper.py:1207-1214, :1243). */
+int tdcc_get_state(tdcc_ctx* ctx, int which, double* t1, double* t2, double* l1, double* l2);
+
+/* One right-hand side at the given complex F (n,n); out has the shape of the
+ * residual (o,v) or (o,o,v,v).  Unused amplitude arguments may be NULL only
+ * for the kinds that do not read them (T1/T2 kinds ignore lam1/lam2). */
+int tdcc_rhs(tdcc_ctx* ctx, int kind, const double* F, const double* t1, const double* t2,
+             const double* l1, const double* l2, double* out);
+
+/* One named intermediate of the reference's residuals, evaluated with the
+ * same graph builders the residuals use (parity checking): "Fae", "Fmi",
+ * "Fme", "Wmnij", "Wmnij_2", "Wabef", "Wabef_2", "Wmbej" (CCSD_Helper.py:136-265),
+ * "LRFea", "LRFim", "LSWieam", "LRWefam", "LRWibjm", "Gfe", "Gmn", "LWfiea",
+ * "LWmina", "LSWmnij", "LSWabef" (:447-607), "Dij", "Dab", "Dai" (:984-1006).
+ * out receives out_len complex values in the reference's index order. */
+int tdcc_intermediate(tdcc_ctx* ctx, const char* name, const double* F, const double* t1,
+                      const double* t2, const double* l1, const double* l2, double* out,
+                      int64_t out_len);
+
+/* Dipole expectation value of the correlated density for (t1,t2,lam1,lam2):
+ * mu_out[3] complex (x, y, z).  CCSD_Helper.dipole_moment, CCSD_Helper.py:1066. */
+int tdcc_dipole(tdcc_ctx* ctx, const double* t1, const double* t2, const double* l1,
+                const double* l2, double* mu_out);
+
+/* Run `nsteps` iterations of the reference's RK4 loop body on the current state
+ * (CCSD_Helper.py:1206-1240 / CC2_Helper.py:453-485 without the stop rules and
+ * the file output, which the caller applies): time starts at t0 and is
+ * advanced by dt before each step; the field is -A mu[pol] exp(i 2 pi w0 t).
+ * Amplitudes are rounded to `precs` decimals before each update.
+ * mu_out: nsteps x 3 complex (dipole after each step); t_out: nsteps times. */
+int tdcc_rk4_run(tdcc_ctx* ctx, int method, int nsteps, double t0, double dt, double w0,
+                 double A, int pol, int precs, double* mu_out, double* t_out);
+
+/* Take one step of the same program eagerly with a HIP event pair around
+ * every kernel launch; `report` receives one line per launch:
+ * "<kernel name>\t<algorithmic flops>\t<algorithmic bytes>\t<milliseconds>\n". */
+int tdcc_profile_step(tdcc_ctx* ctx, int method, double t0, double dt, double w0, double A, int pol,
+                      int precs, char* report, int64_t report_len);
+
+/* Device-side copy of the state: op 0 saves it, op 1 restores it. */
+int tdcc_checkpoint(tdcc_ctx* ctx, int op);
+
+/* Executed contraction flops (real) of one RK4 step program, and its device
+ * arena size in bytes. */
+int tdcc_step_info(tdcc_ctx* ctx, int method, double dt, int64_t* flops, int64_t* arena_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TDCC_H */

@@ -1,0 +1,222 @@
+"""ctypes binding of libtdcc_hip.so (the C ABI declared in include/tdcc.h).
+
+There is no CPU fallback: if the library is missing or no HIP device is
+available, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libtdcc_hip.so")
+
+# Every symbol include/tdcc.h declares.
+EXPORTS = (
+    "tdcc_create", "tdcc_destroy", "tdcc_last_error", "tdcc_set_system", "tdcc_synth_tei",
+    "tdcc_set_state", "tdcc_get_state", "tdcc_rhs", "tdcc_intermediate", "tdcc_dipole",
+    "tdcc_rk4_run", "tdcc_profile_step", "tdcc_checkpoint", "tdcc_step_info",
+)
+
+RHS_T1, RHS_T2, RHS_L1, RHS_L2, RHS_T2_CC2, RHS_L1_CC2, RHS_L2_CC2 = range(7)
+METHODS = {"CCSD": 0, "CC2": 1}
+
+
+class TdccError(RuntimeError):
+    pass
+
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def load():
+    """Load the HIP library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise TdccError(f"{LIB_PATH} is missing; build it with "
+                        "`python -m python_code_to_check_c_amd.build`")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    i64p = ctypes.POINTER(ctypes.c_int64)
+    sig = {
+        "tdcc_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]),
+        "tdcc_destroy": (None, [vp]),
+        "tdcc_last_error": (ctypes.c_char_p, [vp]),
+        "tdcc_set_system": (ctypes.c_int, [vp, _dp, _dp, _dp]),
+        "tdcc_synth_tei": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_double]),
+        "tdcc_set_state": (ctypes.c_int, [vp, _dp, _dp, _dp, _dp]),
+        "tdcc_get_state": (ctypes.c_int, [vp, ctypes.c_int, _dp, _dp, _dp, _dp]),
+        "tdcc_rhs": (ctypes.c_int, [vp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
+        "tdcc_intermediate": (ctypes.c_int, [vp, ctypes.c_char_p, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int64]),
+        "tdcc_dipole": (ctypes.c_int, [vp, _dp, _dp, _dp, _dp, _dp]),
+        "tdcc_rk4_run": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp]),
+        "tdcc_profile_step": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
+                                             ctypes.c_int64]),
+        "tdcc_checkpoint": (ctypes.c_int, [vp, ctypes.c_int]),
+        "tdcc_step_info": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, i64p, i64p]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _c(x):
+    a = np.ascontiguousarray(x, dtype=np.complex128)
+    return a, a.ctypes.data_as(_dp)
+
+
+def _r(x):
+    a = np.ascontiguousarray(x, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+def _opt(x):
+    if x is None:
+        return None, None
+    return _c(x)
+
+
+class Context:
+    """One device context: integrals, Fock, dipoles and the propagated state."""
+
+    def __init__(self, nocc, nvir, device=0):
+        self.lib = load()
+        self.o, self.v, self.n = int(nocc), int(nvir), int(nocc) + int(nvir)
+        h = ctypes.c_void_p()
+        if self.lib.tdcc_create(self.o, self.v, int(device), ctypes.byref(h)) != 0 or not h.value:
+            raise TdccError(f"tdcc_create({nocc}, {nvir}, device={device}) failed "
+                            "(no HIP device, or out of device memory)")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self.lib.tdcc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.tdcc_last_error(self._h)
+            raise TdccError(f"{what}: {msg.decode() if msg else 'error'}")
+
+    # ------------------------------------------------------------------ setup
+    def set_system(self, TEI, F, mu):
+        n = self.n
+        F_, pF = _r(F)
+        mu_, pmu = _r(mu)
+        if F_.shape != (n, n) or mu_.shape != (3, n, n):
+            raise ValueError("F must be (n,n) and mu (3,n,n)")
+        if TEI is None:
+            self._check(self.lib.tdcc_set_system(self._h, None, pF, pmu), "tdcc_set_system")
+            return
+        T_, pT = _r(TEI)
+        if T_.shape != (n, n, n, n):
+            raise ValueError("TEI must be (n,n,n,n)")
+        self._check(self.lib.tdcc_set_system(self._h, pT, pF, pmu), "tdcc_set_system")
+
+    def synth_tei(self, config_id, sigma):
+        self._check(self.lib.tdcc_synth_tei(self._h, int(config_id), float(sigma)), "tdcc_synth_tei")
+
+    def _shape_ok(self, t1, t2):
+        if t1.shape != (self.o, self.v) or t2.shape != (self.o, self.o, self.v, self.v):
+            raise ValueError(f"amplitude shapes {t1.shape}, {t2.shape} do not match o={self.o}, v={self.v}")
+
+    def set_state(self, t1, t2, lam1, lam2):
+        a, pa = _c(t1)
+        b, pb = _c(t2)
+        c, pc = _c(lam1)
+        d, pd = _c(lam2)
+        self._shape_ok(a, b)
+        self._shape_ok(c, d)
+        self._check(self.lib.tdcc_set_state(self._h, pa, pb, pc, pd), "tdcc_set_state")
+
+    def get_state(self, which=0):
+        o, v = self.o, self.v
+        out = [np.empty((o, v), complex), np.empty((o, o, v, v), complex),
+               np.empty((o, v), complex), np.empty((o, o, v, v), complex)]
+        ptrs = [x.ctypes.data_as(_dp) for x in out]
+        self._check(self.lib.tdcc_get_state(self._h, int(which), *ptrs), "tdcc_get_state")
+        return tuple(out)
+
+    # ------------------------------------------------------------- evaluation
+    def rhs(self, kind, F, t1, t2, lam1=None, lam2=None):
+        F_, pF = _c(F)
+        a, pa = _c(t1)
+        b, pb = _c(t2)
+        self._shape_ok(a, b)
+        c, pc = _opt(lam1)
+        d, pd = _opt(lam2)
+        if F_.shape != (self.n, self.n):
+            raise ValueError("F must be (n,n)")
+        one_body = kind in (RHS_T1, RHS_L1, RHS_L1_CC2)
+        shape = (self.o, self.v) if one_body else (self.o, self.o, self.v, self.v)
+        out = np.empty(shape, complex)
+        self._check(self.lib.tdcc_rhs(self._h, int(kind), pF, pa, pb, pc, pd, out.ctypes.data_as(_dp)),
+                    "tdcc_rhs")
+        return out
+
+    def intermediate(self, name, shape, F, t1, t2, lam1, lam2):
+        F_, pF = _c(F)
+        a, pa = _c(t1)
+        b, pb = _c(t2)
+        c, pc = _c(lam1)
+        d, pd = _c(lam2)
+        out = np.empty(shape, complex)
+        self._check(self.lib.tdcc_intermediate(self._h, name.encode(), pF, pa, pb, pc, pd,
+                                               out.ctypes.data_as(_dp), out.size), f"tdcc_intermediate({name})")
+        return out
+
+    def dipole(self, t1, t2, lam1, lam2):
+        a, pa = _c(t1)
+        b, pb = _c(t2)
+        c, pc = _c(lam1)
+        d, pd = _c(lam2)
+        out = np.empty(3, complex)
+        self._check(self.lib.tdcc_dipole(self._h, pa, pb, pc, pd, out.ctypes.data_as(_dp)), "tdcc_dipole")
+        return out
+
+    def rk4_run(self, method, nsteps, t0, dt, w0, A, pol=2, precs=15):
+        m = METHODS[method] if isinstance(method, str) else int(method)
+        mu = np.empty((max(nsteps, 1), 3), complex)
+        ts = np.empty(max(nsteps, 1))
+        self._check(self.lib.tdcc_rk4_run(self._h, m, int(nsteps), float(t0), float(dt), float(w0), float(A),
+                                          int(pol), int(precs), mu.ctypes.data_as(_dp),
+                                          ts.ctypes.data_as(_dp)), "tdcc_rk4_run")
+        return mu[:nsteps], ts[:nsteps]
+
+    def profile_step(self, method, t0, dt, w0, A, pol=2, precs=15):
+        """One eager step with per-launch HIP event timing: list of (name, flops, bytes, ms)."""
+        m = METHODS[method] if isinstance(method, str) else int(method)
+        buf = ctypes.create_string_buffer(1 << 22)
+        self._check(self.lib.tdcc_profile_step(self._h, m, float(t0), float(dt), float(w0), float(A), int(pol),
+                                               int(precs), buf, len(buf)), "tdcc_profile_step")
+        out = []
+        for line in buf.value.decode().splitlines():
+            name, fl, by, ms = line.split("\t")
+            out.append((name, float(fl), float(by), float(ms)))
+        return out
+
+    def checkpoint(self, op):
+        self._check(self.lib.tdcc_checkpoint(self._h, int(op)), "tdcc_checkpoint")
+
+    def step_info(self, method, dt):
+        m = METHODS[method] if isinstance(method, str) else int(method)
+        f = ctypes.c_int64()
+        a = ctypes.c_int64()
+        self._check(self.lib.tdcc_step_info(self._h, m, float(dt), ctypes.byref(f), ctypes.byref(a)),
+                    "tdcc_step_info")
+        return f.value, a.value

@@ -1,0 +1,650 @@
+// C ABI of the propagation library (declared in include/tdcc.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/tdcc.h"
+#include "cc.hpp"
+#include "engine.hpp"
+
+using namespace tdcc;
+
+namespace {
+
+struct Planes {  // complex tensor as two planes in one allocation
+  int64_t n = 0;
+  double* d = nullptr;
+  double* re() const { return d; }
+  double* im() const { return d + n; }
+};
+
+struct Program {
+  std::unique_ptr<Graph> g;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool bound = false;
+  bool step = false;
+  // runtime parameters the launch records were bound with
+  double negA = 0, dt6 = 0, scale = 0;
+  int pol = -1;
+  const void* table = nullptr;
+  const void* mu_out = nullptr;
+};
+
+}  // namespace
+
+struct tdcc_ctx {
+  int o = 0, v = 0, n = 0, device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  double* tei = nullptr;
+  double* F = nullptr;
+  double* mu = nullptr;
+  bool have_tei = false, have_system = false;
+  Planes state[4], base[4], backup[4], io_in[4], io_out, io_F, int_out;
+  double* staging = nullptr;
+  int64_t* counter = nullptr;
+  double* table = nullptr;
+  int64_t table_cap = 0;
+  double* mu_out = nullptr;
+  int64_t mu_cap = 0;
+  double* arena = nullptr;
+  int64_t arena_bytes = 0;
+  Runtime rt;
+  std::map<std::string, Program> progs;
+};
+
+namespace {
+
+int64_t block_size(const tdcc_ctx* c, int k) {
+  const int64_t o = c->o, v = c->v;
+  return (k == 0 || k == 2) ? o * v : o * o * v * v;
+}
+
+void alloc_planes(Planes& p, int64_t n) {
+  p.n = n;
+  TDCC_HIP(hipMalloc(&p.d, 2 * n * sizeof(double)));
+  TDCC_HIP(hipMemset(p.d, 0, 2 * n * sizeof(double)));
+}
+
+void free_planes(Planes& p) {
+  if (p.d) (void)hipFree(p.d);
+  p.d = nullptr;
+}
+
+void upload_complex(tdcc_ctx* c, const double* host, const Planes& p) {
+  TDCC_HIP(hipMemcpyAsync(c->staging, host, 2 * p.n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  launch_split_complex(c->staging, p.re(), p.im(), p.n, c->stream);
+  TDCC_HIP(hipGetLastError());
+  TDCC_HIP(hipStreamSynchronize(c->stream));
+}
+
+void download_complex(tdcc_ctx* c, const Planes& p, double* host) {
+  launch_join_complex(p.re(), p.im(), c->staging, p.n, c->stream);
+  TDCC_HIP(hipGetLastError());
+  TDCC_HIP(hipMemcpyAsync(host, c->staging, 2 * p.n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  TDCC_HIP(hipStreamSynchronize(c->stream));
+}
+
+Shape shape_of_block(int k) {
+  Shape s;
+  if (k == 0 || k == 2) {
+    s.rank = 2;
+    s.cls[0] = Cls::O;
+    s.cls[1] = Cls::V;
+  } else {
+    s.rank = 4;
+    s.cls[0] = s.cls[1] = Cls::O;
+    s.cls[2] = s.cls[3] = Cls::V;
+  }
+  return s;
+}
+
+int ext_planes(Graph& g, const Planes& p, const Shape& s, const char* name) {
+  Dev d;
+  d.re = p.re();
+  d.im = p.im();
+  int64_t acc = 1;
+  for (int k = s.rank - 1; k >= 0; --k) {
+    d.st[k] = acc;
+    acc *= g.extent(s.cls[k]);
+  }
+  return g.external(s, true, d, name);
+}
+
+int ext_tei(tdcc_ctx* c, Graph& g) {
+  Shape s;
+  s.rank = 4;
+  s.cls[0] = s.cls[1] = s.cls[2] = s.cls[3] = Cls::N;
+  Dev d;
+  d.re = c->tei;
+  const int64_t n = c->n;
+  d.st[0] = n * n * n;
+  d.st[1] = n * n;
+  d.st[2] = n;
+  d.st[3] = 1;
+  return g.external(s, false, d, "tei");
+}
+
+void invalidate_exec(Program& p) {
+  if (p.exec) (void)hipGraphExecDestroy(p.exec);
+  if (p.graph) (void)hipGraphDestroy(p.graph);
+  p.exec = nullptr;
+  p.graph = nullptr;
+  p.bound = false;
+}
+
+void ensure_arena(tdcc_ctx* c, int64_t bytes) {
+  if (bytes <= c->arena_bytes) return;
+  TDCC_HIP(hipStreamSynchronize(c->stream));
+  for (auto& kv : c->progs) invalidate_exec(kv.second);
+  if (c->arena) TDCC_HIP(hipFree(c->arena));
+  c->arena = nullptr;
+  c->arena_bytes = 0;
+  TDCC_HIP(hipMalloc(&c->arena, bytes));
+  c->arena_bytes = bytes;
+}
+
+void bind(tdcc_ctx* c, Program& p) {
+  const bool stale = !p.bound || p.negA != c->rt.negA || p.dt6 != c->rt.dt6 || p.scale != c->rt.round_scale ||
+                     p.pol != c->rt.pol || p.table != c->table || p.mu_out != c->mu_out;
+  if (!stale) return;
+  invalidate_exec(p);
+  p.g->bind(c->arena);
+  p.bound = true;
+  p.negA = c->rt.negA;
+  p.dt6 = c->rt.dt6;
+  p.scale = c->rt.round_scale;
+  p.pol = c->rt.pol;
+  p.table = c->table;
+  p.mu_out = c->mu_out;
+}
+
+Program& finish_program(tdcc_ctx* c, const std::string& key, std::unique_ptr<Graph> g, bool step) {
+  g->plan();
+  Program& p = c->progs[key];
+  p.g = std::move(g);
+  p.step = step;
+  ensure_arena(c, p.g->arena_bytes());
+  return p;
+}
+
+int build_rhs_node(CCBuilder& b, int kind, int F, int t1, int t2, int l1, int l2) {
+  switch (kind) {
+    case TDCC_RHS_T1: return b.T1eq_rhs(t1, t2, F);
+    case TDCC_RHS_T2: return b.T2eq_rhs(t1, t2, F);
+    case TDCC_RHS_L1: return b.lam_1eq_rhs(t1, t2, l1, l2, F);
+    case TDCC_RHS_L2: return b.lam2eq_rhs(t1, t2, l1, l2, F);
+    case TDCC_RHS_T2_CC2: return b.T2eq_rhs_CC2(t1, t2, F);
+    case TDCC_RHS_L1_CC2: return b.L1eq_rhs_CC2(t1, t2, l1, l2, F);
+    case TDCC_RHS_L2_CC2: return b.L2eq_rhs_CC2(t1, t2, l1, l2, F);
+    default: throw Error("unknown right-hand side kind " + std::to_string(kind));
+  }
+}
+
+Program& rhs_program(tdcc_ctx* c, int kind) {
+  const std::string key = "rhs:" + std::to_string(kind);
+  auto it = c->progs.find(key);
+  if (it != c->progs.end()) return it->second;
+  auto g = std::make_unique<Graph>(c->o, c->v, &c->rt);
+  const int tei = ext_tei(c, *g);
+  Shape sF;
+  sF.rank = 2;
+  sF.cls[0] = sF.cls[1] = Cls::N;
+  const int F = ext_planes(*g, c->io_F, sF, "F");
+  int y[4];
+  for (int k = 0; k < 4; ++k) y[k] = ext_planes(*g, c->io_in[k], shape_of_block(k), "in");
+  CCBuilder b(*g, tei);
+  const int r = build_rhs_node(b, kind, F, y[0], y[1], y[2], y[3]);
+  const int out_k = (kind == TDCC_RHS_T1 || kind == TDCC_RHS_L1 || kind == TDCC_RHS_L1_CC2) ? 0 : 1;
+  Planes out = c->io_out;
+  out.n = block_size(c, out_k);
+  // io_out holds the largest block; view it with the residual's shape
+  Dev d;
+  d.re = c->io_out.re();
+  d.im = c->io_out.re() + out.n;
+  const Shape so = shape_of_block(out_k);
+  int64_t acc = 1;
+  for (int k = so.rank - 1; k >= 0; --k) {
+    d.st[k] = acc;
+    acc *= g->extent(so.cls[k]);
+  }
+  const int dst = g->external(so, true, d, "out");
+  g->assign(dst, r);
+  return finish_program(c, key, std::move(g), false);
+}
+
+Program& dipole_program(tdcc_ctx* c) {
+  const std::string key = "dipole";
+  auto it = c->progs.find(key);
+  if (it != c->progs.end()) return it->second;
+  auto g = std::make_unique<Graph>(c->o, c->v, &c->rt);
+  const int tei = ext_tei(c, *g);
+  int y[4];
+  for (int k = 0; k < 4; ++k) y[k] = ext_planes(*g, c->io_in[k], shape_of_block(k), "in");
+  CCBuilder b(*g, tei);
+  g->trace(b.Dij(y[0], y[1], y[2], y[3]), y[2], b.Dai(y[0], y[1], y[2], y[3]), b.Dab(y[0], y[1], y[2], y[3]));
+  return finish_program(c, key, std::move(g), false);
+}
+
+int build_intermediate(CCBuilder& b, const std::string& name, int F, int t1, int t2, int l1, int l2) {
+  if (name == "Fae") return b.Fae(t1, t2, F);
+  if (name == "Fmi") return b.Fmi(t1, t2, F);
+  if (name == "Fme") return b.Fme(t1, t2, F);
+  if (name == "Wmnij") return b.Wmnij(t1, t2, F);
+  if (name == "Wmnij_2") return b.Wmnij_2(t1, t2, F);
+  if (name == "Wabef") return b.Wabef(t1, t2, F);
+  if (name == "Wabef_2") return b.Wabef_2(t1, t2, F);
+  if (name == "Wmbej") return b.Wmbej(t1, t2, F);
+  if (name == "LRFea") return b.LRFea(t1, t2, F);
+  if (name == "LRFim") return b.LRFim(t1, t2, F);
+  if (name == "LSWieam") return b.LSWieam(t1, t2, F);
+  if (name == "LRWefam") return b.LRWefam(t1, t2, F);
+  if (name == "LRWibjm") return b.LRWibjm(t1, t2, F);
+  if (name == "LSWmnij") return b.LSWmnij(t1, t2, F);
+  if (name == "LSWabef") return b.LSWabef(t1, t2, F);
+  if (name == "Gfe") return b.Gfe(t2, l2);
+  if (name == "Gmn") return b.Gmn(t2, l2);
+  if (name == "LWfiea") return b.LWfiea(t1);
+  if (name == "LWmina") return b.LWmina(t1);
+  if (name == "Dij") return b.Dij(t1, t2, l1, l2);
+  if (name == "Dab") return b.Dab(t1, t2, l1, l2);
+  if (name == "Dai") return b.Dai(t1, t2, l1, l2);
+  throw Error("unknown intermediate " + name);
+}
+
+Program& intermediate_program(tdcc_ctx* c, const std::string& name, int64_t* out_len) {
+  const std::string key = "int:" + name;
+  auto g = std::make_unique<Graph>(c->o, c->v, &c->rt);
+  const int tei = ext_tei(c, *g);
+  Shape sF;
+  sF.rank = 2;
+  sF.cls[0] = sF.cls[1] = Cls::N;
+  const int F = ext_planes(*g, c->io_F, sF, "F");
+  int y[4];
+  for (int k = 0; k < 4; ++k) y[k] = ext_planes(*g, c->io_in[k], shape_of_block(k), "in");
+  CCBuilder b(*g, tei);
+  const int r = build_intermediate(b, name, F, y[0], y[1], y[2], y[3]);
+  const Shape so = g->node(r).shape;
+  const int64_t len = g->size(so);
+  if (c->int_out.n < len) {
+    TDCC_HIP(hipStreamSynchronize(c->stream));
+    free_planes(c->int_out);
+    for (auto& kv : c->progs)
+      if (kv.first.rfind("int:", 0) == 0) invalidate_exec(kv.second);
+    c->progs.erase(key);
+    alloc_planes(c->int_out, len);
+  }
+  Planes out = c->int_out;
+  out.n = len;
+  const int dst = ext_planes(*g, out, so, "out");
+  g->assign(dst, r);
+  *out_len = len;
+  auto it = c->progs.find(key);
+  if (it != c->progs.end()) {
+    // rebuilt graph replaces the cached one (the output buffer may have moved)
+    invalidate_exec(it->second);
+    c->progs.erase(it);
+  }
+  return finish_program(c, key, std::move(g), false);
+}
+
+std::string hexd(double x) {
+  char buf[64];
+  std::snprintf(buf, sizeof buf, "%a", x);
+  return buf;
+}
+
+Program& step_program(tdcc_ctx* c, int method, double dt) {
+  const std::string key = "step:" + std::to_string(method) + ":" + hexd(dt);
+  auto it = c->progs.find(key);
+  if (it != c->progs.end()) return it->second;
+  auto g = std::make_unique<Graph>(c->o, c->v, &c->rt);
+  StepIO io;
+  io.tei = ext_tei(c, *g);
+  for (int k = 0; k < 4; ++k) {
+    io.y[k] = ext_planes(*g, c->state[k], shape_of_block(k), "y");
+    io.base[k] = ext_planes(*g, c->base[k], shape_of_block(k), "base");
+  }
+  for (int k = 0; k < 4; ++k) io.ynew[k] = ext_planes(*g, c->state[k], shape_of_block(k), "ynew");
+  build_rk4_step(*g, io, method, dt);
+  return finish_program(c, key, std::move(g), true);
+}
+
+void run_program(tdcc_ctx* c, Program& p, bool use_graph) {
+  bind(c, p);
+  if (!use_graph) {
+    if (p.step) launch_counter_inc(c->counter, c->stream);
+    p.g->run(c->stream);
+    return;
+  }
+  if (!p.exec) {
+    TDCC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    try {
+      if (p.step) launch_counter_inc(c->counter, c->stream);
+      p.g->run(c->stream);
+    } catch (...) {
+      hipGraph_t tmp = nullptr;
+      (void)hipStreamEndCapture(c->stream, &tmp);
+      if (tmp) (void)hipGraphDestroy(tmp);
+      throw;
+    }
+    TDCC_HIP(hipStreamEndCapture(c->stream, &p.graph));
+    TDCC_HIP(hipGraphInstantiate(&p.exec, p.graph, nullptr, nullptr, 0));
+  }
+  TDCC_HIP(hipGraphLaunch(p.exec, c->stream));
+}
+
+bool graphs_enabled() {
+  const char* e = std::getenv("TDCC_NO_GRAPH");
+  return !(e && e[0] == '1');
+}
+
+void ensure_rows(tdcc_ctx* c, int64_t rows) {
+  if (rows <= c->table_cap && rows <= c->mu_cap) return;
+  TDCC_HIP(hipStreamSynchronize(c->stream));
+  const int64_t cap = std::max<int64_t>(rows, 1024);
+  if (c->table) TDCC_HIP(hipFree(c->table));
+  if (c->mu_out) TDCC_HIP(hipFree(c->mu_out));
+  c->table = c->mu_out = nullptr;
+  TDCC_HIP(hipMalloc(&c->table, cap * 6 * sizeof(double)));
+  TDCC_HIP(hipMalloc(&c->mu_out, cap * 6 * sizeof(double)));
+  c->table_cap = c->mu_cap = cap;
+  c->rt.field_table = c->table;
+  c->rt.mu_out = c->mu_out;
+}
+
+template <class F>
+int guarded(tdcc_ctx* c, F&& f) {
+  if (!c) return 1;
+  try {
+    TDCC_HIP(hipSetDevice(c->device));
+    f();
+    c->err.clear();
+    return 0;
+  } catch (const std::exception& e) {
+    c->err = e.what();
+    return 1;
+  }
+}
+
+void require_system(tdcc_ctx* c) {
+  if (!c->have_system || !c->have_tei) throw Error("system not set: call tdcc_set_system (and tdcc_synth_tei when tei was NULL)");
+}
+
+Program* prepare_run(tdcc_ctx* c, int method, int nsteps, double t0, double dt, double w0, double A, int pol,
+                     int precs, std::vector<double>& times) {
+  require_system(c);
+  if (method != TDCC_CCSD && method != TDCC_CC2) throw Error("method must be TDCC_CCSD or TDCC_CC2");
+  if (nsteps < 0) throw Error("nsteps must be >= 0");
+  if (pol < 0 || pol > 2) throw Error("pol must be 0, 1 or 2");
+  if (precs < 0 || precs > 22) throw Error("precs must be in [0, 22]");
+  if (!(dt > 0.0) || !std::isfinite(dt)) throw Error("dt must be positive");
+  if (nsteps == 0) return nullptr;
+  ensure_rows(c, nsteps);
+  // Field table, evaluated as the reference does: t advances by dt before
+  // the step; stage times t, t + dt/2, t + dt; exp(i * ((w0*2)*pi) * t).
+  std::vector<double> tab((size_t)nsteps * 6);
+  times.assign(nsteps, 0.0);
+  double t = t0;
+  const double wpi = (w0 * 2.0) * M_PI;
+  for (int i = 0; i < nsteps; ++i) {
+    t += dt;
+    times[i] = t;
+    const double ts[3] = {t, t + dt / 2.0, t + dt};
+    for (int s = 0; s < 3; ++s) {
+      const double th = wpi * ts[s];
+      tab[(size_t)i * 6 + 2 * s] = std::cos(th);
+      tab[(size_t)i * 6 + 2 * s + 1] = std::sin(th);
+    }
+  }
+  TDCC_HIP(hipMemcpyAsync(c->table, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  c->rt.negA = -A;
+  c->rt.pol = pol;
+  c->rt.dt6 = dt / 6.0;
+  double scale = 1.0;
+  for (int k = 0; k < precs; ++k) scale *= 10.0;
+  c->rt.round_scale = scale;
+  Program& p = step_program(c, method, dt);
+  launch_counter_set(c->counter, 0, c->stream);
+  return &p;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tdcc_create(int nocc, int nvir, int device, tdcc_ctx** out) {
+  if (!out) return 1;
+  *out = nullptr;
+  if (nocc <= 0 || nvir <= 0) return 1;
+  auto* c = new tdcc_ctx();
+  c->o = nocc;
+  c->v = nvir;
+  c->n = nocc + nvir;
+  c->device = device;
+  const int rc = guarded(c, [&] {
+    int count = 0;
+    TDCC_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) throw Error("no such HIP device");
+    TDCC_HIP(hipStreamCreate(&c->stream));
+    const int64_t n = c->n;
+    TDCC_HIP(hipMalloc(&c->F, n * n * sizeof(double)));
+    TDCC_HIP(hipMalloc(&c->mu, 3 * n * n * sizeof(double)));
+    TDCC_HIP(hipMalloc(&c->tei, n * n * n * n * sizeof(double)));
+    for (int k = 0; k < 4; ++k) {
+      alloc_planes(c->state[k], block_size(c, k));
+      alloc_planes(c->base[k], block_size(c, k));
+      alloc_planes(c->io_in[k], block_size(c, k));
+    }
+    const int64_t big = std::max<int64_t>(block_size(c, 1), n * n);
+    alloc_planes(c->io_out, big);
+    alloc_planes(c->io_F, n * n);
+    TDCC_HIP(hipMalloc(&c->staging, 2 * big * sizeof(double)));
+    TDCC_HIP(hipMalloc(&c->counter, sizeof(int64_t)));
+    launch_counter_set(c->counter, 0, c->stream);
+    ensure_rows(c, 1024);
+    c->rt.o = c->o;
+    c->rt.v = c->v;
+    c->rt.stream = c->stream;
+    c->rt.F = c->F;
+    c->rt.mu = c->mu;
+    c->rt.counter = c->counter;
+    TDCC_HIP(hipStreamSynchronize(c->stream));
+  });
+  if (rc) {
+    tdcc_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return 0;
+}
+
+void tdcc_destroy(tdcc_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& kv : c->progs) invalidate_exec(kv.second);
+  c->progs.clear();
+  for (int k = 0; k < 4; ++k) {
+    free_planes(c->state[k]);
+    free_planes(c->base[k]);
+    free_planes(c->backup[k]);
+    free_planes(c->io_in[k]);
+  }
+  free_planes(c->io_out);
+  free_planes(c->io_F);
+  free_planes(c->int_out);
+  for (double* p : {c->tei, c->F, c->mu, c->staging, c->table, c->mu_out, c->arena})
+    if (p) (void)hipFree(p);
+  if (c->counter) (void)hipFree(c->counter);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* tdcc_last_error(const tdcc_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int tdcc_set_system(tdcc_ctx* c, const double* tei, const double* fock, const double* mu) {
+  return guarded(c, [&] {
+    if (!fock || !mu) throw Error("fock and mu are required");
+    const int64_t n = c->n;
+    if (tei) {
+      TDCC_HIP(hipMemcpy(c->tei, tei, n * n * n * n * sizeof(double), hipMemcpyHostToDevice));
+      c->have_tei = true;
+    }
+    TDCC_HIP(hipMemcpy(c->F, fock, n * n * sizeof(double), hipMemcpyHostToDevice));
+    TDCC_HIP(hipMemcpy(c->mu, mu, 3 * n * n * sizeof(double), hipMemcpyHostToDevice));
+    c->have_system = true;
+  });
+}
+
+int tdcc_synth_tei(tdcc_ctx* c, uint32_t config_id, double sigma) {
+  return guarded(c, [&] {
+    launch_synth_tei(c->tei, c->n, config_id, sigma, c->stream);
+    TDCC_HIP(hipGetLastError());
+    TDCC_HIP(hipStreamSynchronize(c->stream));
+    c->have_tei = true;
+  });
+}
+
+int tdcc_set_state(tdcc_ctx* c, const double* t1, const double* t2, const double* l1, const double* l2) {
+  return guarded(c, [&] {
+    const double* h[4] = {t1, t2, l1, l2};
+    for (int k = 0; k < 4; ++k) {
+      if (!h[k]) throw Error("all four amplitude arrays are required");
+      upload_complex(c, h[k], c->state[k]);
+    }
+  });
+}
+
+int tdcc_get_state(tdcc_ctx* c, int which, double* t1, double* t2, double* l1, double* l2) {
+  return guarded(c, [&] {
+    if (which != 0 && which != 1) throw Error("which must be 0 (state) or 1 (rounded base)");
+    double* h[4] = {t1, t2, l1, l2};
+    for (int k = 0; k < 4; ++k)
+      if (h[k]) download_complex(c, which == 0 ? c->state[k] : c->base[k], h[k]);
+  });
+}
+
+int tdcc_rhs(tdcc_ctx* c, int kind, const double* F, const double* t1, const double* t2, const double* l1,
+             const double* l2, double* out) {
+  return guarded(c, [&] {
+    require_system(c);
+    if (!F || !t1 || !t2 || !out) throw Error("F, t1, t2 and out are required");
+    const bool lam = kind == TDCC_RHS_L1 || kind == TDCC_RHS_L2 || kind == TDCC_RHS_L1_CC2 || kind == TDCC_RHS_L2_CC2;
+    if (lam && (!l1 || !l2)) throw Error("lam1 and lam2 are required for a Lambda right-hand side");
+    Program& p = rhs_program(c, kind);
+    upload_complex(c, F, c->io_F);
+    const double* h[4] = {t1, t2, l1, l2};
+    for (int k = 0; k < 4; ++k)
+      if (h[k]) upload_complex(c, h[k], c->io_in[k]);
+    run_program(c, p, false);
+    TDCC_HIP(hipStreamSynchronize(c->stream));
+    const int out_k = (kind == TDCC_RHS_T1 || kind == TDCC_RHS_L1 || kind == TDCC_RHS_L1_CC2) ? 0 : 1;
+    Planes o = c->io_out;
+    o.n = block_size(c, out_k);
+    download_complex(c, o, out);
+  });
+}
+
+int tdcc_intermediate(tdcc_ctx* c, const char* name, const double* F, const double* t1, const double* t2,
+                      const double* l1, const double* l2, double* out, int64_t out_len) {
+  return guarded(c, [&] {
+    require_system(c);
+    if (!name || !F || !t1 || !t2 || !l1 || !l2 || !out) throw Error("all arguments are required");
+    int64_t len = 0;
+    Program& p = intermediate_program(c, name, &len);
+    if (out_len != len) throw Error("out_len does not match the intermediate's size " + std::to_string(len));
+    upload_complex(c, F, c->io_F);
+    const double* h[4] = {t1, t2, l1, l2};
+    for (int k = 0; k < 4; ++k) upload_complex(c, h[k], c->io_in[k]);
+    run_program(c, p, false);
+    TDCC_HIP(hipStreamSynchronize(c->stream));
+    Planes o = c->int_out;
+    o.n = len;
+    download_complex(c, o, out);
+  });
+}
+
+int tdcc_dipole(tdcc_ctx* c, const double* t1, const double* t2, const double* l1, const double* l2, double* mu_out) {
+  return guarded(c, [&] {
+    require_system(c);
+    if (!t1 || !t2 || !l1 || !l2 || !mu_out) throw Error("all arguments are required");
+    Program& p = dipole_program(c);
+    const double* h[4] = {t1, t2, l1, l2};
+    for (int k = 0; k < 4; ++k) upload_complex(c, h[k], c->io_in[k]);
+    launch_counter_set(c->counter, 1, c->stream);
+    run_program(c, p, false);
+    TDCC_HIP(hipMemcpyAsync(mu_out, c->mu_out, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    TDCC_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int tdcc_rk4_run(tdcc_ctx* c, int method, int nsteps, double t0, double dt, double w0, double A, int pol,
+                 int precs, double* mu_out, double* t_out) {
+  return guarded(c, [&] {
+    std::vector<double> times;
+    Program* p = prepare_run(c, method, nsteps, t0, dt, w0, A, pol, precs, times);
+    if (!p) return;
+    const bool use_graph = graphs_enabled();
+    for (int i = 0; i < nsteps; ++i) run_program(c, *p, use_graph);
+    TDCC_HIP(hipGetLastError());
+    if (mu_out)
+      TDCC_HIP(hipMemcpyAsync(mu_out, c->mu_out, (size_t)nsteps * 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    TDCC_HIP(hipStreamSynchronize(c->stream));
+    if (t_out) std::memcpy(t_out, times.data(), nsteps * sizeof(double));
+  });
+}
+
+int tdcc_profile_step(tdcc_ctx* c, int method, double t0, double dt, double w0, double A, int pol, int precs,
+                      char* report, int64_t report_len) {
+  return guarded(c, [&] {
+    if (!report || report_len <= 0) throw Error("report buffer required");
+    std::vector<double> times;
+    Program* p = prepare_run(c, method, 1, t0, dt, w0, A, pol, precs, times);
+    bind(c, *p);
+    launch_counter_inc(c->counter, c->stream);
+    std::vector<Graph::LaunchRecord> recs;
+    p->g->run_profiled(c->stream, recs);
+    std::string txt;
+    char line[256];
+    for (const auto& r : recs) {
+      std::snprintf(line, sizeof line, "%s\t%.17g\t%.17g\t%.9g\n", r.name.c_str(), r.flops, r.bytes, (double)r.ms);
+      txt += line;
+    }
+    if ((int64_t)txt.size() + 1 > report_len) throw Error("report buffer too small: need " + std::to_string(txt.size() + 1));
+    std::memcpy(report, txt.c_str(), txt.size() + 1);
+  });
+}
+
+int tdcc_checkpoint(tdcc_ctx* c, int op) {
+  return guarded(c, [&] {
+    if (op != 0 && op != 1) throw Error("op must be 0 (save) or 1 (restore)");
+    for (int k = 0; k < 4; ++k) {
+      if (!c->backup[k].d) {
+        if (op == 1) throw Error("no checkpoint saved");
+        alloc_planes(c->backup[k], c->state[k].n);
+      }
+      const Planes& src = op == 0 ? c->state[k] : c->backup[k];
+      const Planes& dst = op == 0 ? c->backup[k] : c->state[k];
+      TDCC_HIP(hipMemcpyAsync(dst.d, src.d, 2 * src.n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    }
+    TDCC_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int tdcc_step_info(tdcc_ctx* c, int method, double dt, int64_t* flops, int64_t* arena_bytes) {
+  return guarded(c, [&] {
+    Program& p = step_program(c, method, dt);
+    if (flops) *flops = p.g->contract_flops();
+    if (arena_bytes) *arena_bytes = p.g->arena_bytes();
+  });
+}
+
+}  // extern "C"

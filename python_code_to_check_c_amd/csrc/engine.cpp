@@ -1,0 +1,859 @@
+// Tensor-expression engine: construction with structural sharing, planning
+// (dead-node removal, liveness, arena layout, GETT offset tables) and lowering
+// to kernel launches.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <set>
+#include <sstream>
+
+namespace tdcc {
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    throw Error(std::string("HIP error: ") + hipGetErrorString(e) + " in " + what);
+  }
+}
+
+static std::string hexf(double x) {
+  char buf[64];
+  std::snprintf(buf, sizeof buf, "%a", x);
+  return buf;
+}
+
+Term tm(int src, double c) { return Term{src, {0, 1, 2, 3}, c, 0.0}; }
+
+Term tmp(int src, std::initializer_list<int> perm, double c) {
+  Term t{src, {0, 1, 2, 3}, c, 0.0};
+  int d = 0;
+  for (int p : perm) t.perm[d++] = p;
+  return t;
+}
+
+struct Graph::Launch {
+  enum Kind { GETT, LIN, FIELD, FIN, TRACE } kind;
+  int node = -1;
+  GettArgs g{};
+  bool ac = false, bc = false, akc = false, bkc = false, c_cplx = false;
+  int64_t tab[9] = {0, 0, 0, 0, 0, 0, -1, -1, -1};  // am ak bn bk cm cn ab bb cb offsets into tables
+  LincombArgs l{};
+  FieldArgs f{};
+  FinalizeArgs fin{};
+  TraceArgs tr{};
+};
+
+Graph::Graph(int o, int v, const Runtime* rt) : o_(o), v_(v), rt_(rt) {}
+
+Graph::~Graph() {
+  if (dev_tables_) (void)hipFree(dev_tables_);
+  if (ones_buf_) (void)hipFree(ones_buf_);
+}
+
+int64_t Graph::size(const Shape& s) const {
+  int64_t n = 1;
+  for (int d = 0; d < s.rank; ++d) n *= extent(s.cls[d]);
+  return n;
+}
+
+static void contiguous_strides(const Graph& g, const Shape& s, int64_t st[4]) {
+  int64_t acc = 1;
+  for (int d = s.rank - 1; d >= 0; --d) {
+    st[d] = acc;
+    acc *= g.extent(s.cls[d]);
+  }
+  for (int d = s.rank; d < 4; ++d) st[d] = 0;
+}
+
+int Graph::add(Node n) {
+  if (!n.key.empty()) {
+    auto it = cse_.find(n.key);
+    if (it != cse_.end()) return it->second;
+  }
+  const int id = (int)nodes_.size();
+  if (n.key.empty()) n.key = "#" + std::to_string(id);
+  cse_[n.key] = id;
+  nodes_.push_back(std::move(n));
+  return id;
+}
+
+int Graph::external(const Shape& s, bool cplx, const Dev& d, const std::string& name) {
+  Node n;
+  n.op = Op::External;
+  n.shape = s;
+  n.cplx = cplx;
+  n.ext = d;
+  n.key = "X|" + name + "|" + std::to_string(nodes_.size());
+  return add(std::move(n));
+}
+
+int Graph::view(int base, const Shape& s, int64_t off, const int64_t st[4]) {
+  Node n;
+  n.op = Op::View;
+  n.shape = s;
+  n.cplx = nodes_[base].cplx;
+  n.base = base;
+  n.voff = off;
+  std::ostringstream k;
+  k << "V|" << base << "|" << off;
+  for (int d = 0; d < 4; ++d) {
+    n.vst[d] = st[d];
+    k << "|" << st[d];
+  }
+  n.key = k.str();
+  return add(std::move(n));
+}
+
+static Cls cls_of(char c) {
+  if (c == 'o') return Cls::O;
+  if (c == 'v') return Cls::V;
+  throw Error(std::string("bad orbital class ") + c);
+}
+
+int Graph::block(int base, const char* classes) {
+  const int64_t n = o_ + v_;
+  const int64_t st[4] = {n * n * n, n * n, n, 1};
+  Shape s;
+  s.rank = 4;
+  int64_t off = 0;
+  for (int d = 0; d < 4; ++d) {
+    s.cls[d] = cls_of(classes[d]);
+    if (s.cls[d] == Cls::V) off += o_ * st[d];
+  }
+  return view(base, s, off, st);
+}
+
+int Graph::fblock(int base, const char* classes) {
+  const int64_t n = o_ + v_;
+  const int64_t st[4] = {n, 1, 0, 0};
+  Shape s;
+  s.rank = 2;
+  int64_t off = 0;
+  for (int d = 0; d < 2; ++d) {
+    s.cls[d] = cls_of(classes[d]);
+    if (s.cls[d] == Cls::V) off += o_ * st[d];
+  }
+  return view(base, s, off, st);
+}
+
+// ---------------------------------------------------------------------------
+static void parse_spec(const std::string& spec, std::vector<std::string>& ins, std::string& out) {
+  const auto arrow = spec.find("->");
+  if (arrow == std::string::npos) throw Error("einsum spec without '->': " + spec);
+  out = spec.substr(arrow + 2);
+  std::string lhs = spec.substr(0, arrow);
+  ins.clear();
+  std::string cur;
+  for (char c : lhs) {
+    if (c == ',') {
+      ins.push_back(cur);
+      cur.clear();
+    } else if (c != ' ') {
+      cur.push_back(c);
+    }
+  }
+  ins.push_back(cur);
+}
+
+static bool has(const std::string& s, char c) { return s.find(c) != std::string::npos; }
+
+int Graph::contract(const std::string& spec, int a, int b, double ar, double ai) {
+  std::vector<std::string> ins;
+  std::string lc;
+  parse_spec(spec, ins, lc);
+  if (ins.size() != 2) throw Error("contract needs two operands: " + spec);
+  std::string la = ins[0], lb = ins[1];
+  const Node& A = nodes_[a];
+  const Node& B = nodes_[b];
+  if ((int)la.size() != A.shape.rank || (int)lb.size() != B.shape.rank)
+    throw Error("rank mismatch in " + spec);
+  std::map<char, Cls> lcls;
+  auto note = [&](const std::string& ls, const Shape& s) {
+    for (int d = 0; d < (int)ls.size(); ++d) {
+      if (std::count(ls.begin(), ls.end(), ls[d]) != 1) throw Error("repeated index in operand: " + spec);
+      auto it = lcls.find(ls[d]);
+      if (it != lcls.end() && extent(it->second) != extent(s.cls[d]))
+        throw Error("extent mismatch for index in " + spec);
+      lcls[ls[d]] = s.cls[d];
+    }
+  };
+  note(la, A.shape);
+  note(lb, B.shape);
+  // Indices that occur in one operand only and not in the output are summed
+  // inside that operand first (contraction with a ones tensor).
+  auto pre_reduce = [&](int x, const std::string& lx, const std::string& ly) -> std::pair<int, std::string> {
+    std::string keep, drop;
+    for (char c : lx) (has(ly, c) || has(lc, c) ? keep : drop).push_back(c);
+    if (drop.empty()) return {x, lx};
+    Shape os;
+    os.rank = (int)drop.size();
+    for (int d = 0; d < os.rank; ++d) os.cls[d] = lcls[drop[d]];
+    const int ones = ones_view(os);
+    return {contract(lx + "," + drop + "->" + keep, x, ones), keep};
+  };
+  auto ra = pre_reduce(a, la, lb);
+  auto rb = pre_reduce(b, lb, la);
+  a = ra.first;
+  la = ra.second;
+  b = rb.first;
+  lb = rb.second;
+  const std::string nspec = la + "," + lb + "->" + lc;
+
+  Node n;
+  n.op = Op::Contract;
+  n.shape.rank = (int)lc.size();
+  for (int d = 0; d < n.shape.rank; ++d) {
+    if (!lcls.count(lc[d])) throw Error("output index not in inputs: " + spec);
+    n.shape.cls[d] = lcls[lc[d]];
+  }
+  n.cplx = nodes_[a].cplx || nodes_[b].cplx || ai != 0.0;
+  n.spec = nspec;
+  n.a = a;
+  n.b = b;
+  n.ar = ar;
+  n.ai = ai;
+  n.key = "C|" + nspec + "|" + std::to_string(a) + "|" + std::to_string(b) + "|" + hexf(ar) + "|" + hexf(ai);
+  return add(std::move(n));
+}
+
+int Graph::ones_view(const Shape& s) {
+  if (ones_id_ < 0) {
+    TDCC_HIP(hipMalloc(&ones_buf_, sizeof(double)));
+    const double one = 1.0;
+    TDCC_HIP(hipMemcpy(ones_buf_, &one, sizeof(double), hipMemcpyHostToDevice));
+    Shape s0;
+    Dev d;
+    d.re = ones_buf_;
+    ones_id_ = external(s0, false, d, "ones");
+  }
+  const int64_t st[4] = {0, 0, 0, 0};
+  return view(ones_id_, s, 0, st);
+}
+
+int Graph::contract3(const std::string& spec, int a, int b, int c, double ar) {
+  std::vector<std::string> ins;
+  std::string lo;
+  parse_spec(spec, ins, lo);
+  if (ins.size() != 3) throw Error("contract3 needs three operands: " + spec);
+  const int ids[3] = {a, b, c};
+  std::map<char, int64_t> ext;
+  for (int t = 0; t < 3; ++t)
+    for (int d = 0; d < (int)ins[t].size(); ++d) ext[ins[t][d]] = extent(nodes_[ids[t]].shape.cls[d]);
+  double best = -1;
+  int bx = 0, by = 1, bz = 2;
+  std::string bmid;
+  const int pairs[3][3] = {{0, 1, 2}, {0, 2, 1}, {1, 2, 0}};
+  for (auto& p : pairs) {
+    const std::string &x = ins[p[0]], &y = ins[p[1]], &z = ins[p[2]];
+    std::string uni = x;
+    for (char ch : y)
+      if (!has(uni, ch)) uni.push_back(ch);
+    std::string mid;
+    for (char ch : uni)
+      if (has(z, ch) || has(lo, ch)) mid.push_back(ch);
+    double c1 = 1, c2 = 1;
+    for (char ch : uni) c1 *= (double)ext[ch];
+    std::string uni2 = mid;
+    for (char ch : z)
+      if (!has(uni2, ch)) uni2.push_back(ch);
+    for (char ch : uni2) c2 *= (double)ext[ch];
+    if (best < 0 || c1 + c2 < best) {
+      best = c1 + c2;
+      bx = p[0];
+      by = p[1];
+      bz = p[2];
+      bmid = mid;
+    }
+  }
+  const int t = contract(ins[bx] + "," + ins[by] + "->" + bmid, ids[bx], ids[by]);
+  return contract(bmid + "," + ins[bz] + "->" + lo, t, ids[bz], ar);
+}
+
+int Graph::lincomb(std::vector<Term> terms) {
+  if (terms.empty()) throw Error("empty linear combination");
+  // Inline sources that are themselves linear combinations while the total
+  // stays within one kernel.
+  std::vector<Term> flat;
+  for (const Term& t : terms) {
+    const Node& s = nodes_[t.src];
+    if (s.op == Op::Lincomb && flat.size() + s.terms.size() + (terms.size() - 1) <= (size_t)kMaxTerms) {
+      for (const Term& u : s.terms) {
+        Term c;
+        c.src = u.src;
+        for (int d = 0; d < 4; ++d) c.perm[d] = u.perm[t.perm[d]];
+        c.cr = t.cr * u.cr - t.ci * u.ci;
+        c.ci = t.cr * u.ci + t.ci * u.cr;
+        flat.push_back(c);
+      }
+    } else {
+      flat.push_back(t);
+    }
+  }
+  Node n;
+  n.op = Op::Lincomb;
+  const Node& s0 = nodes_[flat[0].src];
+  n.shape.rank = s0.shape.rank;
+  for (int d = 0; d < n.shape.rank; ++d) n.shape.cls[d] = s0.shape.cls[flat[0].perm[d]];
+  std::ostringstream k;
+  k << "L";
+  for (const Term& t : flat) {
+    const Node& s = nodes_[t.src];
+    if (s.shape.rank != n.shape.rank) throw Error("rank mismatch in linear combination");
+    for (int d = 0; d < n.shape.rank; ++d)
+      if (extent(s.shape.cls[t.perm[d]]) != extent(n.shape.cls[d])) throw Error("shape mismatch in linear combination");
+    n.cplx = n.cplx || s.cplx || t.ci != 0.0;
+    k << "|" << t.src << ":" << t.perm[0] << t.perm[1] << t.perm[2] << t.perm[3] << ":" << hexf(t.cr) << ":" << hexf(t.ci);
+  }
+  n.terms = std::move(flat);
+  n.key = k.str();
+  return add(std::move(n));
+}
+
+int Graph::field(int stage) {
+  Node n;
+  n.op = Op::Field;
+  n.shape.rank = 2;
+  n.shape.cls[0] = n.shape.cls[1] = Cls::N;
+  n.cplx = true;
+  n.stage = stage;
+  n.key = "F|" + std::to_string(stage);
+  return add(std::move(n));
+}
+
+void Graph::finalize_rk(int y_ext, int base_ext, const int k[4], bool lambda) {
+  Node n;
+  n.op = Op::Finalize;
+  n.shape = nodes_[y_ext].shape;
+  n.a = y_ext;
+  n.b = base_ext;
+  for (int s = 0; s < 4; ++s) n.k[s] = k[s];
+  n.lambda_sign = lambda ? 1 : 0;
+  add(std::move(n));
+}
+
+void Graph::trace(int dij, int l1, int dai, int dab) {
+  Node n;
+  n.op = Op::Trace;
+  n.k[0] = dij;
+  n.k[1] = l1;
+  n.k[2] = dai;
+  n.k[3] = dab;
+  add(std::move(n));
+}
+
+void Graph::assign(int dst, int src) {
+  Node n;
+  n.op = Op::Assign;
+  n.shape = nodes_[dst].shape;
+  n.a = src;
+  n.b = dst;
+  add(std::move(n));
+}
+
+// ---------------------------------------------------------------------------
+static bool owns_buffer(Op op) { return op == Op::Contract || op == Op::Lincomb || op == Op::Field; }
+
+static void inputs_of(const Node& n, std::vector<int>& in) {
+  in.clear();
+  switch (n.op) {
+    case Op::Contract: in = {n.a, n.b}; break;
+    case Op::Lincomb:
+      for (const Term& t : n.terms) in.push_back(t.src);
+      break;
+    case Op::View: in = {n.base}; break;
+    case Op::Finalize: in = {n.k[0], n.k[1], n.k[2], n.k[3], n.a, n.b}; break;
+    case Op::Trace: in = {n.k[0], n.k[1], n.k[2], n.k[3]}; break;
+    case Op::Assign: in = {n.a, n.b}; break;
+    default: break;
+  }
+}
+
+void Graph::plan() {
+  const int N = (int)nodes_.size();
+  // dead-node removal: keep what side-effect nodes reach
+  std::vector<char> live(N, 0);
+  std::vector<int> in;
+  for (int i = N - 1; i >= 0; --i) {
+    const Op op = nodes_[i].op;
+    if (op == Op::Finalize || op == Op::Trace || op == Op::Assign) live[i] = 1;
+    if (!live[i]) continue;
+    inputs_of(nodes_[i], in);
+    for (int x : in) live[x] = 1;
+  }
+  live_ = live;
+  // liveness: uses of views count as uses of their base
+  auto root = [&](int x) {
+    while (nodes_[x].op == Op::View) x = nodes_[x].base;
+    return x;
+  };
+  for (auto& n : nodes_) n.last_use = -1;
+  for (int i = 0; i < N; ++i) {
+    if (!live[i] || nodes_[i].op == Op::View) continue;
+    inputs_of(nodes_[i], in);
+    for (int x : in) {
+      const int r = root(x);
+      nodes_[r].last_use = std::max(nodes_[r].last_use, i);
+    }
+  }
+  // arena layout: first fit over a coalescing free list
+  std::map<int64_t, int64_t> free_blocks;  // offset -> bytes
+  int64_t top = 0;
+  auto alloc = [&](int64_t bytes) -> int64_t {
+    bytes = (bytes + 255) & ~int64_t(255);
+    for (auto it = free_blocks.begin(); it != free_blocks.end(); ++it) {
+      if (it->second >= bytes) {
+        const int64_t off = it->first;
+        const int64_t rest = it->second - bytes;
+        free_blocks.erase(it);
+        if (rest > 0) free_blocks[off + bytes] = rest;
+        return off;
+      }
+    }
+    const int64_t off = top;
+    top += bytes;
+    return off;
+  };
+  auto release = [&](int64_t off, int64_t bytes) {
+    bytes = (bytes + 255) & ~int64_t(255);
+    auto it = free_blocks.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != free_blocks.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_blocks.erase(nx);
+    }
+    if (it != free_blocks.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_blocks.erase(it);
+      }
+    }
+  };
+  std::vector<std::vector<int>> ends(N);
+  for (int i = 0; i < N; ++i)
+    if (live[i] && owns_buffer(nodes_[i].op)) ends[std::max(nodes_[i].last_use, i)].push_back(i);
+  host_tables_.clear();
+  launches_.clear();
+  for (int i = 0; i < N; ++i) {
+    if (!live[i]) continue;
+    Node& n = nodes_[i];
+    if (owns_buffer(n.op)) n.arena_off = alloc(size(n.shape) * (n.cplx ? 2 : 1) * (int64_t)sizeof(double));
+    if (n.op == Op::Contract) {
+      lower_contract(i);
+      if (n.scratch_len > 0) {
+        n.scratch_off = alloc(n.scratch_len);
+        release(n.scratch_off, n.scratch_len);
+      }
+    }
+    for (int x : ends[i]) release(nodes_[x].arena_off, size(nodes_[x].shape) * (nodes_[x].cplx ? 2 : 1) * (int64_t)sizeof(double));
+  }
+  arena_bytes_ = top;
+  if (dev_tables_) {
+    (void)hipFree(dev_tables_);
+    dev_tables_ = nullptr;
+  }
+  if (!host_tables_.empty()) {
+    TDCC_HIP(hipMalloc(&dev_tables_, host_tables_.size() * sizeof(int64_t)));
+    TDCC_HIP(hipMemcpy(dev_tables_, host_tables_.data(), host_tables_.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  }
+}
+
+Dev Graph::resolve(int id) const {
+  const Node& n = nodes_[id];
+  Dev d;
+  switch (n.op) {
+    case Op::External: return n.ext;
+    case Op::View: {
+      Dev b = resolve(n.base);
+      d.re = b.re + n.voff;
+      d.im = b.im ? b.im + n.voff : nullptr;
+      for (int k = 0; k < 4; ++k) d.st[k] = n.vst[k];
+      return d;
+    }
+    default: {
+      if (n.arena_off < 0) throw Error("node has no buffer");
+      d.re = arena_ + n.arena_off / (int64_t)sizeof(double);
+      d.im = n.cplx ? d.re + size(n.shape) : nullptr;
+      contiguous_strides(*this, n.shape, d.st);
+      return d;
+    }
+  }
+}
+
+// Offset tables of one index group: row-major enumeration of `letters` with
+// the given strides.
+static void group_table(const std::string& letters, const std::map<char, int64_t>& ext,
+                        const std::map<char, int64_t>& st, std::vector<int64_t>& out) {
+  int64_t total = 1;
+  for (char c : letters) total *= ext.at(c);
+  out.assign(total, 0);
+  for (int64_t idx = 0; idx < total; ++idx) {
+    int64_t rem = idx, off = 0;
+    for (int d = (int)letters.size() - 1; d >= 0; --d) {
+      const int64_t e = ext.at(letters[d]);
+      const int64_t i = rem % e;
+      rem /= e;
+      off += i * st.at(letters[d]);
+    }
+    out[idx] = off;
+  }
+}
+
+void Graph::lower_contract(int id) {
+  Node& n = nodes_[id];
+  std::vector<std::string> ins;
+  std::string lc;
+  parse_spec(n.spec, ins, lc);
+  std::string la = ins[0], lb = ins[1];
+  int a = n.a, b = n.b;
+  // strides are known before binding (external/view strides are fixed,
+  // owned buffers are contiguous)
+  auto strides_of = [&](int x, int64_t st[4]) {
+    const Node& m = nodes_[x];
+    if (m.op == Op::External) {
+      for (int d = 0; d < 4; ++d) st[d] = m.ext.st[d];
+    } else if (m.op == Op::View) {
+      for (int d = 0; d < 4; ++d) st[d] = m.vst[d];
+    } else {
+      contiguous_strides(*this, m.shape, st);
+    }
+  };
+  int64_t sa4[4], sb4[4], sc4[4];
+  strides_of(a, sa4);
+  strides_of(b, sb4);
+  contiguous_strides(*this, n.shape, sc4);
+  std::map<char, int64_t> ext, sa, sb, sc;
+  for (int d = 0; d < (int)la.size(); ++d) {
+    ext[la[d]] = extent(nodes_[a].shape.cls[d]);
+    sa[la[d]] = sa4[d];
+  }
+  for (int d = 0; d < (int)lb.size(); ++d) {
+    ext[lb[d]] = extent(nodes_[b].shape.cls[d]);
+    sb[lb[d]] = sb4[d];
+  }
+  for (int d = 0; d < (int)lc.size(); ++d) sc[lc[d]] = sc4[d];
+  // put the output's fastest index on the N side when it is not a batch index
+  if (!lc.empty()) {
+    const char inner = lc.back();
+    if (has(la, inner) && !has(lb, inner)) {
+      std::swap(la, lb);
+      std::swap(a, b);
+      std::swap(sa, sb);
+    }
+  }
+  std::string gb, gm, gn, gk;
+  for (char c : lc) {
+    if (has(la, c) && has(lb, c)) gb.push_back(c);
+    else if (has(la, c)) gm.push_back(c);
+    else gn.push_back(c);
+  }
+  for (char c : la)
+    if (has(lb, c) && !has(lc, c)) gk.push_back(c);
+  auto innermost = [&](const std::string& ls, const std::map<char, int64_t>& st) -> char {
+    char best = 0;
+    int64_t bs = -1;
+    for (char c : ls) {
+      const int64_t s = st.at(c);
+      if (s > 0 && (bs < 0 || s < bs)) {
+        bs = s;
+        best = c;
+      }
+    }
+    return best;
+  };
+  const char ia = innermost(la, sa), ib = innermost(lb, sb);
+  const bool akc = ia && has(gk, ia);
+  const bool bkc = ib && has(gk, ib);
+  auto by_stride = [&](std::string g, const std::map<char, int64_t>& st) {
+    std::stable_sort(g.begin(), g.end(), [&](char x, char y) { return st.at(x) > st.at(y); });
+    return g;
+  };
+  gm = akc ? gm : by_stride(gm, sa);
+  gn = bkc ? gn : by_stride(gn, sb);
+  gk = akc ? by_stride(gk, sa) : (bkc ? by_stride(gk, sb) : by_stride(gk, sa));
+
+  auto push = [&](const std::vector<int64_t>& t) {
+    const int64_t off = (int64_t)host_tables_.size();
+    host_tables_.insert(host_tables_.end(), t.begin(), t.end());
+    return off;
+  };
+  std::vector<int64_t> t;
+  auto L = std::make_unique<Launch>();
+  L->kind = Launch::GETT;
+  group_table(gm, ext, sa, t);
+  L->tab[0] = push(t);
+  const int64_t M = (int64_t)t.size();
+  group_table(gk, ext, sa, t);
+  L->tab[1] = push(t);
+  const int64_t K = (int64_t)t.size();
+  group_table(gn, ext, sb, t);
+  L->tab[2] = push(t);
+  const int64_t Nn = (int64_t)t.size();
+  group_table(gk, ext, sb, t);
+  L->tab[3] = push(t);
+  group_table(gm, ext, sc, t);
+  L->tab[4] = push(t);
+  group_table(gn, ext, sc, t);
+  L->tab[5] = push(t);
+  int64_t Bt = 1;
+  if (!gb.empty()) {
+    group_table(gb, ext, sa, t);
+    L->tab[6] = push(t);
+    Bt = (int64_t)t.size();
+    group_table(gb, ext, sb, t);
+    L->tab[7] = push(t);
+    group_table(gb, ext, sc, t);
+    L->tab[8] = push(t);
+  }
+  if (M > INT32_MAX || Nn > INT32_MAX || K > INT32_MAX || Bt > 65535) throw Error("contraction too large: " + n.spec);
+  GettArgs& g = L->g;
+  g.M = (int)M;
+  g.N = (int)Nn;
+  g.K = (int)K;
+  g.batch = (int)Bt;
+  g.tiles_m = (int)((M + 63) / 64);
+  g.tiles_n = (int)((Nn + 63) / 64);
+  g.alpha_re = n.ar;
+  g.alpha_im = n.ai;
+  g.accumulate = 0;
+  const int64_t tiles = (int64_t)g.tiles_m * g.tiles_n * Bt;
+  int splits = 1;
+  if (tiles < 256 && K >= 512) {
+    splits = (int)std::min<int64_t>((K + 255) / 256, std::max<int64_t>(1, 1024 / tiles));
+  }
+  int64_t chunk = (K + splits - 1) / splits;
+  chunk = (chunk + 15) / 16 * 16;
+  if (chunk < 16) chunk = 16;
+  splits = (int)((K + chunk - 1) / chunk);
+  if (splits < 1) splits = 1;
+  g.splits = splits;
+  g.k_chunk = (int)chunk;
+  L->ac = nodes_[a].cplx;
+  L->bc = nodes_[b].cplx;
+  L->akc = akc;
+  L->bkc = bkc;
+  L->c_cplx = n.cplx;
+  n.a = a;  // orientation used by the launch
+  n.b = b;
+  n.scratch_len = splits > 1 ? (int64_t)splits * Bt * M * Nn * (L->ac || L->bc ? 2 : 1) * (int64_t)sizeof(double) : 0;
+  L->node = id;
+  launches_.push_back(std::move(L));
+}
+
+void Graph::bind(double* arena) {
+  arena_ = arena;
+  // rebuild every launch record with device pointers
+  std::vector<std::unique_ptr<Launch>> gett;
+  for (auto& L : launches_)
+    if (L->kind == Launch::GETT) gett.push_back(std::move(L));
+  launches_.clear();
+  size_t gi = 0;
+  const int N = (int)nodes_.size();
+  const int64_t n2 = (int64_t)(o_ + v_) * (o_ + v_);
+  for (int i = 0; i < N; ++i) {
+    if (!live_[i]) continue;
+    const Node& n = nodes_[i];
+    switch (n.op) {
+      case Op::Contract: {
+        auto L = std::move(gett.at(gi++));
+        if (L->node != i) throw Error("internal: contraction launch order");
+        GettArgs& g = L->g;
+        const Dev A = resolve(n.a), B = resolve(n.b), C = resolve(i);
+        g.a_re = A.re;
+        g.a_im = L->ac ? A.im : nullptr;
+        g.b_re = B.re;
+        g.b_im = L->bc ? B.im : nullptr;
+        g.c_re = C.re;
+        g.c_im = n.cplx ? C.im : nullptr;
+        g.am = dev_tables_ + L->tab[0];
+        g.ak = dev_tables_ + L->tab[1];
+        g.bn = dev_tables_ + L->tab[2];
+        g.bk = dev_tables_ + L->tab[3];
+        g.cm = dev_tables_ + L->tab[4];
+        g.cn = dev_tables_ + L->tab[5];
+        g.ab = L->tab[6] >= 0 ? dev_tables_ + L->tab[6] : nullptr;
+        g.bb = L->tab[7] >= 0 ? dev_tables_ + L->tab[7] : nullptr;
+        g.cb = L->tab[8] >= 0 ? dev_tables_ + L->tab[8] : nullptr;
+        g.partial = n.scratch_len > 0 ? arena_ + n.scratch_off / (int64_t)sizeof(double) : nullptr;
+        launches_.push_back(std::move(L));
+        break;
+      }
+      case Op::Lincomb:
+      case Op::Assign: {
+        const Dev O = n.op == Op::Assign ? resolve(n.b) : resolve(i);
+        std::vector<Term> terms = n.op == Op::Assign ? std::vector<Term>{tm(n.a)} : n.terms;
+        const bool out_cplx = n.op == Op::Assign ? nodes_[n.b].cplx : n.cplx;
+        for (size_t t0 = 0; t0 < terms.size(); t0 += kMaxTerms) {
+          auto L = std::make_unique<Launch>();
+          L->kind = Launch::LIN;
+          LincombArgs& l = L->l;
+          l.out_re = O.re;
+          l.out_im = out_cplx ? O.im : nullptr;
+          l.rank = n.shape.rank;
+          l.total = size(n.shape);
+          for (int d = 0; d < 4; ++d) {
+            l.ost[d] = O.st[d];
+            l.dims[d] = d < n.shape.rank ? extent(n.shape.cls[d]) : 1;
+          }
+          l.accumulate = t0 > 0;
+          l.nterms = (int)std::min<size_t>(kMaxTerms, terms.size() - t0);
+          for (int k = 0; k < l.nterms; ++k) {
+            const Term& T = terms[t0 + k];
+            const Dev S = resolve(T.src);
+            l.t[k].re = S.re;
+            l.t[k].im = nodes_[T.src].cplx ? S.im : nullptr;
+            for (int d = 0; d < 4; ++d) l.t[k].st[d] = d < n.shape.rank ? S.st[T.perm[d]] : 0;
+            l.t[k].cr = T.cr;
+            l.t[k].ci = T.ci;
+          }
+          launches_.push_back(std::move(L));
+        }
+        break;
+      }
+      case Op::Field: {
+        auto L = std::make_unique<Launch>();
+        L->kind = Launch::FIELD;
+        const Dev O = resolve(i);
+        L->f.F = rt_->F;
+        L->f.mu_pol = rt_->mu + (int64_t)rt_->pol * n2;
+        L->f.out_re = O.re;
+        L->f.out_im = O.im;
+        L->f.n2 = n2;
+        L->f.negA = rt_->negA;
+        L->f.table = rt_->field_table;
+        L->f.counter = rt_->counter;
+        L->f.stage = n.stage;
+        launches_.push_back(std::move(L));
+        break;
+      }
+      case Op::Finalize: {
+        auto L = std::make_unique<Launch>();
+        L->kind = Launch::FIN;
+        const Dev Y = resolve(n.a);
+        const Dev Bs = resolve(n.b);
+        L->fin.y_re = Y.re;
+        L->fin.y_im = Y.im;
+        L->fin.base_re = Bs.re;
+        L->fin.base_im = Bs.im;
+        for (int s = 0; s < 4; ++s) {
+          const Dev K = resolve(n.k[s]);
+          L->fin.k_re[s] = K.re;
+          L->fin.k_im[s] = K.im;
+          if (!nodes_[n.k[s]].cplx) throw Error("internal: real stage derivative");
+        }
+        L->fin.n = size(n.shape);
+        L->fin.dt6 = rt_->dt6;
+        L->fin.scale = rt_->round_scale;
+        L->fin.lambda_sign = n.lambda_sign;
+        launches_.push_back(std::move(L));
+        break;
+      }
+      case Op::Trace: {
+        auto L = std::make_unique<Launch>();
+        L->kind = Launch::TRACE;
+        const Dev d0 = resolve(n.k[0]), d1 = resolve(n.k[1]), d2 = resolve(n.k[2]), d3 = resolve(n.k[3]);
+        L->tr.dij_re = d0.re;
+        L->tr.dij_im = d0.im;
+        L->tr.l1_re = d1.re;
+        L->tr.l1_im = d1.im;
+        L->tr.dai_re = d2.re;
+        L->tr.dai_im = d2.im;
+        L->tr.dab_re = d3.re;
+        L->tr.dab_im = d3.im;
+        L->tr.mu = rt_->mu;
+        L->tr.o = o_;
+        L->tr.v = v_;
+        L->tr.out = rt_->mu_out;
+        L->tr.counter = rt_->counter;
+        launches_.push_back(std::move(L));
+        break;
+      }
+      default: break;
+    }
+  }
+}
+
+void Graph::run(hipStream_t s) const {
+  for (const auto& L : launches_) {
+    switch (L->kind) {
+      case Launch::GETT:
+        if (L->g.M == 0 || L->g.N == 0) break;
+        launch_gett(L->g, L->ac, L->bc, L->akc, L->bkc, s);
+        if (L->g.splits > 1) launch_gett_reduce(L->g, L->c_cplx, s);
+        break;
+      case Launch::LIN: launch_lincomb(L->l, s); break;
+      case Launch::FIELD: launch_field(L->f, s); break;
+      case Launch::FIN: launch_finalize(L->fin, s); break;
+      case Launch::TRACE: launch_trace(L->tr, s); break;
+    }
+  }
+  TDCC_HIP(hipGetLastError());
+}
+
+void Graph::run_profiled(hipStream_t s, std::vector<LaunchRecord>& out) const {
+  std::vector<hipEvent_t> ev(2 * launches_.size() + 4);
+  for (auto& e : ev) TDCC_HIP(hipEventCreate(&e));
+  std::vector<LaunchRecord> recs;
+  size_t k = 0;
+  auto timed = [&](const std::string& name, double flops, double bytes, auto&& fn) {
+    TDCC_HIP(hipEventRecord(ev[k], s));
+    fn();
+    TDCC_HIP(hipEventRecord(ev[k + 1], s));
+    recs.push_back({name, flops, bytes, 0.f});
+    k += 2;
+  };
+  try {
+    for (const auto& L : launches_) {
+      switch (L->kind) {
+        case Launch::GETT: {
+          if (L->g.M == 0 || L->g.N == 0) break;
+          const double mul = (L->ac && L->bc) ? 8 : ((L->ac || L->bc) ? 4 : 2);
+          const double fl = mul * L->g.M * (double)L->g.N * L->g.K * L->g.batch;
+          char nm[96];
+          std::snprintf(nm, sizeof nm, "gett_kernel<%d,%d,%d,%d>%s", (int)L->ac, (int)L->bc, (int)L->akc, (int)L->bkc,
+                        L->g.splits > 1 ? "+splitk" : "");
+          timed(nm, fl, 0.0, [&] {
+            launch_gett(L->g, L->ac, L->bc, L->akc, L->bkc, s);
+            if (L->g.splits > 1) launch_gett_reduce(L->g, L->c_cplx, s);
+          });
+          break;
+        }
+        case Launch::LIN: {
+          const double bytes = 8.0 * L->l.total * ((L->l.out_im ? 2 : 1) * (1 + L->l.accumulate) + 2 * L->l.nterms);
+          timed("lincomb_kernel", 0.0, bytes, [&] { launch_lincomb(L->l, s); });
+          break;
+        }
+        case Launch::FIELD:
+          timed("field_kernel", 0.0, 8.0 * 4 * L->f.n2, [&] { launch_field(L->f, s); });
+          break;
+        case Launch::FIN:
+          timed("finalize_kernel", 0.0, 16.0 * 7 * L->fin.n, [&] { launch_finalize(L->fin, s); });
+          break;
+        case Launch::TRACE:
+          timed("trace_kernel", 0.0, 0.0, [&] { launch_trace(L->tr, s); });
+          break;
+      }
+    }
+    TDCC_HIP(hipGetLastError());
+    TDCC_HIP(hipStreamSynchronize(s));
+    for (size_t i = 0; i < recs.size(); ++i) TDCC_HIP(hipEventElapsedTime(&recs[i].ms, ev[2 * i], ev[2 * i + 1]));
+  } catch (...) {
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    throw;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  out.insert(out.end(), recs.begin(), recs.end());
+}
+
+int64_t Graph::contract_flops() const {
+  int64_t f = 0;
+  for (const auto& L : launches_) {
+    if (L->kind != Launch::GETT) continue;
+    const int64_t mul = (L->ac && L->bc) ? 8 : ((L->ac || L->bc) ? 4 : 2);
+    f += mul * (int64_t)L->g.M * L->g.N * L->g.K * L->g.batch;
+  }
+  return f;
+}
+
+}  // namespace tdcc

@@ -1,0 +1,169 @@
+// Tensor-expression engine: the residual builders describe each reference
+// einsum as a node; identical nodes are shared (which hoists everything that
+// the four RK4 chains and their stages have in common); the node list is then
+// planned into one device arena by liveness and lowered to kernel launches.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace tdcc {
+
+struct Error : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char* what);
+#define TDCC_HIP(x) ::tdcc::hip_check((x), #x)
+
+enum class Cls : uint8_t { O = 0, V = 1, N = 2 };  // occupied, virtual, all orbitals
+
+struct Shape {
+  int rank = 0;
+  Cls cls[4] = {Cls::O, Cls::O, Cls::O, Cls::O};
+};
+
+struct Dev {  // device data of a tensor: planes + element strides
+  double* re = nullptr;
+  double* im = nullptr;
+  int64_t st[4] = {0, 0, 0, 0};
+};
+
+enum class Op { External, View, Contract, Lincomb, Field, Finalize, Trace, Assign };
+
+struct Term {
+  int src;
+  int perm[4];  // output axis d reads source axis perm[d]
+  double cr, ci;
+};
+
+struct Node {
+  Op op;
+  Shape shape;
+  bool cplx = false;
+  std::string key;
+  // Contract
+  std::string spec;
+  int a = -1, b = -1;
+  double ar = 1.0, ai = 0.0;
+  // Lincomb
+  std::vector<Term> terms;
+  // View
+  int base = -1;
+  int64_t voff = 0;
+  int64_t vst[4] = {0, 0, 0, 0};
+  // Field
+  int stage = -1;
+  // External
+  Dev ext;
+  // Finalize: inputs k[4], target external `a`, base external `b`
+  int k[4] = {-1, -1, -1, -1};
+  int lambda_sign = 0;
+  // Trace: inputs dij, l1, dai, dab in k[0..3]
+  // Assign: src in a, destination external in b
+  // planning
+  int last_use = -1;
+  int64_t arena_off = -1;
+  int64_t scratch_off = -1;  // split-K partials
+  int64_t scratch_len = 0;
+};
+
+struct Handle {
+  int id = -1;
+};
+
+class Graph;
+
+// Runtime parameters shared by a graph's kernels (owned by the context).
+struct Runtime {
+  int o = 0, v = 0;
+  hipStream_t stream = nullptr;
+  const double* F = nullptr;        // (n,n) real Fock
+  const double* mu = nullptr;       // (3,n,n) real dipoles
+  int pol = 2;
+  double negA = 0.0;
+  const double* field_table = nullptr;  // [steps][3][2]
+  int64_t* counter = nullptr;
+  double* mu_out = nullptr;         // [steps][3][2]
+  double dt6 = 0.0, round_scale = 1e15;
+};
+
+class Graph {
+ public:
+  Graph(int o, int v, const Runtime* rt);
+  ~Graph();
+  Graph(const Graph&) = delete;
+  Graph& operator=(const Graph&) = delete;
+
+  int64_t extent(Cls c) const { return c == Cls::O ? o_ : (c == Cls::V ? v_ : o_ + v_); }
+  int64_t size(const Shape& s) const;
+  const Node& node(int id) const { return nodes_[id]; }
+  int count() const { return (int)nodes_.size(); }
+
+  // ---- construction ----
+  int external(const Shape& s, bool cplx, const Dev& d, const std::string& name);
+  int view(int base, const Shape& s, int64_t off, const int64_t st[4]);
+  int block(int base_n4, const char* classes);       // TEI block, e.g. "ovvv"
+  int fblock(int base_nn, const char* classes);      // (n,n) block, e.g. "ov"
+  int contract(const std::string& spec, int a, int b, double ar = 1.0, double ai = 0.0);
+  int contract3(const std::string& spec, int a, int b, int c, double ar = 1.0);
+  int lincomb(std::vector<Term> terms);
+  int field(int stage);
+  int ones_view(const Shape& s);                     // all-ones tensor (stride 0)
+  void finalize_rk(int y_ext, int base_ext, const int k[4], bool lambda);
+  void trace(int dij, int l1, int dai, int dab);
+  void assign(int dst_ext, int src);
+
+  // ---- lowering ----
+  void plan();                  // liveness + arena layout + offset tables
+  int64_t arena_bytes() const { return arena_bytes_; }
+  void bind(double* arena);     // resolve device pointers, build launch records
+  void run(hipStream_t s) const;
+
+  // One eager pass with a HIP event pair around every kernel launch.
+  struct LaunchRecord {
+    std::string name;   // kernel name incl. template variant
+    double flops;       // algorithmic real flops of the launch (contractions)
+    double bytes;       // algorithmic bytes moved (elementwise kernels)
+    float ms;
+  };
+  void run_profiled(hipStream_t s, std::vector<LaunchRecord>& out) const;
+
+  // statistics
+  int64_t contract_flops() const;  // executed real flops of all contractions
+
+ private:
+  int add(Node n);
+  Dev resolve(int id) const;
+  void lower_contract(int id);
+  void lower_lincomb(int id);
+
+  int o_, v_;
+  const Runtime* rt_;
+  std::vector<Node> nodes_;
+  std::map<std::string, int> cse_;
+  int64_t arena_bytes_ = 0;
+  double* arena_ = nullptr;
+
+  struct Launch;
+  std::vector<std::unique_ptr<Launch>> launches_;
+  std::vector<int64_t> host_tables_;
+  int64_t* dev_tables_ = nullptr;
+  std::vector<char> live_;
+  int ones_id_ = -1;
+  double* ones_buf_ = nullptr;
+};
+
+// Elementwise helpers for the builders.
+Term tm(int src, double c = 1.0);                           // identity permutation
+Term tmp(int src, std::initializer_list<int> perm, double c = 1.0);
+
+}  // namespace tdcc

@@ -1,0 +1,541 @@
+// Device kernels for the TDCC propagation library (gfx950 / CDNA4).
+#include "kernels.hpp"
+
+#include <cstdio>
+
+namespace tdcc {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// ===========================================================================
+// GETT: strided tensor contraction on v_mfma_f64_16x16x4_f64.
+//
+// Workgroup tile 64x64 (M x N), K step 16, 256 threads = 4 waves in a 2x2
+// arrangement; each wave owns a 32x32 sub-tile = 2x2 MFMA blocks of 16x16.
+// Complex operands are FP64 planes; a complex product uses 4 MFMAs
+// (re*re - im*im, re*im + im*re), real x complex uses 2.
+//
+// MFMA 16x16x4 f64 operand layout (lane l of the wave):
+//   A[i = l%16][k = l/16], B[k = l/16][j = l%16],
+//   D[i = 4*(l/16) + r][j = l%16] for the four accumulator elements r.
+// ===========================================================================
+constexpr int BM = 64, BN = 64, BKK = 16, NT = 256;
+constexpr int LDA = BM + 1, LDB = BN + 1;
+
+__device__ __forceinline__ int remap_tile(int bid, int tiles) {
+  // Workgroups are dispatched round-robin over the 8 XCDs (bid % 8).  Give
+  // every XCD a contiguous range of tiles so that tiles sharing a B panel
+  // (same tn, consecutive tm) run on one XCD and share its L2.
+  const int q = tiles >> 3, r = tiles & 7;
+  const int x = bid & 7, loc = bid >> 3;
+  return x * q + (x < r ? x : r) + loc;
+}
+
+template <bool AC, bool BC, bool AKC, bool BKC>
+__global__ __launch_bounds__(NT) void gett_kernel(GettArgs g) {
+  constexpr int PA = AC ? 2 : 1;
+  constexpr int PB = BC ? 2 : 1;
+  __shared__ double As[2][PA][BKK][LDA];
+  __shared__ double Bs[2][PB][BKK][LDB];
+
+  const int tiles = g.tiles_m * g.tiles_n;
+  const int L = remap_tile(blockIdx.x, tiles);
+  const int tm = L % g.tiles_m, tn = L / g.tiles_m;
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int split = blockIdx.y, bz = blockIdx.z;
+  const int kb = split * g.k_chunk;
+  const int ke = min(g.K, kb + g.k_chunk);
+
+  const int64_t aoff = g.ab ? g.ab[bz] : 0;
+  const int64_t boff = g.bb ? g.bb[bz] : 0;
+  const double* __restrict__ a_re = g.a_re + aoff;
+  const double* __restrict__ a_im = AC ? g.a_im + aoff : nullptr;
+  const double* __restrict__ b_re = g.b_re + boff;
+  const double* __restrict__ b_im = BC ? g.b_im + boff : nullptr;
+
+  const int tid = threadIdx.x;
+
+  // ---- loader state -------------------------------------------------------
+  // A tile element (mm, kk); fixed per-thread m offsets, per-step k offsets.
+  int64_t a_mo[4];
+  int a_lm[4], a_lk[4];
+  if (AKC) {
+    const int kk = tid & 15, mr = tid >> 4;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const int m = m0 + mr + 16 * p;
+      a_mo[p] = (m < g.M) ? g.am[m] : -1;
+      a_lm[p] = mr + 16 * p;
+      a_lk[p] = kk;
+    }
+  } else {
+    const int mm = tid & 63, kr = tid >> 6;
+    const int m = m0 + mm;
+    const int64_t mo = (m < g.M) ? g.am[m] : -1;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      a_mo[p] = mo;
+      a_lm[p] = mm;
+      a_lk[p] = kr + 4 * p;
+    }
+  }
+  int64_t b_no[4];
+  int b_ln[4], b_lk[4];
+  if (BKC) {
+    const int kk = tid & 15, nr = tid >> 4;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const int n = n0 + nr + 16 * p;
+      b_no[p] = (n < g.N) ? g.bn[n] : -1;
+      b_ln[p] = nr + 16 * p;
+      b_lk[p] = kk;
+    }
+  } else {
+    const int nn = tid & 63, kr = tid >> 6;
+    const int n = n0 + nn;
+    const int64_t no = (n < g.N) ? g.bn[n] : -1;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      b_no[p] = no;
+      b_ln[p] = nn;
+      b_lk[p] = kr + 4 * p;
+    }
+  }
+
+  double ra[PA][4], rb[PB][4];
+
+  auto load_global = [&](int k0) {
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const int k = k0 + a_lk[p];
+      const bool ok = (a_mo[p] >= 0) && (k < ke);
+      const int64_t off = ok ? a_mo[p] + g.ak[k] : 0;
+      ra[0][p] = ok ? a_re[off] : 0.0;
+      if (AC) ra[PA - 1][p] = ok ? a_im[off] : 0.0;
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const int k = k0 + b_lk[p];
+      const bool ok = (b_no[p] >= 0) && (k < ke);
+      const int64_t off = ok ? b_no[p] + g.bk[k] : 0;
+      rb[0][p] = ok ? b_re[off] : 0.0;
+      if (BC) rb[PB - 1][p] = ok ? b_im[off] : 0.0;
+    }
+  };
+  auto store_lds = [&](int buf) {
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+#pragma unroll
+      for (int q = 0; q < PA; ++q) As[buf][q][a_lk[p]][a_lm[p]] = ra[q][p];
+#pragma unroll
+      for (int q = 0; q < PB; ++q) Bs[buf][q][b_lk[p]][b_ln[p]] = rb[q][p];
+    }
+  };
+
+  // ---- accumulators -------------------------------------------------------
+  d4 acc_re[2][2], acc_im[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      acc_re[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+      acc_im[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+    }
+
+  const int lane = tid & 63, w = tid >> 6;
+  const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  if (kb < ke) {
+    load_global(kb);
+    store_lds(0);
+  }
+  __syncthreads();
+  int buf = 0;
+  for (int k0 = kb; k0 < ke; k0 += BKK) {
+    const bool more = (k0 + BKK) < ke;
+    if (more) load_global(k0 + BKK);
+#pragma unroll
+    for (int k4 = 0; k4 < BKK / 4; ++k4) {
+      const int kr = k4 * 4 + lk;
+      double a[2][PA], b[2][PB];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int q = 0; q < PA; ++q) a[i][q] = As[buf][q][kr][wm + i * 16 + lr];
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int q = 0; q < PB; ++q) b[j][q] = Bs[buf][q][kr][wn + j * 16 + lr];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][0], b[j][0], acc_re[i][j], 0, 0, 0);
+          if (AC && BC) {
+            acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][1], -b[j][1], acc_re[i][j], 0, 0, 0);
+            acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][0], b[j][1], acc_im[i][j], 0, 0, 0);
+            acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][1], b[j][0], acc_im[i][j], 0, 0, 0);
+          } else if (BC) {
+            acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][0], b[j][PB - 1], acc_im[i][j], 0, 0, 0);
+          } else if (AC) {
+            acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][PA - 1], b[j][0], acc_im[i][j], 0, 0, 0);
+          }
+        }
+    }
+    if (more) store_lds(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+
+  // ---- epilogue -----------------------------------------------------------
+  constexpr bool CC = AC || BC;
+  if (g.splits > 1) {
+    const int64_t plane = (int64_t)g.splits * g.batch * g.M * g.N;
+    double* P = g.partial + ((int64_t)split * g.batch + bz) * g.M * (int64_t)g.N;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int col = n0 + wn + j * 16 + lr;
+        if (col >= g.N) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wm + i * 16 + lk * 4 + r;
+          if (row >= g.M) continue;
+          const int64_t o = (int64_t)row * g.N + col;
+          P[o] = acc_re[i][j][r];
+          if (CC) P[plane + o] = acc_im[i][j][r];
+        }
+      }
+    return;
+  }
+  const int64_t coff = g.cb ? g.cb[bz] : 0;
+  double* __restrict__ c_re = g.c_re + coff;
+  double* __restrict__ c_im = g.c_im ? g.c_im + coff : nullptr;
+  const double ar = g.alpha_re, ai = g.alpha_im;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int col = n0 + wn + j * 16 + lr;
+    if (col >= g.N) continue;
+    const int64_t cno = g.cn[col];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm + i * 16 + lk * 4 + r;
+        if (row >= g.M) continue;
+        const int64_t o = g.cm[row] + cno;
+        const double xr = acc_re[i][j][r];
+        const double xi = CC ? acc_im[i][j][r] : 0.0;
+        double vr = ar * xr - ai * xi;
+        double vi = ar * xi + ai * xr;
+        if (g.accumulate) {
+          vr += c_re[o];
+          if (c_im) vi += c_im[o];
+        }
+        c_re[o] = vr;
+        if (c_im) c_im[o] = vi;
+      }
+  }
+}
+
+__global__ void gett_reduce_kernel(GettArgs g, int c_cplx) {
+  const int64_t mn = (int64_t)g.M * g.N;
+  const int64_t total = mn * g.batch;
+  const int64_t plane = (int64_t)g.splits * total;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t bz = idx / mn;
+    const int64_t rem = idx - bz * mn;
+    const int row = (int)(rem / g.N), col = (int)(rem - (int64_t)(rem / g.N) * g.N);
+    double xr = 0.0, xi = 0.0;
+    for (int s = 0; s < g.splits; ++s) {
+      const int64_t o = (int64_t)s * total + idx;
+      xr += g.partial[o];
+      if (c_cplx) xi += g.partial[plane + o];
+    }
+    const int64_t o = (g.cb ? g.cb[bz] : 0) + g.cm[row] + g.cn[col];
+    double vr = g.alpha_re * xr - g.alpha_im * xi;
+    double vi = g.alpha_re * xi + g.alpha_im * xr;
+    if (g.accumulate) {
+      vr += g.c_re[o];
+      if (g.c_im) vi += g.c_im[o];
+    }
+    g.c_re[o] = vr;
+    if (g.c_im) g.c_im[o] = vi;
+  }
+}
+
+template <bool AC, bool BC>
+static void launch_gett_ab(const GettArgs& g, bool akc, bool bkc, hipStream_t s) {
+  dim3 grid(g.tiles_m * g.tiles_n, g.splits, g.batch);
+  if (akc && bkc) hipLaunchKernelGGL((gett_kernel<AC, BC, true, true>), grid, dim3(NT), 0, s, g);
+  else if (akc) hipLaunchKernelGGL((gett_kernel<AC, BC, true, false>), grid, dim3(NT), 0, s, g);
+  else if (bkc) hipLaunchKernelGGL((gett_kernel<AC, BC, false, true>), grid, dim3(NT), 0, s, g);
+  else hipLaunchKernelGGL((gett_kernel<AC, BC, false, false>), grid, dim3(NT), 0, s, g);
+}
+
+void launch_gett(const GettArgs& g, bool a_cplx, bool b_cplx, bool akc, bool bkc, hipStream_t s) {
+  if (a_cplx && b_cplx) launch_gett_ab<true, true>(g, akc, bkc, s);
+  else if (a_cplx) launch_gett_ab<true, false>(g, akc, bkc, s);
+  else if (b_cplx) launch_gett_ab<false, true>(g, akc, bkc, s);
+  else launch_gett_ab<false, false>(g, akc, bkc, s);
+}
+
+void launch_gett_reduce(const GettArgs& g, bool c_cplx, hipStream_t s) {
+  const int64_t total = (int64_t)g.M * g.N * g.batch;
+  int blocks = (int)((total + 255) / 256);
+  if (blocks > 4096) blocks = 4096;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(gett_reduce_kernel, dim3(blocks), dim3(256), 0, s, g, c_cplx ? 1 : 0);
+}
+
+// ===========================================================================
+// Linear combination of permuted tensors.  __dmul_rn/__dadd_rn keep every
+// product and sum individually rounded (no contraction into FMA), which is
+// how numpy evaluates `y + c * k`.
+// ===========================================================================
+__global__ void lincomb_kernel(LincombArgs a) {
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < a.total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    int64_t ix[4] = {0, 0, 0, 0};
+    int64_t rem = idx;
+    for (int d = a.rank - 1; d >= 0; --d) {
+      const int64_t q = rem / a.dims[d];
+      ix[d] = rem - q * a.dims[d];
+      rem = q;
+    }
+    int64_t oo = 0;
+    for (int d = 0; d < a.rank; ++d) oo += ix[d] * a.ost[d];
+    double sr = 0.0, si = 0.0;
+    if (a.accumulate) {
+      sr = a.out_re[oo];
+      if (a.out_im) si = a.out_im[oo];
+    }
+    for (int t = 0; t < a.nterms; ++t) {
+      const LincombTerm& T = a.t[t];
+      int64_t so = 0;
+      for (int d = 0; d < a.rank; ++d) so += ix[d] * T.st[d];
+      const double xr = T.re[so];
+      const double xi = T.im ? T.im[so] : 0.0;
+      double pr, pi;
+      if (T.ci == 0.0) {
+        pr = __dmul_rn(T.cr, xr);
+        pi = __dmul_rn(T.cr, xi);
+      } else {
+        pr = __dsub_rn(__dmul_rn(T.cr, xr), __dmul_rn(T.ci, xi));
+        pi = __dadd_rn(__dmul_rn(T.cr, xi), __dmul_rn(T.ci, xr));
+      }
+      if (t == 0 && !a.accumulate) {
+        sr = pr;
+        si = pi;
+      } else {
+        sr = __dadd_rn(sr, pr);
+        si = __dadd_rn(si, pi);
+      }
+    }
+    a.out_re[oo] = sr;
+    if (a.out_im) a.out_im[oo] = si;
+  }
+}
+
+static int grid_for(int64_t total) {
+  int64_t b = (total + 255) / 256;
+  if (b > 16384) b = 16384;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+void launch_lincomb(const LincombArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(lincomb_kernel, dim3(grid_for(a.total)), dim3(256), 0, s, a);
+}
+
+// ===========================================================================
+// Field + Fock build (FP64): bitwise the same operations as F + V(t) in the
+// reference, V(t) = (-A * mu) * exp(i theta).
+// ===========================================================================
+__global__ void field_kernel(FieldArgs a) {
+  const int64_t row = *a.counter - 1;
+  const double c = a.table[(row * 3 + a.stage) * 2 + 0];
+  const double s = a.table[(row * 3 + a.stage) * 2 + 1];
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < a.n2;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const double x = __dmul_rn(a.negA, a.mu_pol[idx]);
+    a.out_re[idx] = __dadd_rn(a.F[idx], __dmul_rn(x, c));
+    a.out_im[idx] = __dmul_rn(x, s);
+  }
+}
+
+void launch_field(const FieldArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(field_kernel, dim3(grid_for(a.n2)), dim3(256), 0, s, a);
+}
+
+// ===========================================================================
+// Fused RK4 finalize: rounded base + (-/+ i) dt/6 (k1 + 2k2 + 2k3 + k4).
+// ===========================================================================
+__global__ void finalize_kernel(FinalizeArgs a) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double yr = a.y_re[i], yi = a.y_im[i];
+    const double br = __ddiv_rn(rint(__dmul_rn(yr, a.scale)), a.scale);
+    const double bi = __ddiv_rn(rint(__dmul_rn(yi, a.scale)), a.scale);
+    double sr = __dadd_rn(a.k_re[0][i], __dmul_rn(2.0, a.k_re[1][i]));
+    double si = __dadd_rn(a.k_im[0][i], __dmul_rn(2.0, a.k_im[1][i]));
+    sr = __dadd_rn(sr, __dmul_rn(2.0, a.k_re[2][i]));
+    si = __dadd_rn(si, __dmul_rn(2.0, a.k_im[2][i]));
+    sr = __dadd_rn(sr, a.k_re[3][i]);
+    si = __dadd_rn(si, a.k_im[3][i]);
+    const double pr = __dmul_rn(a.dt6, sr);
+    const double pi = __dmul_rn(a.dt6, si);
+    // -1j * (pr + i pi) = pi - i pr ;  +1j * (pr + i pi) = -pi + i pr
+    const double ur = a.lambda_sign ? -pi : pi;
+    const double ui = a.lambda_sign ? pr : -pr;
+    a.y_re[i] = __dadd_rn(br, ur);
+    a.y_im[i] = __dadd_rn(bi, ui);
+    if (a.base_re) {
+      a.base_re[i] = br;
+      a.base_im[i] = bi;
+    }
+  }
+}
+
+void launch_finalize(const FinalizeArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(finalize_kernel, dim3(grid_for(a.n)), dim3(256), 0, s, a);
+}
+
+// ===========================================================================
+// Dipole trace over the assembled correlated density, one workgroup,
+// fixed-order tree reduction.
+// ===========================================================================
+__global__ __launch_bounds__(1024) void trace_kernel(TraceArgs a) {
+  __shared__ double red[6][1024];
+  const int n = a.o + a.v;
+  const int64_t n2 = (int64_t)n * n;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int64_t idx = threadIdx.x; idx < n2; idx += blockDim.x) {
+    const int p = (int)(idx / n), q = (int)(idx - (int64_t)p * n);
+    double dr, di;
+    if (p < a.o && q < a.o) {
+      const int64_t k = (int64_t)p * a.o + q;
+      dr = a.dij_re[k]; di = a.dij_im[k];
+    } else if (p < a.o) {
+      const int64_t k = (int64_t)p * a.v + (q - a.o);
+      dr = a.l1_re[k]; di = a.l1_im[k];
+    } else if (q < a.o) {
+      const int64_t k = (int64_t)(p - a.o) * a.o + q;
+      dr = a.dai_re[k]; di = a.dai_im[k];
+    } else {
+      const int64_t k = (int64_t)(p - a.o) * a.v + (q - a.o);
+      dr = a.dab_re[k]; di = a.dab_im[k];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double m = a.mu[c * n2 + idx];
+      acc[2 * c] += m * dr;
+      acc[2 * c + 1] += m * di;
+    }
+  }
+  for (int c = 0; c < 6; ++c) red[c][threadIdx.x] = acc[c];
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int c = 0; c < 6; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) {
+    const int64_t row = a.counter ? (*a.counter - 1) : 0;
+    a.out[row * 6 + threadIdx.x] = red[threadIdx.x][0];
+  }
+}
+
+void launch_trace(const TraceArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(trace_kernel, dim3(1), dim3(1024), 0, s, a);
+}
+
+__global__ void counter_inc_kernel(int64_t* c) { c[0] += 1; }
+__global__ void counter_set_kernel(int64_t* c, int64_t v) { c[0] = v; }
+
+void launch_counter_inc(int64_t* counter, hipStream_t s) {
+  hipLaunchKernelGGL(counter_inc_kernel, dim3(1), dim3(1), 0, s, counter);
+}
+void launch_counter_set(int64_t* counter, int64_t value, hipStream_t s) {
+  hipLaunchKernelGGL(counter_set_kernel, dim3(1), dim3(1), 0, s, counter, value);
+}
+
+// ===========================================================================
+// Synthetic TEI: Philox4x32-10 keyed by (0x20261203, config id), counter
+// (component 3, canonical orbit index), Box-Muller cosine branch.
+// ===========================================================================
+__device__ __forceinline__ void philox10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+  }
+}
+
+__device__ __forceinline__ double u53(uint32_t a, uint32_t b) {
+  const double x = (double)(a >> 5), y = (double)(b >> 6);
+  return __dmul_rn(__dadd_rn(__dadd_rn(__dmul_rn(x, 67108864.0), y), 0.5), 1.0 / 9007199254740992.0);
+}
+
+__global__ void synth_tei_kernel(double* tei, int n, uint32_t cid, double sigma) {
+  const int64_t n2 = (int64_t)n * n, total = n2 * n2;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    int64_t s = idx % n, r = (idx / n) % n, q = (idx / n2) % n, p = idx / (n2 * n);
+    double sign = (p == q || r == s) ? 0.0 : 1.0;
+    if (p > q) { const int64_t t = p; p = q; q = t; sign = -sign; }
+    if (r > s) { const int64_t t = r; r = s; s = t; sign = -sign; }
+    int64_t a = p * n + q, b = r * n + s;
+    if (a > b) { const int64_t t = a; a = b; b = t; }
+    const uint64_t cidx = (uint64_t)(a * n2 + b);
+    uint32_t c[4] = {3u, (uint32_t)cidx, (uint32_t)(cidx >> 32), 0u};
+    philox10(c, 0x20261203u, cid);
+    const double u1 = u53(c[0], c[1]), u2 = u53(c[2], c[3]);
+    const double z = __dmul_rn(sqrt(__dmul_rn(-2.0, log(u1))), cos(__dmul_rn(6.283185307179586, u2)));
+    tei[idx] = __dmul_rn(sign, __dmul_rn(sigma, z));
+  }
+}
+
+void launch_synth_tei(double* tei, int n, uint32_t cid, double sigma, hipStream_t s) {
+  const int64_t total = (int64_t)n * n * n * n;
+  hipLaunchKernelGGL(synth_tei_kernel, dim3(grid_for(total)), dim3(256), 0, s, tei, n, cid, sigma);
+}
+
+}  // namespace tdcc
+
+namespace tdcc {
+
+__global__ void split_complex_kernel(const double* in, double* re, double* im, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    re[i] = in[2 * i];
+    im[i] = in[2 * i + 1];
+  }
+}
+
+__global__ void join_complex_kernel(const double* re, const double* im, double* out, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    out[2 * i] = re[i];
+    out[2 * i + 1] = im ? im[i] : 0.0;
+  }
+}
+
+void launch_split_complex(const double* inter, double* re, double* im, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(split_complex_kernel, dim3(grid_for(n)), dim3(256), 0, s, inter, re, im, n);
+}
+
+void launch_join_complex(const double* re, const double* im, double* inter, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(join_complex_kernel, dim3(grid_for(n)), dim3(256), 0, s, re, im, inter, n);
+}
+
+}  // namespace tdcc

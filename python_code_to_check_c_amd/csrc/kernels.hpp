@@ -1,0 +1,119 @@
+// Device kernels of the TDCC propagation library and their launch arguments.
+//
+// Complex tensors are stored as two FP64 planes (re, im) of the same shape;
+// a real tensor has im == nullptr.  Every kernel here is deterministic: no
+// floating-point atomics, fixed reduction orders.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace tdcc {
+
+// ---------------------------------------------------------------------------
+// Strided tensor contraction on FP64 MFMA ("GEMM-like tensor-tensor", GETT).
+// The M, N, K and batch index groups of an einsum are flattened on the host
+// into offset tables, so C[cm[m] + cn[n] + cb[b]] = alpha * sum_k
+// A[am[m] + ak[k] + ab[b]] * B[bn[n] + bk[k] + bb[b]] (+ C if accumulate).
+struct GettArgs {
+  const double* a_re; const double* a_im;
+  const double* b_re; const double* b_im;
+  double* c_re; double* c_im;
+  const int64_t* am; const int64_t* ak;
+  const int64_t* bn; const int64_t* bk;
+  const int64_t* cm; const int64_t* cn;
+  const int64_t* ab; const int64_t* bb; const int64_t* cb;  // batch (may be null when batch == 1)
+  int M, N, K, batch;
+  int tiles_m, tiles_n;
+  int splits, k_chunk;          // split-K: splits > 1 writes partials
+  double alpha_re, alpha_im;
+  int accumulate;
+  double* partial;              // [splits][batch][M][N] x planes, when splits > 1
+};
+
+// Loader mapping: whether A's / B's memory-contiguous index lies in K.
+void launch_gett(const GettArgs& g, bool a_cplx, bool b_cplx, bool a_kcontig,
+                 bool b_kcontig, hipStream_t s);
+void launch_gett_reduce(const GettArgs& g, bool c_cplx, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Linear combination with per-term permuted strides:
+//   out[idx] (+)= sum_t coef_t * src_t[sum_d idx_d * st_t[d]]
+constexpr int kMaxTerms = 8;
+struct LincombTerm {
+  const double* re; const double* im;
+  int64_t st[4];
+  double cr, ci;
+};
+struct LincombArgs {
+  double* out_re; double* out_im;
+  int64_t ost[4];
+  int64_t dims[4];
+  int rank;
+  int64_t total;
+  int nterms;
+  int accumulate;
+  LincombTerm t[kMaxTerms];
+};
+void launch_lincomb(const LincombArgs& a, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Field + Fock build for one stage time of the current step:
+//   x = (-A) * mu_pol;  F_s = (F + x*cos) + i (x*sin)
+// (cos, sin) come from table[(counter - 1) * 3 + stage].
+struct FieldArgs {
+  const double* F; const double* mu_pol;
+  double* out_re; double* out_im;
+  int64_t n2;
+  double negA;
+  const double* table;   // [steps][3][2]
+  const int64_t* counter;
+  int stage;
+};
+void launch_field(const FieldArgs& a, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Fused RK4 finalize for one amplitude block:
+//   base = rint(y * scale) / scale
+//   inc  = dt6 * (((k1 + 2 k2) + 2 k3) + k4)
+//   y    = base + sign_i * inc      (sign_i = -1j for T, +1j for Lambda)
+struct FinalizeArgs {
+  double* y_re; double* y_im;
+  double* base_re; double* base_im;   // may be null
+  const double* k_re[4]; const double* k_im[4];
+  int64_t n;
+  double dt6, scale;
+  int lambda_sign;                    // 0: -1j (T), 1: +1j (Lambda)
+};
+void launch_finalize(const FinalizeArgs& a, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Dipole trace: out[c] = sum_pq mu[c][p][q] * D[p][q] with the correlated
+// density assembled from its blocks [[Dij, lam1], [Dai, Dab]].
+struct TraceArgs {
+  const double* dij_re; const double* dij_im;
+  const double* l1_re;  const double* l1_im;
+  const double* dai_re; const double* dai_im;
+  const double* dab_re; const double* dab_im;
+  const double* mu;     // [3][n][n]
+  int o, v;
+  double* out;          // [rows][3][2]
+  const int64_t* counter;   // row = counter - 1 (row 0 when null)
+};
+void launch_trace(const TraceArgs& a, hipStream_t s);
+
+void launch_counter_inc(int64_t* counter, hipStream_t s);
+void launch_counter_set(int64_t* counter, int64_t value, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Synthetic antisymmetrized TEI on the device (same generator as
+// python_code_to_check_c_amd/synthetic.py).
+void launch_synth_tei(double* tei, int n, uint32_t config_id, double sigma, hipStream_t s);
+
+}  // namespace tdcc
+
+namespace tdcc {
+// Interleaved complex <-> two planes.
+void launch_split_complex(const double* inter, double* re, double* im, int64_t n, hipStream_t s);
+void launch_join_complex(const double* re, const double* im, double* inter, int64_t n, hipStream_t s);
+}  // namespace tdcc

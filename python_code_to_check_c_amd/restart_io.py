@@ -1,0 +1,124 @@
+"""Restart and output files in the reference's layout.
+
+Writers follow CCSD_Helper.py:1102-1145, readers CC_Calculator.py:70-135:
+  Parameters.csv        pandas frame ``,w0,A,t0,dt,precs,i,a,CCSD_or_CC2`` with
+                        row index 1, floats ``%.10f``; t0 holds the stop time t - dt
+  {F,t1,lam1}_{real,imag}.dat   ``i<TAB>a<TAB>value`` per element, C order
+  {t2,lam2}_{real,imag}.dat     ``i<TAB>j<TAB>a<TAB>b<TAB>value``
+                        value = str(numpy.around(x, precs)), csv module, '\\n' endings
+  H2O.csv               ``,time,mu_real,mu_imag`` (appended without header on restart)
+  timing.csv            ``,total,t1,t2,l1,l2``
+"""
+from __future__ import annotations
+
+import csv
+import os
+
+import numpy as np
+import pandas as pd
+
+
+def _writer(f):
+    return csv.writer(f, delimiter="\t", quotechar="|", quoting=csv.QUOTE_MINIMAL, lineterminator="\n")
+
+
+def write_2data(F, FileName, precs):
+    """CCSD_Helper.py:1107-1114"""
+    F = np.asarray(F)
+    r = np.around(F, decimals=precs)
+    with open(FileName, "w") as f:
+        w = _writer(f)
+        for i in range(F.shape[0]):
+            for a in range(F.shape[1]):
+                w.writerow([i, a, r[i, a]])
+
+
+def write_4data(F, FileName, precs):
+    """CCSD_Helper.py:1116-1125"""
+    F = np.asarray(F)
+    r = np.around(F, decimals=precs)
+    o, _, v, _ = F.shape
+    with open(FileName, "w") as f:
+        w = _writer(f)
+        for i in range(o):
+            for j in range(F.shape[1]):
+                for a in range(v):
+                    row = r[i, j, a]
+                    for b in range(F.shape[3]):
+                        w.writerow([i, j, a, b, row[b]])
+
+
+def save_parameters(w0, A, t, dt, precs, i, a, method, path="."):
+    """CCSD_Helper.py:1102-1105 (the t0 column receives the time passed in)."""
+    df = pd.DataFrame(columns=("w0", "A", "t0", "dt", "precs", "i", "a", "CCSD_or_CC2"))
+    df.loc[1] = [w0, A, t, dt, precs, i, a, method]
+    df.to_csv(os.path.join(path, "Parameters.csv"), float_format="%.10f")
+
+
+def save_data(F, t1, t2, lam1, lam2, data, timing, precs, restart, path="."):
+    """CCSD_Helper.py:1127-1145"""
+    j = lambda name: os.path.join(path, name)
+    if restart is None:
+        data.to_csv(j("H2O.csv"))
+        timing.to_csv(j("timing.csv"))
+    else:
+        with open(j("H2O.csv"), "a") as f:
+            data.to_csv(f, header=False)
+        with open(j("timing.csv"), "a") as f:
+            timing.to_csv(f, header=False)
+    F = np.asarray(F)
+    write_2data(F.real, j("F_real.dat"), precs)
+    write_2data(F.imag if np.iscomplexobj(F) else np.zeros_like(F), j("F_imag.dat"), precs)
+    write_2data(t1.real, j("t1_real.dat"), precs)
+    write_2data(t1.imag, j("t1_imag.dat"), precs)
+    write_4data(t2.real, j("t2_real.dat"), precs)
+    write_4data(t2.imag, j("t2_imag.dat"), precs)
+    write_2data(lam1.real, j("lam1_real.dat"), precs)
+    write_2data(lam1.imag, j("lam1_imag.dat"), precs)
+    write_4data(lam2.real, j("lam2_real.dat"), precs)
+    write_4data(lam2.imag, j("lam2_imag.dat"), precs)
+
+
+def convert_2data(FileName, i, a):
+    """CC_Calculator.py:85-96: rows (i, a, value) -> dense (i, a)."""
+    df = pd.read_csv(FileName, sep="\t", header=None, names=["i", "a", "F"])
+    out = np.zeros((i, a))
+    out[np.around(df["i"].to_numpy()).astype(int), np.around(df["a"].to_numpy()).astype(int)] = df["F"].to_numpy()
+    return out
+
+
+def convert_4data(FileName, i, a):
+    """CC_Calculator.py:98-113: rows (i, j, a, b, value) -> dense (i, i, a, a)."""
+    df = pd.read_csv(FileName, sep="\t", header=None, names=["i", "j", "a", "b", "F"])
+    out = np.zeros((i, i, a, a))
+    idx = [np.around(df[k].to_numpy()).astype(int) for k in ("i", "j", "a", "b")]
+    out[idx[0], idx[1], idx[2], idx[3]] = df["F"].to_numpy()
+    return out
+
+
+def read_restart(path="."):
+    """Everything TDCC_restart reads (CC_Calculator.py:70-135).
+
+    Returns a dict with w0, A, t0 (already advanced by dt, :80), tf (= 50 + the
+    saved t0, :77), dt, precs, method, o, v, F (real part only, :135), t1, t2,
+    lam1, lam2.
+    """
+    j = lambda name: os.path.join(path, name)
+    p = pd.read_csv(j("Parameters.csv"))
+    w0 = p["w0"][0]
+    A = p["A"][0]
+    t0 = p["t0"][0]
+    method = p["CCSD_or_CC2"][0]
+    tf = 50.0 + t0
+    dt = p["dt"][0]
+    t0 = t0 + dt
+    precs = int(p["precs"][0])
+    i = int(p["i"][0])
+    a = int(p["a"][0])
+    t1 = convert_2data(j("t1_real.dat"), i, a) + 1.0 * 1j * convert_2data(j("t1_imag.dat"), i, a)
+    lam1 = convert_2data(j("lam1_real.dat"), i, a) + 1.0 * 1j * convert_2data(j("lam1_imag.dat"), i, a)
+    t2 = convert_4data(j("t2_real.dat"), i, a) + 1.0 * 1j * convert_4data(j("t2_imag.dat"), i, a)
+    lam2 = convert_4data(j("lam2_real.dat"), i, a) + 1.0 * 1j * convert_4data(j("lam2_imag.dat"), i, a)
+    F = convert_2data(j("F_real.dat"), i + a, i + a)
+    return dict(w0=w0, A=A, t0=t0, tf=tf, dt=dt, precs=precs, method=method, o=i, v=a, F=F,
+                t1=t1, t2=t2, lam1=lam1, lam2=lam2)

@@ -164,14 +164,38 @@ def w0_draws(config_id: int, count: int, lo: float = 0.3, hi: float = 1.2):
     return lo + (hi - lo) * uniform(config_id, COMP_W0, np.arange(count))
 
 
-class SyntheticSystem:
-    """Bundle of (F, TEI, mu) for one (config id, o, v)."""
+def tei_block(config_id: int, o: int, v: int, classes: str, sigma: float = 0.05):
+    """One block of the TEI, e.g. 'oovv', without building the full (n,n,n,n)."""
+    n = o + v
+    rng = {"o": np.arange(o), "v": np.arange(o, n)}
+    axes = [rng[c] for c in classes]
+    p, q, r, s = np.meshgrid(*axes, indexing="ij")
+    idx, sign = tei_canonical(p, q, r, s, n)
+    return sign * (sigma * normal(config_id, COMP_TEI, idx))
 
-    def __init__(self, config_id: int, o: int, v: int, with_tei: bool = True):
+
+class SyntheticSystem:
+    """System provider for the propagation path: F, mu and the TEI of one
+    (config id, o, v).
+
+    With `tei_on_device=True` the (n,n,n,n) TEI is not built on the host;
+    the HIP library generates it from `tei_config = (config_id, sigma)`.
+    """
+
+    def __init__(self, config_id: int, o: int, v: int, with_tei: bool = True,
+                 tei_sigma: float = 0.05, tei_on_device: bool = False):
         self.config_id = config_id
-        self.o = o
-        self.v = v
+        self.o = self.nocc = o
+        self.v = self.nvir = v
         self.n = o + v
+        self.tei_sigma = tei_sigma
         self.F = fock(config_id, o, v)
         self.mu = dipoles(config_id, o, v)
-        self.TEI = tei(config_id, o, v) if with_tei else None
+        self.tei_config = (config_id, tei_sigma)
+        self.TEI = tei(config_id, o, v, sigma=tei_sigma) if (with_tei and not tei_on_device) else None
+
+    def tei_block(self, classes: str):
+        if self.TEI is not None:
+            sl = {"o": slice(0, self.o), "v": slice(self.o, self.n)}
+            return self.TEI[tuple(sl[c] for c in classes)]
+        return tei_block(self.config_id, self.o, self.v, classes, self.tei_sigma)

@@ -1,0 +1,59 @@
+"""The oracle against golden vectors written from the reference itself.
+
+tests/golden/golden_*.npz hold inputs and the reference's outputs for every
+residual, intermediate, density block, the dipole and short Runge_Kutta_solver
+trajectories (CCSD and CC2); see tests/golden/make_golden.py.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import rk4
+from oracle.cc_equations import CCEquations
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CASES = ["o4v6", "c1_o10v4"]
+TOL = 1e-13
+
+T_F = ["Fae", "Fmi", "Fme", "T1eq_rhs", "Wmnij", "Wmnij_2", "Wabef", "Wabef_2", "Wmbej", "T2eq_rhs",
+       "T2eq_rhs_CC2", "LRFea", "LRFim", "LSWieam", "LRWefam", "LRWibjm", "LSWmnij", "LSWabef"]
+L_F = ["lam_1eq_rhs", "lam2eq_rhs", "L1eq_rhs_CC2", "L2eq_rhs_CC2"]
+D_F = ["Dij", "Dab", "Dai"]
+
+
+def load(name):
+    g = dict(np.load(os.path.join(HERE, "golden", f"golden_{name}.npz")))
+    return g, CCEquations(g["TEI"], int(g["o"]))
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_functions(name):
+    g, eq = load(name)
+    t1, t2, l1, l2, Fc = g["t1"], g["t2"], g["lam1"], g["lam2"], g["Fc"]
+    for f in T_F:
+        assert np.abs(getattr(eq, f)(t1, t2, Fc) - g["out_" + f]).max() <= TOL, f
+    for f in L_F:
+        assert np.abs(getattr(eq, f)(t1, t2, l1, l2, Fc) - g["out_" + f]).max() <= TOL, f
+    for f in D_F:
+        assert np.abs(getattr(eq, f)(t1, t2, l1, l2) - g["out_" + f]).max() <= TOL, f
+    assert np.abs(eq.Gfe(t2, l2) - g["out_Gfe"]).max() <= TOL
+    assert np.abs(eq.Gmn(t2, l2) - g["out_Gmn"]).max() <= TOL
+    assert np.abs(eq.LWfiea(t1) - g["out_LWfiea"]).max() <= TOL
+    assert np.abs(eq.LWmina(t1) - g["out_LWmina"]).max() <= TOL
+    assert abs(eq.CCSD_Corr_E(t1, t2, Fc) - g["out_CCSD_Corr_E"]) <= TOL
+    mu = np.array(eq.dipole_moment(t1, t2, l1, l2, g["mu"]))
+    assert np.abs(mu - g["out_dipole_moment"]).max() <= TOL
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("method", ["CCSD", "CC2"])
+def test_rk4_trajectory(name, method):
+    g, eq = load(name)
+    tr = rk4.propagate(eq, g["F"], g["t1"], g["t2"], g["lam1"], g["lam2"], g["mu"], float(g["rk_w0"]),
+                       float(g["rk_A"]), 0.0, float(g["rk_dt"]), method=method, max_steps=int(g["rk_steps"]))
+    ref_mu = g[f"rk_{method}_mu"]
+    assert np.abs(tr.mu0 - ref_mu[0]).max() <= TOL
+    assert np.abs(tr.mu - ref_mu[1:]).max() <= TOL
+    for k, key in enumerate(("t1", "t2", "lam1", "lam2")):
+        assert np.abs(tr.state[k] - g[f"rk_{method}_{key}"][-1]).max() <= TOL, key
