@@ -49,6 +49,7 @@ struct tdcc_ctx {
   bool have_tei = false, have_system = false;
   Planes state[4], base[4], backup[4], io_in[4], io_out, io_F, int_out;
   double* staging = nullptr;
+  int64_t staging_n = 0;   // complex elements the staging buffer holds
   int64_t* counter = nullptr;
   double* table = nullptr;
   int64_t table_cap = 0;
@@ -78,7 +79,18 @@ void free_planes(Planes& p) {
   p.d = nullptr;
 }
 
+void ensure_staging(tdcc_ctx* c, int64_t n) {
+  if (n <= c->staging_n) return;
+  TDCC_HIP(hipStreamSynchronize(c->stream));
+  if (c->staging) TDCC_HIP(hipFree(c->staging));
+  c->staging = nullptr;
+  c->staging_n = 0;
+  TDCC_HIP(hipMalloc(&c->staging, 2 * n * sizeof(double)));
+  c->staging_n = n;
+}
+
 void upload_complex(tdcc_ctx* c, const double* host, const Planes& p) {
+  ensure_staging(c, p.n);
   TDCC_HIP(hipMemcpyAsync(c->staging, host, 2 * p.n * sizeof(double), hipMemcpyHostToDevice, c->stream));
   launch_split_complex(c->staging, p.re(), p.im(), p.n, c->stream);
   TDCC_HIP(hipGetLastError());
@@ -86,6 +98,7 @@ void upload_complex(tdcc_ctx* c, const double* host, const Planes& p) {
 }
 
 void download_complex(tdcc_ctx* c, const Planes& p, double* host) {
+  ensure_staging(c, p.n);
   launch_join_complex(p.re(), p.im(), c->staging, p.n, c->stream);
   TDCC_HIP(hipGetLastError());
   TDCC_HIP(hipMemcpyAsync(host, c->staging, 2 * p.n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -446,7 +459,7 @@ int tdcc_create(int nocc, int nvir, int device, tdcc_ctx** out) {
     const int64_t big = std::max<int64_t>(block_size(c, 1), n * n);
     alloc_planes(c->io_out, big);
     alloc_planes(c->io_F, n * n);
-    TDCC_HIP(hipMalloc(&c->staging, 2 * big * sizeof(double)));
+    ensure_staging(c, big);
     TDCC_HIP(hipMalloc(&c->counter, sizeof(int64_t)));
     launch_counter_set(c->counter, 0, c->stream);
     ensure_rows(c, 1024);

@@ -17,7 +17,8 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 //
 // MFMA 16x16x4 f64 operand layout (lane l of the wave):
 //   A[i = l%16][k = l/16], B[k = l/16][j = l%16],
-//   D[i = 4*(l/16) + r][j = l%16] for the four accumulator elements r.
+//   D[i = l/16 + 4*r][j = l%16] for the four accumulator elements r
+//   (rows interleaved by 4; measured on gfx950, unlike the f32 variant).
 // ===========================================================================
 constexpr int BM = 64, BN = 64, BKK = 16, NT = 256;
 constexpr int LDA = BM + 1, LDB = BN + 1;
@@ -201,7 +202,7 @@ __global__ __launch_bounds__(NT) void gett_kernel(GettArgs g) {
         if (col >= g.N) continue;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int row = m0 + wm + i * 16 + lk * 4 + r;
+          const int row = m0 + wm + i * 16 + lk + 4 * r;
           if (row >= g.M) continue;
           const int64_t o = (int64_t)row * g.N + col;
           P[o] = acc_re[i][j][r];
@@ -223,7 +224,7 @@ __global__ __launch_bounds__(NT) void gett_kernel(GettArgs g) {
     for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm + i * 16 + lk * 4 + r;
+        const int row = m0 + wm + i * 16 + lk + 4 * r;
         if (row >= g.M) continue;
         const int64_t o = g.cm[row] + cno;
         const double xr = acc_re[i][j][r];
