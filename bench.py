@@ -80,7 +80,7 @@ def cpu_baseline(method, o_full, v_full, budget_s=20.0):
 
 def roofline(records):
     groups = {}
-    for name, fl, by, ms in records:
+    for name, fl, by, ms, _ in records:
         g = groups.setdefault(name, [0.0, 0.0, 0.0, 0])
         g[0] += fl
         g[1] += by

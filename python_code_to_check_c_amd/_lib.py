@@ -199,15 +199,15 @@ class Context:
         return mu[:nsteps], ts[:nsteps]
 
     def profile_step(self, method, t0, dt, w0, A, pol=2, precs=15):
-        """One eager step with per-launch HIP event timing: list of (name, flops, bytes, ms)."""
+        """One eager step with per-launch HIP event timing: list of (name, flops, bytes, ms, detail)."""
         m = METHODS[method] if isinstance(method, str) else int(method)
         buf = ctypes.create_string_buffer(1 << 22)
         self._check(self.lib.tdcc_profile_step(self._h, m, float(t0), float(dt), float(w0), float(A), int(pol),
                                                int(precs), buf, len(buf)), "tdcc_profile_step")
         out = []
         for line in buf.value.decode().splitlines():
-            name, fl, by, ms = line.split("\t")
-            out.append((name, float(fl), float(by), float(ms)))
+            name, fl, by, ms, detail = line.split("\t")
+            out.append((name, float(fl), float(by), float(ms), detail))
         return out
 
     def checkpoint(self, op):

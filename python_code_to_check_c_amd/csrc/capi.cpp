@@ -57,6 +57,8 @@ struct tdcc_ctx {
   int64_t mu_cap = 0;
   double* arena = nullptr;
   int64_t arena_bytes = 0;
+  double* vpack[2] = {nullptr, nullptr};   // packed <ab||ef>, <ef||ab> (may alias)
+  unsigned long long* counts = nullptr;    // kCountBlocks + 1 partial counts
   Runtime rt;
   std::map<std::string, Program> progs;
 };
@@ -387,6 +389,44 @@ int guarded(tdcc_ctx* c, F&& f) {
   }
 }
 
+unsigned long long device_count(tdcc_ctx* c) {
+  unsigned long long n = 0;
+  TDCC_HIP(hipMemcpyAsync(&n, c->counts + kCountBlocks, sizeof n, hipMemcpyDeviceToHost, c->stream));
+  TDCC_HIP(hipStreamSynchronize(c->stream));
+  return n;
+}
+
+// Packed antisymmetric <ab||ef> for the particle-particle ladders.  The
+// packing is exact only for antisymmetrized integrals, which is checked here.
+void build_vpack(tdcc_ctx* c) {
+  TDCC_HIP(hipStreamSynchronize(c->stream));
+  for (auto& kv : c->progs) invalidate_exec(kv.second);
+  if (c->vpack[1] && c->vpack[1] != c->vpack[0]) TDCC_HIP(hipFree(c->vpack[1]));
+  if (c->vpack[0]) TDCC_HIP(hipFree(c->vpack[0]));
+  c->vpack[0] = c->vpack[1] = nullptr;
+  c->rt.vpack[0] = c->rt.vpack[1] = nullptr;
+  if (!c->counts) TDCC_HIP(hipMalloc(&c->counts, (kCountBlocks + 1) * sizeof(unsigned long long)));
+  launch_vvvv_antisym_check(c->tei, c->o, c->v, c->counts, c->stream);
+  TDCC_HIP(hipGetLastError());
+  const unsigned long long bad = device_count(c);
+  if (bad) throw Error("the <ab||ef> block is not antisymmetric (" + std::to_string(bad) + " entries); "
+                       "the integrals must be antisymmetrized spin-orbital <pq||rs>");
+  const int64_t P = (int64_t)c->v * (c->v - 1) / 2;
+  if (P == 0) return;
+  TDCC_HIP(hipMalloc(&c->vpack[0], P * P * sizeof(double)));
+  TDCC_HIP(hipMalloc(&c->vpack[1], P * P * sizeof(double)));
+  launch_build_vpack(c->tei, c->o, c->v, 0, c->vpack[0], c->stream);
+  launch_build_vpack(c->tei, c->o, c->v, 1, c->vpack[1], c->stream);
+  launch_count_mismatch(c->vpack[0], c->vpack[1], P * P, c->counts, c->stream);
+  TDCC_HIP(hipGetLastError());
+  if (device_count(c) == 0) {  // <ab||ef> == <ef||ab>: one copy serves both ladders
+    TDCC_HIP(hipFree(c->vpack[1]));
+    c->vpack[1] = c->vpack[0];
+  }
+  c->rt.vpack[0] = c->vpack[0];
+  c->rt.vpack[1] = c->vpack[1];
+}
+
 void require_system(tdcc_ctx* c) {
   if (!c->have_system || !c->have_tei) throw Error("system not set: call tdcc_set_system (and tdcc_synth_tei when tei was NULL)");
 }
@@ -496,6 +536,9 @@ void tdcc_destroy(tdcc_ctx* c) {
   free_planes(c->int_out);
   for (double* p : {c->tei, c->F, c->mu, c->staging, c->table, c->mu_out, c->arena})
     if (p) (void)hipFree(p);
+  if (c->vpack[1] && c->vpack[1] != c->vpack[0]) (void)hipFree(c->vpack[1]);
+  if (c->vpack[0]) (void)hipFree(c->vpack[0]);
+  if (c->counts) (void)hipFree(c->counts);
   if (c->counter) (void)hipFree(c->counter);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -509,6 +552,8 @@ int tdcc_set_system(tdcc_ctx* c, const double* tei, const double* fock, const do
     const int64_t n = c->n;
     if (tei) {
       TDCC_HIP(hipMemcpy(c->tei, tei, n * n * n * n * sizeof(double), hipMemcpyHostToDevice));
+      c->have_tei = false;
+      build_vpack(c);
       c->have_tei = true;
     }
     TDCC_HIP(hipMemcpy(c->F, fock, n * n * sizeof(double), hipMemcpyHostToDevice));
@@ -522,6 +567,8 @@ int tdcc_synth_tei(tdcc_ctx* c, uint32_t config_id, double sigma) {
     launch_synth_tei(c->tei, c->n, config_id, sigma, c->stream);
     TDCC_HIP(hipGetLastError());
     TDCC_HIP(hipStreamSynchronize(c->stream));
+    c->have_tei = false;
+    build_vpack(c);
     c->have_tei = true;
   });
 }
@@ -628,7 +675,8 @@ int tdcc_profile_step(tdcc_ctx* c, int method, double t0, double dt, double w0, 
     std::string txt;
     char line[256];
     for (const auto& r : recs) {
-      std::snprintf(line, sizeof line, "%s\t%.17g\t%.17g\t%.9g\n", r.name.c_str(), r.flops, r.bytes, (double)r.ms);
+      std::snprintf(line, sizeof line, "%s\t%.17g\t%.17g\t%.9g\t%s\n", r.name.c_str(), r.flops, r.bytes, (double)r.ms,
+                    r.detail.c_str());
       txt += line;
     }
     if ((int64_t)txt.size() + 1 > report_len) throw Error("report buffer too small: need " + std::to_string(txt.size() + 1));

@@ -336,8 +336,9 @@ int CCBuilder::T2_vvvv_terms(int t1, int t2, int tau) {
   const int V = blk("vvvv"), vovv = blk("vovv"), oovv = blk("oovv");
   const int tt = t1t1(t1);
   const int u = L(Terms().add(t2).add(tau));
-  // <ab||ef> part of both, on u = t2 + tau
-  const int ladder = C("abef,ijef->ijab", V, u);
+  // <ab||ef> part of both, on u = t2 + tau ('abef,ijef->ijab', over pairs e<f, a<b)
+  (void)V;
+  const int ladder = g_.ladder(u, 0);
   // -P(ab) sum_m <am||ef> t_mb  ->  sum_m t_mb (sum_ef <am||ef> u_ijef)
   const int q = C("amef,ijef->ijam", vovv, u);
   const int wq = C("ijam,mb->ijab", q, t1);
@@ -369,9 +370,10 @@ int CCBuilder::T2_vvvv_terms_CC2(int t1, int tau) {
 
 int CCBuilder::L2_vvvv_term(int t1, int t2, int l2) {
   // sum_ef LSWabef_efab lam_ijef, LSWabef = 1/2 <ef||ab> - <em||ab> t_mf + <nm||ab> tau'_nmef
-  const int V = blk("vvvv"), vovv = blk("vovv"), oovv = blk("oovv");
+  const int vovv = blk("vovv"), oovv = blk("oovv");
   const int tau = L(Terms().add(t2, 0.25).add(t1t1(t1), 0.5));
-  const int ladder = C("efab,ijef->ijab", V, l2);
+  // 'efab,ijef->ijab' over pairs e<f, a<b
+  const int ladder = g_.ladder(l2, 1);
   const int p = C("mf,ijef->ijem", t1, l2);
   const int h = C("nmef,ijef->ijnm", tau, l2);
   return L(Terms()

@@ -32,9 +32,23 @@ Term tmp(int src, std::initializer_list<int> perm, double c) {
   return t;
 }
 
+struct LadderArgs {
+  const double* u_re = nullptr;
+  const double* u_im = nullptr;
+  double* out_re = nullptr;
+  double* out_im = nullptr;
+  double* up = nullptr;   // (2 rows) x P packed input
+  double* rp = nullptr;   // (2 rows) x P packed result
+  const double* vp = nullptr;
+  int64_t rows = 0;
+  int v = 0;
+  int64_t P = 0;
+};
+
 struct Graph::Launch {
-  enum Kind { GETT, LIN, FIELD, FIN, TRACE } kind;
+  enum Kind { GETT, LIN, FIELD, FIN, TRACE, LADDER } kind;
   int node = -1;
+  LadderArgs lad{};
   GettArgs g{};
   bool ac = false, bc = false, akc = false, bkc = false, c_cplx = false;
   int64_t tab[9] = {0, 0, 0, 0, 0, 0, -1, -1, -1};  // am ak bn bk cm cn ab bb cb offsets into tables
@@ -321,6 +335,21 @@ int Graph::field(int stage) {
   return add(std::move(n));
 }
 
+int Graph::ladder(int u, int ket) {
+  const Node& U = nodes_[u];
+  if (U.shape.rank != 4 || U.shape.cls[0] != Cls::O || U.shape.cls[1] != Cls::O || U.shape.cls[2] != Cls::V ||
+      U.shape.cls[3] != Cls::V)
+    throw Error("ladder input must be (o,o,v,v)");
+  Node n;
+  n.op = Op::Ladder;
+  n.shape = U.shape;
+  n.cplx = true;
+  n.a = u;
+  n.stage = ket;
+  n.key = "LAD|" + std::to_string(u) + "|" + std::to_string(ket);
+  return add(std::move(n));
+}
+
 void Graph::finalize_rk(int y_ext, int base_ext, const int k[4], bool lambda) {
   Node n;
   n.op = Op::Finalize;
@@ -352,7 +381,9 @@ void Graph::assign(int dst, int src) {
 }
 
 // ---------------------------------------------------------------------------
-static bool owns_buffer(Op op) { return op == Op::Contract || op == Op::Lincomb || op == Op::Field; }
+static bool owns_buffer(Op op) {
+  return op == Op::Contract || op == Op::Lincomb || op == Op::Field || op == Op::Ladder;
+}
 
 static void inputs_of(const Node& n, std::vector<int>& in) {
   in.clear();
@@ -365,6 +396,7 @@ static void inputs_of(const Node& n, std::vector<int>& in) {
     case Op::Finalize: in = {n.k[0], n.k[1], n.k[2], n.k[3], n.a, n.b}; break;
     case Op::Trace: in = {n.k[0], n.k[1], n.k[2], n.k[3]}; break;
     case Op::Assign: in = {n.a, n.b}; break;
+    case Op::Ladder: in = {n.a}; break;
     default: break;
   }
 }
@@ -439,8 +471,9 @@ void Graph::plan() {
     if (!live[i]) continue;
     Node& n = nodes_[i];
     if (owns_buffer(n.op)) n.arena_off = alloc(size(n.shape) * (n.cplx ? 2 : 1) * (int64_t)sizeof(double));
-    if (n.op == Op::Contract) {
-      lower_contract(i);
+    if (n.op == Op::Ladder) n.scratch_len = 2 * (2 * (int64_t)o_ * o_ * ((int64_t)v_ * (v_ - 1) / 2)) * (int64_t)sizeof(double);
+    if (n.op == Op::Contract || n.op == Op::Ladder) {
+      if (n.op == Op::Contract) lower_contract(i);
       if (n.scratch_len > 0) {
         n.scratch_off = alloc(n.scratch_len);
         release(n.scratch_off, n.scratch_len);
@@ -711,6 +744,27 @@ void Graph::bind(double* arena) {
         }
         break;
       }
+      case Op::Ladder: {
+        auto L = std::make_unique<Launch>();
+        L->kind = Launch::LADDER;
+        L->node = i;
+        const Dev U = resolve(n.a), O = resolve(i);
+        if (!nodes_[n.a].cplx) throw Error("internal: real ladder input");
+        LadderArgs& a = L->lad;
+        a.u_re = U.re;
+        a.u_im = U.im;
+        a.out_re = O.re;
+        a.out_im = O.im;
+        a.rows = (int64_t)o_ * o_;
+        a.v = v_;
+        a.P = (int64_t)v_ * (v_ - 1) / 2;
+        a.up = arena_ + n.scratch_off / (int64_t)sizeof(double);
+        a.rp = a.up + 2 * a.rows * a.P;
+        a.vp = rt_->vpack[n.stage];
+        if (!a.vp) throw Error("packed <ab||ef> not built");
+        launches_.push_back(std::move(L));
+        break;
+      }
       case Op::Field: {
         auto L = std::make_unique<Launch>();
         L->kind = Launch::FIELD;
@@ -774,6 +828,26 @@ void Graph::bind(double* arena) {
   }
 }
 
+static DgemmArgs ladder_gemm(const LadderArgs& a) {
+  DgemmArgs g{};
+  g.A = a.up;
+  g.B = a.vp;
+  g.C = a.rp;
+  g.M = (int)(2 * a.rows);
+  g.N = (int)a.P;
+  g.K = (int)a.P;
+  g.lda = g.ldb = g.ldc = a.P;
+  g.alpha = 1.0;
+  g.beta = 0.0;
+  return g;
+}
+
+static void run_ladder(const LadderArgs& a, hipStream_t s) {
+  launch_pack_pairs(a.u_re, a.u_im, a.up, a.rows, a.v, s);
+  launch_dgemm(ladder_gemm(a), true, s);
+  launch_unpack_pairs(a.rp, a.out_re, a.out_im, a.rows, a.v, 1.0, s);
+}
+
 void Graph::run(hipStream_t s) const {
   for (const auto& L : launches_) {
     switch (L->kind) {
@@ -783,6 +857,7 @@ void Graph::run(hipStream_t s) const {
         if (L->g.splits > 1) launch_gett_reduce(L->g, L->c_cplx, s);
         break;
       case Launch::LIN: launch_lincomb(L->l, s); break;
+      case Launch::LADDER: run_ladder(L->lad, s); break;
       case Launch::FIELD: launch_field(L->f, s); break;
       case Launch::FIN: launch_finalize(L->fin, s); break;
       case Launch::TRACE: launch_trace(L->tr, s); break;
@@ -792,16 +867,21 @@ void Graph::run(hipStream_t s) const {
 }
 
 void Graph::run_profiled(hipStream_t s, std::vector<LaunchRecord>& out) const {
-  std::vector<hipEvent_t> ev(2 * launches_.size() + 4);
-  for (auto& e : ev) TDCC_HIP(hipEventCreate(&e));
+  std::vector<hipEvent_t> ev;  // one pair per timed launch, created as needed
+  ev.reserve(8 * launches_.size() + 8);
   std::vector<LaunchRecord> recs;
-  size_t k = 0;
+  std::string detail;
   auto timed = [&](const std::string& name, double flops, double bytes, auto&& fn) {
-    TDCC_HIP(hipEventRecord(ev[k], s));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    TDCC_HIP(hipEventCreate(&e0));
+    ev.push_back(e0);
+    TDCC_HIP(hipEventCreate(&e1));
+    ev.push_back(e1);
+    TDCC_HIP(hipEventRecord(e0, s));
     fn();
-    TDCC_HIP(hipEventRecord(ev[k + 1], s));
-    recs.push_back({name, flops, bytes, 0.f});
-    k += 2;
+    TDCC_HIP(hipEventRecord(e1, s));
+    recs.push_back({name, flops, bytes, 0.f, detail});
+    detail.clear();
   };
   try {
     for (const auto& L : launches_) {
@@ -813,6 +893,10 @@ void Graph::run_profiled(hipStream_t s, std::vector<LaunchRecord>& out) const {
           char nm[96];
           std::snprintf(nm, sizeof nm, "gett_kernel<%d,%d,%d,%d>%s", (int)L->ac, (int)L->bc, (int)L->akc, (int)L->bkc,
                         L->g.splits > 1 ? "+splitk" : "");
+          char dt[160];
+          std::snprintf(dt, sizeof dt, "%s M=%d N=%d K=%d B=%d S=%d", nodes_[L->node].spec.c_str(), L->g.M, L->g.N,
+                        L->g.K, L->g.batch, L->g.splits);
+          detail = dt;
           timed(nm, fl, 0.0, [&] {
             launch_gett(L->g, L->ac, L->bc, L->akc, L->bkc, s);
             if (L->g.splits > 1) launch_gett_reduce(L->g, L->c_cplx, s);
@@ -822,6 +906,20 @@ void Graph::run_profiled(hipStream_t s, std::vector<LaunchRecord>& out) const {
         case Launch::LIN: {
           const double bytes = 8.0 * L->l.total * ((L->l.out_im ? 2 : 1) * (1 + L->l.accumulate) + 2 * L->l.nterms);
           timed("lincomb_kernel", 0.0, bytes, [&] { launch_lincomb(L->l, s); });
+          break;
+        }
+        case Launch::LADDER: {
+          const LadderArgs& a = L->lad;
+          const double elems = 2.0 * a.rows * a.v * a.v;
+          timed("pack_pairs_kernel", 0.0, 8.0 * (elems + 2.0 * a.rows * a.P),
+                [&] { launch_pack_pairs(a.u_re, a.u_im, a.up, a.rows, a.v, s); });
+          const DgemmArgs g = ladder_gemm(a);
+          char dt[96];
+          std::snprintf(dt, sizeof dt, "packed ladder M=%d N=%d K=%d", g.M, g.N, g.K);
+          detail = dt;
+          timed("dgemm_kernel<BT>", 2.0 * g.M * (double)g.N * g.K, 0.0, [&] { launch_dgemm(g, true, s); });
+          timed("unpack_pairs_kernel", 0.0, 8.0 * (elems + 2.0 * a.rows * a.P),
+                [&] { launch_unpack_pairs(a.rp, a.out_re, a.out_im, a.rows, a.v, 1.0, s); });
           break;
         }
         case Launch::FIELD:
@@ -853,6 +951,9 @@ int64_t Graph::contract_flops() const {
     const int64_t mul = (L->ac && L->bc) ? 8 : ((L->ac || L->bc) ? 4 : 2);
     f += mul * (int64_t)L->g.M * L->g.N * L->g.K * L->g.batch;
   }
+  const int64_t P = (int64_t)v_ * (v_ - 1) / 2;
+  for (size_t i = 0; i < nodes_.size(); ++i)
+    if (nodes_[i].op == Op::Ladder && i < live_.size() && live_[i]) f += 2 * (2 * (int64_t)o_ * o_) * P * P;
   return f;
 }
 

@@ -37,7 +37,7 @@ struct Dev {  // device data of a tensor: planes + element strides
   int64_t st[4] = {0, 0, 0, 0};
 };
 
-enum class Op { External, View, Contract, Lincomb, Field, Finalize, Trace, Assign };
+enum class Op { External, View, Contract, Lincomb, Field, Finalize, Trace, Assign, Ladder };
 
 struct Term {
   int src;
@@ -94,6 +94,9 @@ struct Runtime {
   int64_t* counter = nullptr;
   double* mu_out = nullptr;         // [steps][3][2]
   double dt6 = 0.0, round_scale = 1e15;
+  // packed antisymmetric <ab||ef>: vpack[0][q(ab)][q(ef)] = <ab||ef>,
+  // vpack[1][q(ab)][q(ef)] = <ef||ab> (P = v(v-1)/2 pairs a < b)
+  const double* vpack[2] = {nullptr, nullptr};
 };
 
 class Graph {
@@ -117,6 +120,11 @@ class Graph {
   int contract3(const std::string& spec, int a, int b, int c, double ar = 1.0);
   int lincomb(std::vector<Term> terms);
   int field(int stage);
+  // Particle-particle ladder over the antisymmetric integral block:
+  //   ket == 0: out_ijab = sum_ef <ab||ef> u_ijef   ('abef,ijef->ijab')
+  //   ket == 1: out_ijab = sum_ef <ef||ab> u_ijef   ('efab,ijef->ijab')
+  // computed over pairs e < f and a < b (exact for antisymmetrized integrals).
+  int ladder(int u, int ket);
   int ones_view(const Shape& s);                     // all-ones tensor (stride 0)
   void finalize_rk(int y_ext, int base_ext, const int k[4], bool lambda);
   void trace(int dij, int l1, int dai, int dab);
@@ -134,6 +142,7 @@ class Graph {
     double flops;       // algorithmic real flops of the launch (contractions)
     double bytes;       // algorithmic bytes moved (elementwise kernels)
     float ms;
+    std::string detail; // contraction string and M, N, K, batch, splits
   };
   void run_profiled(hipStream_t s, std::vector<LaunchRecord>& out) const;
 

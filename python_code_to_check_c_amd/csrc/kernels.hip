@@ -540,3 +540,240 @@ void launch_join_complex(const double* re, const double* im, double* inter, int6
 }
 
 }  // namespace tdcc
+
+namespace tdcc {
+
+// ===========================================================================
+// Dense real DGEMM, 128 x 128 x 16 tiles on v_mfma_f64_16x16x4_f64.
+// 256 threads = 4 waves (2 x 2), each wave owns 64 x 64 = 4 x 4 MFMA blocks.
+// Global -> registers -> LDS double buffering, one barrier per K step.
+// ===========================================================================
+constexpr int DG_BM = 128, DG_BN = 128, DG_BK = 16, DG_LD = 130;
+
+template <bool BT>
+__global__ __launch_bounds__(256) void dgemm_kernel(DgemmArgs g) {
+  __shared__ double As[2][DG_BK][DG_LD];
+  __shared__ double Bs[2][DG_BK][DG_LD];
+  const int tiles_m = (g.M + DG_BM - 1) / DG_BM, tiles_n = (g.N + DG_BN - 1) / DG_BN;
+  const int L = remap_tile(blockIdx.x, tiles_m * tiles_n);
+  const int tm = L % tiles_m, tn = L / tiles_m;
+  const int m0 = tm * DG_BM, n0 = tn * DG_BN;
+  const int tid = threadIdx.x;
+  // A: row ar, k offset ak (8 consecutive k); B likewise (BT) or k row bk, n offset bn (!BT)
+  const int ar = tid >> 1, ak = (tid & 1) * 8;
+  const int brow = BT ? (tid >> 1) : (tid >> 4);
+  const int bcol = BT ? (tid & 1) * 8 : (tid & 15) * 8;
+  double ra[8], rb[8];
+  auto load8 = [&](const double* p, bool row_ok, int c0, int cmax, double* r) {
+    if (row_ok && c0 + 8 <= cmax && ((reinterpret_cast<uintptr_t>(p + c0) & 15) == 0)) {
+      const double2* q = reinterpret_cast<const double2*>(p + c0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const double2 x = q[j];
+        r[2 * j] = x.x;
+        r[2 * j + 1] = x.y;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) r[j] = (row_ok && c0 + j < cmax) ? p[c0 + j] : 0.0;
+    }
+  };
+  auto load_global = [&](int k0) {
+    const int m = m0 + ar;
+    load8(g.A + (int64_t)(m < g.M ? m : 0) * g.lda, m < g.M, k0 + ak, g.K, ra);
+    if (BT) {
+      const int n = n0 + brow;
+      load8(g.B + (int64_t)(n < g.N ? n : 0) * g.ldb, n < g.N, k0 + bcol, g.K, rb);
+    } else {
+      const int k = k0 + brow;
+      load8(g.B + (int64_t)(k < g.K ? k : 0) * g.ldb, k < g.K, n0 + bcol, g.N, rb);
+    }
+  };
+  auto store_lds = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) As[buf][ak + j][ar] = ra[j];
+    if (BT) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) Bs[buf][bcol + j][brow] = rb[j];
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) Bs[buf][brow][bcol + j] = rb[j];
+    }
+  };
+  d4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+  const int lane = tid & 63, w = tid >> 6;
+  const int wm = (w & 1) * 64, wn = (w >> 1) * 64;
+  const int lr = lane & 15, lk = lane >> 4;
+  load_global(0);
+  store_lds(0);
+  __syncthreads();
+  int buf = 0;
+  for (int k0 = 0; k0 < g.K; k0 += DG_BK) {
+    const bool more = k0 + DG_BK < g.K;
+    if (more) load_global(k0 + DG_BK);
+#pragma unroll
+    for (int k4 = 0; k4 < DG_BK / 4; ++k4) {
+      const int kr = k4 * 4 + lk;
+      double a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[buf][kr][wm + i * 16 + lr];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = Bs[buf][kr][wn + j * 16 + lr];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    if (more) store_lds(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int col = n0 + wn + j * 16 + lr;
+    if (col >= g.N) continue;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm + i * 16 + lk + 4 * r;
+        if (row >= g.M) continue;
+        double* c = g.C + (int64_t)row * g.ldc + col;
+        const double x = g.alpha * acc[i][j][r];
+        *c = g.beta != 0.0 ? x + g.beta * *c : x;
+      }
+  }
+}
+
+void launch_dgemm(const DgemmArgs& g, bool b_trans, hipStream_t s) {
+  if (g.M <= 0 || g.N <= 0) return;
+  const int tiles = ((g.M + DG_BM - 1) / DG_BM) * ((g.N + DG_BN - 1) / DG_BN);
+  if (b_trans) hipLaunchKernelGGL(dgemm_kernel<true>, dim3(tiles), dim3(256), 0, s, g);
+  else hipLaunchKernelGGL(dgemm_kernel<false>, dim3(tiles), dim3(256), 0, s, g);
+}
+
+// ===========================================================================
+// Antisymmetric pair packing / unpacking of the trailing (v, v) index pair.
+// ===========================================================================
+__device__ __forceinline__ int64_t pair_index(int a, int b, int v) {
+  return (int64_t)a * v - (int64_t)a * (a + 1) / 2 + (b - a - 1);
+}
+
+__global__ void pack_pairs_kernel(const double* xr, const double* xi, double* X, int64_t rows, int v) {
+  const int64_t P = (int64_t)v * (v - 1) / 2, vv = (int64_t)v * v, total = 2 * rows * vv;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t pr = idx / vv;  // plane * rows + r
+    const int64_t ef = idx - pr * vv;
+    const int e = (int)(ef / v), f = (int)(ef - (int64_t)e * v);
+    if (e >= f) continue;
+    const int64_t r = pr % rows;
+    const double* x = (pr < rows) ? xr : xi;
+    const double val = __dsub_rn(x[r * vv + (int64_t)e * v + f], x[r * vv + (int64_t)f * v + e]);
+    X[pr * P + pair_index(e, f, v)] = val;
+  }
+}
+
+__global__ void unpack_pairs_kernel(const double* Y, double* yr, double* yi, int64_t rows, int v, double alpha) {
+  const int64_t P = (int64_t)v * (v - 1) / 2, vv = (int64_t)v * v, total = 2 * rows * vv;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t pr = idx / vv;
+    const int64_t ab = idx - pr * vv;
+    const int a = (int)(ab / v), b = (int)(ab - (int64_t)a * v);
+    double val = 0.0;
+    if (a < b) val = __dmul_rn(alpha, Y[pr * P + pair_index(a, b, v)]);
+    else if (a > b) val = -__dmul_rn(alpha, Y[pr * P + pair_index(b, a, v)]);
+    const int64_t r = pr % rows;
+    double* y = (pr < rows) ? yr : yi;
+    y[r * vv + ab] = val;
+  }
+}
+
+void launch_pack_pairs(const double* xr, const double* xi, double* X, int64_t rows, int v, hipStream_t s) {
+  hipLaunchKernelGGL(pack_pairs_kernel, dim3(grid_for(2 * rows * v * v)), dim3(256), 0, s, xr, xi, X, rows, v);
+}
+
+void launch_unpack_pairs(const double* Y, double* yr, double* yi, int64_t rows, int v, double alpha, hipStream_t s) {
+  hipLaunchKernelGGL(unpack_pairs_kernel, dim3(grid_for(2 * rows * v * v)), dim3(256), 0, s, Y, yr, yi, rows, v,
+                     alpha);
+}
+
+__global__ void build_vpack_kernel(const double* tei, int o, int v, int ket, double* Vp) {
+  const int64_t n = o + v, P = (int64_t)v * (v - 1) / 2, vv = (int64_t)v * v, total = vv * vv;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t abi = idx / vv, efi = idx - abi * vv;
+    const int a = (int)(abi / v), b = (int)(abi - (int64_t)a * v);
+    const int e = (int)(efi / v), f = (int)(efi - (int64_t)e * v);
+    if (a >= b || e >= f) continue;
+    const int64_t p = o + a, q = o + b, r = o + e, t = o + f;
+    const int64_t src = ket ? ((r * n + t) * n + p) * n + q : ((p * n + q) * n + r) * n + t;
+    Vp[pair_index(a, b, v) * P + pair_index(e, f, v)] = tei[src];
+  }
+}
+
+void launch_build_vpack(const double* tei, int o, int v, int ket, double* Vp, hipStream_t s) {
+  const int64_t total = (int64_t)v * v * v * v;
+  hipLaunchKernelGGL(build_vpack_kernel, dim3(grid_for(total)), dim3(256), 0, s, tei, o, v, ket, Vp);
+}
+
+// Deterministic counting: per-block partial counts, then one block sums them.
+
+__device__ void block_count_store(unsigned long long local, unsigned long long* partial) {
+  __shared__ unsigned long long red[256];
+  red[threadIdx.x] = local;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void antisym_check_kernel(const double* tei, int o, int v, unsigned long long* partial) {
+  const int64_t n = o + v, vv = (int64_t)v * v, total = vv * vv;
+  unsigned long long bad = 0;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t abi = idx / vv, efi = idx - abi * vv;
+    const int64_t a = o + abi / v, b = o + abi % v, e = o + efi / v, f = o + efi % v;
+    const double x = tei[((a * n + b) * n + e) * n + f];
+    const double y = tei[((b * n + a) * n + e) * n + f];
+    const double z = tei[((a * n + b) * n + f) * n + e];
+    bad += (x != -y) || (x != -z);
+  }
+  block_count_store(bad, partial);
+}
+
+__global__ void mismatch_kernel(const double* x, const double* y, int64_t n, unsigned long long* partial) {
+  unsigned long long bad = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    bad += x[i] != y[i];
+  block_count_store(bad, partial);
+}
+
+__global__ void sum_counts_kernel(const unsigned long long* partial, int nb, unsigned long long* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    unsigned long long s = 0;
+    for (int i = 0; i < nb; ++i) s += partial[i];
+    out[0] = s;
+  }
+}
+
+// `count` must hold kCountBlocks + 1 entries; the total lands in count[kCountBlocks].
+void launch_vvvv_antisym_check(const double* tei, int o, int v, unsigned long long* count, hipStream_t s) {
+  hipLaunchKernelGGL(antisym_check_kernel, dim3(kCountBlocks), dim3(256), 0, s, tei, o, v, count);
+  hipLaunchKernelGGL(sum_counts_kernel, dim3(1), dim3(64), 0, s, count, kCountBlocks, count + kCountBlocks);
+}
+
+void launch_count_mismatch(const double* x, const double* y, int64_t n, unsigned long long* count, hipStream_t s) {
+  hipLaunchKernelGGL(mismatch_kernel, dim3(kCountBlocks), dim3(256), 0, s, x, y, n, count);
+  hipLaunchKernelGGL(sum_counts_kernel, dim3(1), dim3(64), 0, s, count, kCountBlocks, count + kCountBlocks);
+}
+
+}  // namespace tdcc

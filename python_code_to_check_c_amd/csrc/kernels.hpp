@@ -113,6 +113,39 @@ void launch_synth_tei(double* tei, int n, uint32_t config_id, double sigma, hipS
 }  // namespace tdcc
 
 namespace tdcc {
+// ---------------------------------------------------------------------------
+// Dense real GEMM on FP64 MFMA: C = alpha * A * op(B) + beta * C, all row-major.
+// A is M x K (lda); op(B) = B^T with B stored N x K (ldb) when b_trans, else
+// B stored K x N (ldb); C is M x N (ldc).
+struct DgemmArgs {
+  const double* A; const double* B; double* C;
+  int M, N, K;
+  int64_t lda, ldb, ldc;
+  double alpha, beta;
+};
+void launch_dgemm(const DgemmArgs& g, bool b_trans, hipStream_t s);
+
+// Antisymmetric pair packing of the virtual-virtual index pair (e < f,
+// lexicographic; P = v (v - 1) / 2):
+//   pack:   X[p * rows + r][q] = x[p][r][e_q][f_q] - x[p][r][f_q][e_q]
+//           for planes p of a complex (rows, v, v) tensor, written as a
+//           (2 rows) x P real matrix;
+//   unpack: y[p][r][a][b] = alpha * Y[p * rows + r][q(a,b)] for a < b,
+//           -alpha * Y[...][q(b,a)] for a > b, 0 for a == b.
+void launch_pack_pairs(const double* x_re, const double* x_im, double* X, int64_t rows, int v, hipStream_t s);
+void launch_unpack_pairs(const double* Y, double* y_re, double* y_im, int64_t rows, int v, double alpha,
+                         hipStream_t s);
+// Packed <ab||ef> of the virtual block of a (n,n,n,n) TEI with o occupied:
+//   ladder_ket == 0: Vp[q(a,b)][q(e,f)] = <ab||ef>
+//   ladder_ket == 1: Vp[q(a,b)][q(e,f)] = <ef||ab>
+void launch_build_vpack(const double* tei, int o, int v, int ladder_ket, double* Vp, hipStream_t s);
+// Counts entries of the virtual block violating <ab||ef> = -<ba||ef> = -<ab||fe>.
+// `count` holds kCountBlocks + 1 entries; the total lands in count[kCountBlocks].
+constexpr int kCountBlocks = 1024;
+void launch_vvvv_antisym_check(const double* tei, int o, int v, unsigned long long* count, hipStream_t s);
+// Counts mismatches between two equally shaped arrays.
+void launch_count_mismatch(const double* x, const double* y, int64_t n, unsigned long long* count, hipStream_t s);
+
 // Interleaved complex <-> two planes.
 void launch_split_complex(const double* inter, double* re, double* im, int64_t n, hipStream_t s);
 void launch_join_complex(const double* re, const double* im, double* inter, int64_t n, hipStream_t s);
