@@ -112,6 +112,7 @@ def main():
     ap.add_argument("--tei-sigma", type=float, default=0.005)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--dump-profile", default=None, help="write every profiled launch (tsv) to this file")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding of this "
@@ -164,6 +165,11 @@ def main():
     }
     if not args.no_profile:
         recs = ctx.profile_step(args.method, t, dt, w0, A, pol, precs)
+        if args.dump_profile:
+            with open(args.dump_profile, "w") as f:
+                f.write("kernel\tflops\tbytes\tms\tdetail\n")
+                for r in sorted(recs, key=lambda r: -r[3]):
+                    f.write("\t".join(str(x) for x in r) + "\n")
         name, roof, groups = roofline(recs)
         out["roofline"] = roof
         top = sorted(groups.items(), key=lambda kv: -kv[1][2])[:6]

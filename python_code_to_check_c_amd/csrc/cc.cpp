@@ -384,7 +384,7 @@ int CCBuilder::L2_vvvv_term(int t1, int t2, int l2) {
 
 int CCBuilder::L1_wefam_term(int t1, int t2, int l2, int F) {
   // sum_efm Wefam_efam lam_imef with Wefam = LRWefam (CCSD_Helper.py:488-505)
-  const int V = blk("vvvv"), vovv = blk("vovv"), oovv = blk("oovv");
+  const int oovv = blk("oovv");
   const int fme = Fme(t1, t2, F);
   const int tau = L(Terms().add(t2, 0.25).add(t1t1(t1), 0.5));
   const int ring_a = L(Terms().add(blk("ovvo"), -1.0).add(C("jnab,nmfb->jfam", oovv, t2)));
@@ -393,13 +393,10 @@ int CCBuilder::L1_wefam_term(int t1, int t2, int l2, int F) {
   // 0.5 Fme_na t_mnef
   const int k = C("mnef,imef->in", t2, l2);
   const int c2 = C("na,in->ia", fme, k);
-  // LSWabef_efab t_mb, with R_ibef = sum_m t_mb lam_imef
-  const int r = C("mb,imef->ibef", t1, l2);
-  const int c3a = C("efab,ibef->ia", V, r);
-  const int s = C("mf,ibef->ibem", t1, r);
-  const int c3b = C("emab,ibem->ia", vovv, s);
-  const int tm_ = C("nmef,ibef->nmib", tau, r);
-  const int c3c = C("nmab,nmib->ia", oovv, tm_);
+  // sum_efm sum_b LSWabef_efab t_mb lam_imef = sum_mb t_mb W_imab with
+  // W_imab = sum_ef LSWabef_efab lam_imef, the <ef||ab> term of lam2eq_rhs at
+  // this lam2 (the same node the L2 chain evaluates at its first stage)
+  const int c3 = C("imab,mb->ia", L2_vvvv_term(t1, t2, l2), t1);
   // ring_a_jfam t_je
   const int u = C("je,imef->ijmf", t1, l2);
   const int c4 = C("jfam,ijmf->ia", ring_a, u);
@@ -410,7 +407,7 @@ int CCBuilder::L1_wefam_term(int t1, int t2, int l2, int F) {
   const int w = C("jmeb,imef->jbif", t2, l2);
   const int c6 = C("jfab,jbif->ia", blk("ovvv"), w);
   Terms t;
-  t.add(c1, 0.5).add(c2, 0.5).add(c3a, 0.5).add(c3b, -1.0).add(c3c).add(c4).add(c5).add(c6, -1.0);
+  t.add(c1, 0.5).add(c2, 0.5).add(c3).add(c4).add(c5).add(c6, -1.0);
   return L(t);
 }
 

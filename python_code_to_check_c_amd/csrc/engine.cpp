@@ -45,10 +45,27 @@ struct LadderArgs {
   int64_t P = 0;
 };
 
+// A contraction lowered to dense GEMM form: gather A' (planes stacked along
+// rows, pa*M x K), gather B' (pb*N x K when bt, else K x pb*N), one DGEMM
+// (optionally split over K) into Z (pa*M x pb*N per split), scatter into C.
+struct GemmPath {
+  bool on = false;
+  bool bt = false;
+  int pa = 1, pb = 1;
+  int splits = 1;
+  int64_t k_chunk = 0;
+  int64_t a_off = 0, b_off = 0, z_off = 0;  // element offsets inside the node scratch
+  int64_t ldb = 0;
+};
+
 struct Graph::Launch {
   enum Kind { GETT, LIN, FIELD, FIN, TRACE, LADDER } kind;
   int node = -1;
   LadderArgs lad{};
+  GemmPath gp{};
+  Gather2dArgs ga{}, gb{};
+  DgemmArgs dg{};
+  GemmScatterArgs sc{};
   GettArgs g{};
   bool ac = false, bc = false, akc = false, bkc = false, c_cplx = false;
   int64_t tab[9] = {0, 0, 0, 0, 0, 0, -1, -1, -1};  // am ak bn bk cm cn ab bb cb offsets into tables
@@ -171,6 +188,13 @@ static void parse_spec(const std::string& spec, std::vector<std::string>& ins, s
 }
 
 static bool has(const std::string& s, char c) { return s.find(c) != std::string::npos; }
+
+// Contractions with at least this many flops (and GEMM-sized groups) take the
+// dense-GEMM path; TDCC_GEMM_MIN_FLOPS overrides it (0: every batch-free one).
+static double gemm_min_flops() {
+  const char* e = std::getenv("TDCC_GEMM_MIN_FLOPS");
+  return e ? std::atof(e) : 2e9;
+}
 
 int Graph::contract(const std::string& spec, int a, int b, double ar, double ai) {
   std::vector<std::string> ins;
@@ -576,13 +600,38 @@ void Graph::lower_contract(int id) {
     }
   }
   std::string gb, gm, gn, gk;
-  for (char c : lc) {
-    if (has(la, c) && has(lb, c)) gb.push_back(c);
-    else if (has(la, c)) gm.push_back(c);
-    else gn.push_back(c);
+  auto groups = [&] {
+    gb.clear();
+    gm.clear();
+    gn.clear();
+    gk.clear();
+    for (char c : lc) {
+      if (has(la, c) && has(lb, c)) gb.push_back(c);
+      else if (has(la, c)) gm.push_back(c);
+      else gn.push_back(c);
+    }
+    for (char c : la)
+      if (has(lb, c) && !has(lc, c)) gk.push_back(c);
+  };
+  groups();
+  auto gsize = [&](const std::string& ls) {
+    int64_t s = 1;
+    for (char c : ls) s *= ext[c];
+    return s;
+  };
+  // Large contractions run as dense GEMMs on the DGEMM kernel; the complex
+  // operand (if only one is complex) goes on the row side, planes stacked.
+  const double mul0 = (nodes_[a].cplx && nodes_[b].cplx) ? 8 : ((nodes_[a].cplx || nodes_[b].cplx) ? 4 : 2);
+  const double gflops = mul0 * gsize(gm) * (double)gsize(gn) * gsize(gk);
+  const double min_flops = gemm_min_flops();
+  const bool use_gemm = gb.empty() && gflops >= min_flops &&
+                        (min_flops == 0.0 || (gsize(gm) >= 64 && gsize(gn) >= 64 && gsize(gk) >= 32));
+  if (use_gemm && !nodes_[a].cplx && nodes_[b].cplx) {
+    std::swap(la, lb);
+    std::swap(a, b);
+    std::swap(sa, sb);
+    groups();
   }
-  for (char c : la)
-    if (has(lb, c) && !has(lc, c)) gk.push_back(c);
   auto innermost = [&](const std::string& ls, const std::map<char, int64_t>& st) -> char {
     char best = 0;
     int64_t bs = -1;
@@ -670,6 +719,30 @@ void Graph::lower_contract(int id) {
   n.a = a;  // orientation used by the launch
   n.b = b;
   n.scratch_len = splits > 1 ? (int64_t)splits * Bt * M * Nn * (L->ac || L->bc ? 2 : 1) * (int64_t)sizeof(double) : 0;
+  if (use_gemm) {
+    GemmPath& gp = L->gp;
+    gp.on = true;
+    gp.pa = L->ac ? 2 : 1;
+    gp.pb = L->bc ? 2 : 1;
+    gp.bt = bkc;
+    const int64_t R = gp.pa * M, NB = gp.pb * Nn;
+    const int64_t lda = K + (K & 1);
+    gp.ldb = gp.bt ? lda : NB + (NB & 1);
+    const int64_t tiles = ((R + 127) / 128) * ((NB + 127) / 128);
+    int64_t sp = 1;
+    if (tiles < 240 && K >= 2048) sp = std::min<int64_t>((K + 1023) / 1024, (480 + tiles - 1) / tiles);
+    if (const char* fs = std::getenv("TDCC_GEMM_SPLITS")) sp = std::max<int64_t>(1, std::min<int64_t>(std::atol(fs), (K + 15) / 16));
+    int64_t chunk = (K + sp - 1) / sp;
+    chunk = (chunk + 15) / 16 * 16;
+    sp = (K + chunk - 1) / chunk;
+    gp.splits = (int)std::max<int64_t>(sp, 1);
+    gp.k_chunk = gp.splits > 1 ? chunk : 0;
+    auto al = [](int64_t x) { return (x + 31) / 32 * 32; };  // 256-byte alignment in doubles
+    gp.a_off = 0;
+    gp.b_off = al(R * lda);
+    gp.z_off = gp.b_off + al(gp.bt ? NB * lda : K * gp.ldb);
+    n.scratch_len = (gp.z_off + al((int64_t)gp.splits * R * NB)) * (int64_t)sizeof(double);
+  }
   L->node = id;
   launches_.push_back(std::move(L));
 }
@@ -709,6 +782,73 @@ void Graph::bind(double* arena) {
         g.bb = L->tab[7] >= 0 ? dev_tables_ + L->tab[7] : nullptr;
         g.cb = L->tab[8] >= 0 ? dev_tables_ + L->tab[8] : nullptr;
         g.partial = n.scratch_len > 0 ? arena_ + n.scratch_off / (int64_t)sizeof(double) : nullptr;
+        if (L->gp.on) {
+          const GemmPath& gp = L->gp;
+          double* scr = arena_ + n.scratch_off / (int64_t)sizeof(double);
+          const int64_t M = g.M, Nn = g.N, K = g.K;
+          const int64_t lda = K + (K & 1);
+          Gather2dArgs& ga = L->ga;  // A' = [A_re; A_im] rows x K
+          ga.src_re = A.re;
+          ga.src_im = L->ac ? A.im : nullptr;
+          ga.rtab = g.am;
+          ga.ctab = g.ak;
+          ga.Rb = M;
+          ga.Cb = K;
+          ga.planes = gp.pa;
+          ga.planes_along_rows = 1;
+          ga.dst = scr + gp.a_off;
+          ga.ld = lda;
+          Gather2dArgs& gbb = L->gb;
+          gbb.src_re = B.re;
+          gbb.src_im = L->bc ? B.im : nullptr;
+          gbb.planes = gp.pb;
+          gbb.dst = scr + gp.b_off;
+          gbb.ld = gp.ldb;
+          if (gp.bt) {  // B' = [B_re; B_im] (pb*N) x K
+            gbb.rtab = g.bn;
+            gbb.ctab = g.bk;
+            gbb.Rb = Nn;
+            gbb.Cb = K;
+            gbb.planes_along_rows = 1;
+          } else {      // B' = [B_re | B_im] K x (pb*N)
+            gbb.rtab = g.bk;
+            gbb.ctab = g.bn;
+            gbb.Rb = K;
+            gbb.Cb = Nn;
+            gbb.planes_along_rows = 0;
+          }
+          DgemmArgs& dg = L->dg;
+          dg.A = ga.dst;
+          dg.B = gbb.dst;
+          dg.C = scr + gp.z_off;
+          dg.M = (int)(gp.pa * M);
+          dg.N = (int)(gp.pb * Nn);
+          dg.K = (int)K;
+          dg.lda = lda;
+          dg.ldb = gp.ldb;
+          dg.ldc = gp.pb * Nn;
+          dg.alpha = 1.0;
+          dg.beta = 0.0;
+          dg.k_chunk = (int)gp.k_chunk;
+          dg.splits = gp.splits;
+          dg.c_split_stride = (int64_t)dg.M * dg.ldc;
+          GemmScatterArgs& sc = L->sc;
+          sc.Z = dg.C;
+          sc.ldz = dg.ldc;
+          sc.splits = gp.splits;
+          sc.z_stride = dg.c_split_stride;
+          sc.M = (int)M;
+          sc.N = (int)Nn;
+          sc.kind = (L->ac && L->bc) ? 2 : (L->ac ? 1 : 0);
+          if (!L->ac && L->bc) throw Error("internal: complex operand not on the row side");
+          sc.cm = g.cm;
+          sc.cn = g.cn;
+          sc.out_re = C.re;
+          sc.out_im = n.cplx ? C.im : nullptr;
+          sc.alpha_re = n.ar;
+          sc.alpha_im = n.ai;
+          sc.accumulate = 0;
+        }
         launches_.push_back(std::move(L));
         break;
       }
@@ -853,6 +993,13 @@ void Graph::run(hipStream_t s) const {
     switch (L->kind) {
       case Launch::GETT:
         if (L->g.M == 0 || L->g.N == 0) break;
+        if (L->gp.on) {
+          launch_gather2d(L->ga, s);
+          launch_gather2d(L->gb, s);
+          launch_dgemm(L->dg, L->gp.bt, s);
+          launch_gemm_scatter(L->sc, s);
+          break;
+        }
         launch_gett(L->g, L->ac, L->bc, L->akc, L->bkc, s);
         if (L->g.splits > 1) launch_gett_reduce(L->g, L->c_cplx, s);
         break;
@@ -894,6 +1041,20 @@ void Graph::run_profiled(hipStream_t s, std::vector<LaunchRecord>& out) const {
           std::snprintf(nm, sizeof nm, "gett_kernel<%d,%d,%d,%d>%s", (int)L->ac, (int)L->bc, (int)L->akc, (int)L->bkc,
                         L->g.splits > 1 ? "+splitk" : "");
           char dt[160];
+          if (L->gp.on) {
+            const double gbytes = 8.0 * ((double)L->ga.planes * L->ga.Rb * L->ga.Cb) * 2;
+            timed("gather2d_kernel", 0.0, gbytes, [&] { launch_gather2d(L->ga, s); });
+            const double bbytes = 8.0 * ((double)L->gb.planes * L->gb.Rb * L->gb.Cb) * 2;
+            timed("gather2d_kernel", 0.0, bbytes, [&] { launch_gather2d(L->gb, s); });
+            std::snprintf(dt, sizeof dt, "%s rows=%d N=%d K=%d splits=%d", nodes_[L->node].spec.c_str(), L->dg.M,
+                          L->dg.N, L->dg.K, L->gp.splits);
+            detail = dt;
+            timed(L->gp.bt ? "dgemm_kernel<BT>" : "dgemm_kernel<NN>", fl, 0.0,
+                  [&] { launch_dgemm(L->dg, L->gp.bt, s); });
+            const double sbytes = 8.0 * L->sc.M * (double)L->sc.N * (L->sc.kind == 2 ? 4 : 2) * L->sc.splits;
+            timed("gemm_scatter_kernel", 0.0, sbytes, [&] { launch_gemm_scatter(L->sc, s); });
+            break;
+          }
           std::snprintf(dt, sizeof dt, "%s M=%d N=%d K=%d B=%d S=%d", nodes_[L->node].spec.c_str(), L->g.M, L->g.N,
                         L->g.K, L->g.batch, L->g.splits);
           detail = dt;

@@ -550,7 +550,7 @@ namespace tdcc {
 // ===========================================================================
 constexpr int DG_BM = 128, DG_BN = 128, DG_BK = 16, DG_LD = 130;
 
-template <bool BT>
+template <bool BT, bool VEC>
 __global__ __launch_bounds__(256) void dgemm_kernel(DgemmArgs g) {
   __shared__ double As[2][DG_BK][DG_LD];
   __shared__ double Bs[2][DG_BK][DG_LD];
@@ -564,8 +564,12 @@ __global__ __launch_bounds__(256) void dgemm_kernel(DgemmArgs g) {
   const int brow = BT ? (tid >> 1) : (tid >> 4);
   const int bcol = BT ? (tid & 1) * 8 : (tid & 15) * 8;
   double ra[8], rb[8];
+  // split-K range of this block and its output slice
+  const int kb = g.k_chunk > 0 ? (int)blockIdx.y * g.k_chunk : 0;
+  const int ke = g.k_chunk > 0 ? min(g.K, kb + g.k_chunk) : g.K;
+  double* const Cout = g.C + (int64_t)blockIdx.y * g.c_split_stride;
   auto load8 = [&](const double* p, bool row_ok, int c0, int cmax, double* r) {
-    if (row_ok && c0 + 8 <= cmax && ((reinterpret_cast<uintptr_t>(p + c0) & 15) == 0)) {
+    if (VEC && row_ok && c0 + 8 <= cmax) {
       const double2* q = reinterpret_cast<const double2*>(p + c0);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
@@ -580,13 +584,13 @@ __global__ __launch_bounds__(256) void dgemm_kernel(DgemmArgs g) {
   };
   auto load_global = [&](int k0) {
     const int m = m0 + ar;
-    load8(g.A + (int64_t)(m < g.M ? m : 0) * g.lda, m < g.M, k0 + ak, g.K, ra);
+    load8(g.A + (int64_t)(m < g.M ? m : 0) * g.lda, m < g.M, k0 + ak, ke, ra);
     if (BT) {
       const int n = n0 + brow;
-      load8(g.B + (int64_t)(n < g.N ? n : 0) * g.ldb, n < g.N, k0 + bcol, g.K, rb);
+      load8(g.B + (int64_t)(n < g.N ? n : 0) * g.ldb, n < g.N, k0 + bcol, ke, rb);
     } else {
       const int k = k0 + brow;
-      load8(g.B + (int64_t)(k < g.K ? k : 0) * g.ldb, k < g.K, n0 + bcol, g.N, rb);
+      load8(g.B + (int64_t)(k < ke ? k : 0) * g.ldb, k < ke, n0 + bcol, g.N, rb);
     }
   };
   auto store_lds = [&](int buf) {
@@ -608,12 +612,12 @@ __global__ __launch_bounds__(256) void dgemm_kernel(DgemmArgs g) {
   const int lane = tid & 63, w = tid >> 6;
   const int wm = (w & 1) * 64, wn = (w >> 1) * 64;
   const int lr = lane & 15, lk = lane >> 4;
-  load_global(0);
+  load_global(kb);
   store_lds(0);
   __syncthreads();
   int buf = 0;
-  for (int k0 = 0; k0 < g.K; k0 += DG_BK) {
-    const bool more = k0 + DG_BK < g.K;
+  for (int k0 = kb; k0 < ke; k0 += DG_BK) {
+    const bool more = k0 + DG_BK < ke;
     if (more) load_global(k0 + DG_BK);
 #pragma unroll
     for (int k4 = 0; k4 < DG_BK / 4; ++k4) {
@@ -642,7 +646,7 @@ __global__ __launch_bounds__(256) void dgemm_kernel(DgemmArgs g) {
       for (int r = 0; r < 4; ++r) {
         const int row = m0 + wm + i * 16 + lk + 4 * r;
         if (row >= g.M) continue;
-        double* c = g.C + (int64_t)row * g.ldc + col;
+        double* c = Cout + (int64_t)row * g.ldc + col;
         const double x = g.alpha * acc[i][j][r];
         *c = g.beta != 0.0 ? x + g.beta * *c : x;
       }
@@ -652,8 +656,18 @@ __global__ __launch_bounds__(256) void dgemm_kernel(DgemmArgs g) {
 void launch_dgemm(const DgemmArgs& g, bool b_trans, hipStream_t s) {
   if (g.M <= 0 || g.N <= 0) return;
   const int tiles = ((g.M + DG_BM - 1) / DG_BM) * ((g.N + DG_BN - 1) / DG_BN);
-  if (b_trans) hipLaunchKernelGGL(dgemm_kernel<true>, dim3(tiles), dim3(256), 0, s, g);
-  else hipLaunchKernelGGL(dgemm_kernel<false>, dim3(tiles), dim3(256), 0, s, g);
+  const int splits = g.k_chunk > 0 ? g.splits : 1;
+  // 16-byte vector loads need even leading dimensions and 16-byte aligned bases
+  const bool vec = (g.lda % 2 == 0) && (g.ldb % 2 == 0) && ((reinterpret_cast<uintptr_t>(g.A) & 15) == 0) &&
+                   ((reinterpret_cast<uintptr_t>(g.B) & 15) == 0);
+  const dim3 grid(tiles, splits);
+  if (b_trans) {
+    if (vec) hipLaunchKernelGGL((dgemm_kernel<true, true>), grid, dim3(256), 0, s, g);
+    else hipLaunchKernelGGL((dgemm_kernel<true, false>), grid, dim3(256), 0, s, g);
+  } else {
+    if (vec) hipLaunchKernelGGL((dgemm_kernel<false, true>), grid, dim3(256), 0, s, g);
+    else hipLaunchKernelGGL((dgemm_kernel<false, false>), grid, dim3(256), 0, s, g);
+  }
 }
 
 // ===========================================================================

@@ -122,6 +122,11 @@ struct DgemmArgs {
   int M, N, K;
   int64_t lda, ldb, ldc;
   double alpha, beta;
+  // split-K: grid.y = splits, split s covers k in [s*k_chunk, (s+1)*k_chunk)
+  // and writes C + s * c_split_stride (k_chunk == 0: one split over all K)
+  int k_chunk = 0;
+  int splits = 1;
+  int64_t c_split_stride = 0;
 };
 void launch_dgemm(const DgemmArgs& g, bool b_trans, hipStream_t s);
 
@@ -145,6 +150,32 @@ constexpr int kCountBlocks = 1024;
 void launch_vvvv_antisym_check(const double* tei, int o, int v, unsigned long long* count, hipStream_t s);
 // Counts mismatches between two equally shaped arrays.
 void launch_count_mismatch(const double* x, const double* y, int64_t n, unsigned long long* count, hipStream_t s);
+
+// Dense GEMM form of a contraction: operand gather into row-major panels
+// and result scatter (see kernels_gemm.hip).
+struct Gather2dArgs {
+  const double* src_re; const double* src_im;
+  const int64_t* rtab; const int64_t* ctab;
+  int64_t Rb, Cb;        // base rows / columns (per plane)
+  int planes;            // 1 or 2
+  int planes_along_rows; // planes stacked along rows (1) or columns (0)
+  double* dst;
+  int64_t ld;
+};
+void launch_gather2d(const Gather2dArgs& g, hipStream_t s);
+struct GemmScatterArgs {
+  const double* Z;
+  int64_t ldz;
+  int splits;            // Z holds `splits` partial results z_stride apart, summed in order
+  int64_t z_stride;
+  int M, N;
+  int kind;              // 0 real x real, 1 complex x real, 2 complex x complex
+  const int64_t* cm; const int64_t* cn;
+  double* out_re; double* out_im;
+  double alpha_re, alpha_im;
+  int accumulate;
+};
+void launch_gemm_scatter(const GemmScatterArgs& g, hipStream_t s);
 
 // Interleaved complex <-> two planes.
 void launch_split_complex(const double* inter, double* re, double* im, int64_t n, hipStream_t s);

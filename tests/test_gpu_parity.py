@@ -23,14 +23,34 @@ RHS = [("T1eq_rhs", _lib.RHS_T1), ("T2eq_rhs", _lib.RHS_T2), ("lam_1eq_rhs", _li
        ("L2eq_rhs_CC2", _lib.RHS_L2_CC2)]
 
 
-@pytest.fixture(scope="module", params=["o4v6", "c1_o10v4"])
+# Engine modes: the default lowering (small contractions on the strided MFMA
+# kernel), every contraction through the dense-GEMM path, and the dense-GEMM
+# path with its K dimension split over 3 partial products.
+MODES = {"default": {}, "gemm": {"TDCC_GEMM_MIN_FLOPS": "0"},
+         "gemm_splitk": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_GEMM_SPLITS": "3"}}
+
+
+@pytest.fixture(scope="module", params=[(n, m) for m in MODES for n in ("o4v6", "c1_o10v4")],
+                ids=lambda p: f"{p[0]}-{p[1]}")
 def case(request):
-    g = dict(np.load(os.path.join(HERE, "golden", f"golden_{request.param}.npz")))
-    print(f"\n[case {request.param}]", flush=True)
+    name, mode = request.param
+    g = dict(np.load(os.path.join(HERE, "golden", f"golden_{name}.npz")))
+    print(f"\n[case {name} engine={mode}]", flush=True)
+    saved = {k: os.environ.get(k) for k in ("TDCC_GEMM_MIN_FLOPS", "TDCC_GEMM_SPLITS")}
+    for k in saved:
+        os.environ.pop(k, None)
+    os.environ.update(MODES[mode])
     ctx = _lib.Context(int(g["o"]), int(g["v"]))
     ctx.set_system(g["TEI"], g["F"], g["mu"])
-    yield g, ctx
-    ctx.close()
+    try:
+        yield g, ctx
+    finally:
+        ctx.close()
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
 
 
 @pytest.mark.parametrize("name", INTERMEDIATES)
