@@ -13,8 +13,6 @@ Modules:
                 time-advance semantics (CCSD_Helper.py:1158-1256,
                 CC2_Helper.py:412-500)
   ground_state  Jacobi + DIIS solver for the ground-state T and Lambda inputs
-  restart_io    the reference's restart/output file layout
-                (CCSD_Helper.py:1102-1145, CC_Calculator.py:70-145)
   fci           brute-force two-electron FCI used to pin the CCSD fixed point
   flops         symbolic evaluation of the residuals that counts FLOPs per RK4
                 step (as written, and with stage/chain de-duplication)
