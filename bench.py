@@ -5,17 +5,24 @@
 Workload (defaults): real-time TDCCSD at o = 20, v = 160 spin orbitals, the
 reference's partial-RK4 propagator (four decoupled chains, 16 right-hand sides
 per step, rounding to 15 decimals, dipole after every step), dt = 0.001,
-z field with w0 = 0.923, A = 0.002, FP64 on the GPU.  Inputs are synthetic
-(seeded Philox generator; the TEI is generated on the device), the initial
-state is the MP2 guess for t2 with t1 = 0 and lam = t (the ground-state solve
-is not part of the timed path).
+z field with w0 = 0.923, A = 0.002.  Inputs are synthetic (seeded Philox
+generator; the TEI is generated on the device), the initial state is the MP2
+guess for t2 with t1 = 0 and lam = t (the ground-state solve is not part of the
+timed path).
 
-Prints one JSON line.  `value` is steps/s of the timed region (inputs resident
-in HBM, graph-replayed steps, synchronised on both sides).  `roofline` comes
-from one eager profiled step after the timed region (HIP events around every
-launch): the dominant kernel is the kernel name with the largest summed time.
-`cpu_baseline` times the oracle (numpy complex128, the reference's einsum
-strings) on a bounded sample and scales it to the workload by F_ref.
+Two legs run from the same initial state at t = 0: the FP64 path, then the
+split-precision path (--digits int8 digits per operand for the large
+contractions; FP64 state).  `value` is steps/s of the split leg (of the FP64 leg
+with --digits 0); `config` carries the FP64 rate, the speedup and the max
+|mu_z| difference between the legs over the timed steps (--error-steps N adds
+an N-step comparison).  Each timed region is graph-replayed steps with the
+inputs resident in HBM, synchronised on both sides.
+
+`roofline` comes from one eager profiled step after the timed region (HIP
+events around every launch): the dominant kernel is the kernel name with the
+largest summed time.  `cpu_baseline` times the oracle (numpy complex128, the
+reference's einsum strings) on a bounded sample and scales it to the workload
+by F_ref.
 """
 from __future__ import annotations
 
@@ -33,6 +40,7 @@ sys.path.insert(0, ROOT)
 METRIC = "TDCCSD RK4 steps/s (T+Λ) o=20,v=160: FP32-split vs FP64 at dipole err≤1e-8"
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+PEAK_INT8_TOPS = 5000.0
 
 
 def initial_state(sysm, o, v):
@@ -78,7 +86,10 @@ def cpu_baseline(method, o_full, v_full, budget_s=20.0):
                       f"by F_ref {f_small:.3e} -> {f_full:.3e} flop/step"}
 
 
-def roofline(records):
+def roofline(records, digits):
+    """Dominant kernel (largest summed time) of one profiled step.  MFMA kernels
+    are priced against the FP64 matrix peak, or, for the int8 digit GEMM,
+    against the int8 dense peak divided by the digit products per FP64 product."""
     groups = {}
     for name, fl, by, ms, _ in records:
         g = groups.setdefault(name, [0.0, 0.0, 0.0, 0])
@@ -90,10 +101,15 @@ def roofline(records):
     avg_ms = ms / calls
     if fl > 0:
         achieved = fl / (ms * 1e-3) / 1e12
-        return name, {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                      "kernel": name, "calls_per_step": calls, "avg_ms": avg_ms,
-                      "flops_per_launch": fl / calls}, groups
+        peak, note = PEAK_FP64_TFLOPS, "FP64 dense MFMA"
+        if name.startswith("ozgemm"):
+            pairs = digits * (digits + 1) // 2
+            peak = PEAK_INT8_TOPS / pairs
+            note = (f"FP64-equivalent flops; int8 dense MFMA {PEAK_INT8_TOPS:.0f} TOP/s / {pairs} digit products "
+                    f"per FP64 product ({digits} digits)")
+        return name, {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                      "frac": achieved / peak, "traffic": None, "kernel": name, "peak_note": note,
+                      "calls_per_step": calls, "avg_ms": avg_ms, "flops_per_launch": fl / calls}, groups
     achieved = by / (ms * 1e-3) / 1e9
     return name, {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                   "frac": achieved / PEAK_HBM_GBS, "traffic": None, "kernel": name,
@@ -110,6 +126,10 @@ def main():
     ap.add_argument("--v", type=int, default=160)
     ap.add_argument("--config-id", type=int, default=3)
     ap.add_argument("--tei-sigma", type=float, default=0.005)
+    ap.add_argument("--digits", type=int, default=4,
+                    help="int8 digits per operand of the split-precision path (0: FP64 only)")
+    ap.add_argument("--error-steps", type=int, default=0,
+                    help="also propagate this many steps in both precisions and report the dipole difference")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--dump-profile", default=None, help="write every profiled launch (tsv) to this file")
@@ -130,47 +150,68 @@ def main():
     ctx.set_system(None, sysm.F, sysm.mu)
     ctx.synth_tei(args.config_id, args.tei_sigma)
     ctx.set_state(*initial_state(sysm, o, v))
+    ctx.checkpoint(0)  # X0: both legs start here at t = 0
     flops_exec, arena = ctx.step_info(args.method, dt)
-    print(f"[bench] setup {time.perf_counter() - t_setup:.1f} s; step arena {arena / 2**30:.2f} GiB; "
-          f"executed contraction flops/step {flops_exec:.4e}", file=sys.stderr, flush=True)
+    log(f"setup {time.perf_counter() - t_setup:.1f} s; FP64 step arena {arena / 2**30:.2f} GiB; "
+        f"executed contraction flops/step {flops_exec:.4e}")
 
-    t = 0.0
-    if args.warmup:
-        _, ts = ctx.rk4_run(args.method, args.warmup, t, dt, w0, A, pol, precs)
-        t = float(ts[-1])
-    print("[bench] warmup done", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    mu, ts = ctx.rk4_run(args.method, args.steps, t, dt, w0, A, pol, precs)
-    el = time.perf_counter() - t0
-    t = float(ts[-1])
-    sps = args.steps / el
-    print(f"[bench] timed {args.steps} steps in {el:.3f} s", file=sys.stderr, flush=True)
-    if not np.all(np.isfinite(mu)):
-        raise RuntimeError("non-finite dipole in the timed region")
+    def leg(digits):
+        ctx.set_precision(digits)
+        if args.warmup:
+            ctx.checkpoint(1)
+            ctx.rk4_run(args.method, args.warmup, 0.0, dt, w0, A, pol, precs)
+        ctx.checkpoint(1)
+        t0 = time.perf_counter()
+        mu, ts = ctx.rk4_run(args.method, args.steps, 0.0, dt, w0, A, pol, precs)
+        el = time.perf_counter() - t0
+        if not np.all(np.isfinite(mu)):
+            raise RuntimeError("non-finite dipole in the timed region")
+        log(f"{'FP64' if digits == 0 else f'split({digits} digits)'}: timed {args.steps} steps in {el:.3f} s")
+        return args.steps / el, el, mu, float(ts[-1])
+
+    sps64, el64, mu64, t_end = leg(0)
+    split = args.digits > 0
+    if split:
+        sps, el, mu, t_end = leg(args.digits)
+        err = mu[:, 2] - mu64[:, 2]
+    else:
+        sps, el, mu = sps64, el64, mu64
 
     f_ref, f_alg = step_flops(o, v, args.method)
+    prec = (f"split: int8 digit planes ({args.digits} digits, Ozaki scheme, exact int32 accumulation) for the "
+            "ladders and contractions above 2 GFLOP; FP64 state, field, small contractions and dipole"
+            if split else "fp64")
     out = {
         "metric": METRIC, "value": sps, "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64+int8-split" if split else "f64",
+        "data": "synthetic",
         "config": {"workload": f"T{args.method} (T+Lambda) partial-RK4 propagation, o={o}, v={v} spin orbitals, "
                                f"dt={dt}, z field w0={w0} A={A}, precs={precs}",
-                   "method": args.method, "precision": "fp64", "o": o, "v": v,
+                   "method": args.method, "precision": prec, "digits": args.digits, "o": o, "v": v,
                    "tei_sigma": args.tei_sigma, "config_id": args.config_id,
                    "initial_state": "MP2 t2, t1=0, lam=t",
+                   "fp64_steps_per_s": sps64, "fp64_ms_per_step": 1e3 * el64 / args.steps,
+                   "speedup_vs_fp64": sps / sps64,
                    "F_ref_flop_per_step": f_ref, "F_alg_flop_per_step": f_alg,
                    "F_exec_contraction_flop_per_step": flops_exec,
                    "ref_equiv_TFLOPs": f_alg * sps / 1e12,
                    "exec_TFLOPs": flops_exec * sps / 1e12},
     }
+    if split:
+        out["config"]["dipole_z_err_vs_fp64"] = {"steps": args.steps, "re": float(np.abs(err.real).max()),
+                                                 "im": float(np.abs(err.imag).max())}
+    if split and args.error_steps > 0:
+        out["config"]["dipole_z_err_vs_fp64_long"] = long_error(ctx, args, dt, w0, A, pol, precs)
     if not args.no_profile:
-        recs = ctx.profile_step(args.method, t, dt, w0, A, pol, precs)
+        ctx.set_precision(args.digits)
+        recs = ctx.profile_step(args.method, t_end, dt, w0, A, pol, precs)
         if args.dump_profile:
             with open(args.dump_profile, "w") as f:
                 f.write("kernel\tflops\tbytes\tms\tdetail\n")
                 for r in sorted(recs, key=lambda r: -r[3]):
                     f.write("\t".join(str(x) for x in r) + "\n")
-        name, roof, groups = roofline(recs)
+        name, roof, groups = roofline(recs, args.digits)
         out["roofline"] = roof
         top = sorted(groups.items(), key=lambda kv: -kv[1][2])[:6]
         out["config"]["profile_top"] = {k: {"ms_per_step": g[2], "calls": g[3],
@@ -181,6 +222,32 @@ def main():
         out["cpu_baseline"] = cpu_baseline(args.method, o, v)
     ctx.close()
     print(json.dumps(out), flush=True)
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def long_error(ctx, args, dt, w0, A, pol, precs, chunk=250):
+    """Propagate error_steps steps from X0 in FP64 and in split precision and
+    return the max |mu_z| difference (re and im) over the run."""
+    traj = {}
+    for digits in (0, args.digits):
+        ctx.set_precision(digits)
+        ctx.checkpoint(1)
+        t, mus = 0.0, []
+        done = 0
+        while done < args.error_steps:
+            n = min(chunk, args.error_steps - done)
+            mu, ts = ctx.rk4_run(args.method, n, t, dt, w0, A, pol, precs)
+            mus.append(mu)
+            t = float(ts[-1])
+            done += n
+            log(f"error run digits={digits}: {done}/{args.error_steps} steps")
+        traj[digits] = np.concatenate(mus)
+    d = traj[args.digits][:, 2] - traj[0][:, 2]
+    return {"steps": args.error_steps, "re": float(np.abs(d.real).max()), "im": float(np.abs(d.imag).max()),
+            "mu_z_excursion_fp64": float(np.abs(traj[0][:, 2] - traj[0][0, 2]).max())}
 
 
 if __name__ == "__main__":

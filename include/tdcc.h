@@ -58,6 +58,18 @@ int tdcc_set_system(tdcc_ctx* ctx, const double* tei, const double* fock, const 
  * Philox4x32-10 generator, identical to python_code_to_check_c_amd.synthetic). */
 int tdcc_synth_tei(tdcc_ctx* ctx, uint32_t config_id, double sigma);
 
+/* Arithmetic of the large contractions (the o^2 v^4 ladders and every
+ * contraction above 2 GFLOP, i.e. the o^3 v^3 rings and o^2 v^3 terms).
+ * digits = 0: FP64 (the default).  digits = 2..6: split precision -- both
+ * operands are cut into `digits` int8 digit planes with one power-of-two
+ * scale per row (Ozaki scheme), the digit pairs are multiplied on int8 MFMA
+ * with exact int32 accumulation, and the result is assembled in FP64; the
+ * relative truncation per operand element is <= 0.5 * 128^-digits of its row
+ * maximum.  The RK4 state, the field and Fock build, the small contractions
+ * and the dipole stay FP64.  Replaces the reference's complex128 einsum
+ * arithmetic (opt_einsum contract, CCSD_Helper.py:20) for those terms. */
+int tdcc_set_precision(tdcc_ctx* ctx, int digits);
+
 /* Propagated state t1, t2, lam1, lam2 (complex, interleaved). */
 int tdcc_set_state(tdcc_ctx* ctx, const double* t1, const double* t2, const double* l1, const double* l2);
 /* which = 0: current state; which = 1: rounded base of the last step taken

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libtdcc_hip
 # Every symbol include/tdcc.h declares.
 EXPORTS = (
     "tdcc_create", "tdcc_destroy", "tdcc_last_error", "tdcc_set_system", "tdcc_synth_tei",
-    "tdcc_set_state", "tdcc_get_state", "tdcc_rhs", "tdcc_intermediate", "tdcc_dipole",
+    "tdcc_set_precision", "tdcc_set_state", "tdcc_get_state", "tdcc_rhs", "tdcc_intermediate", "tdcc_dipole",
     "tdcc_rk4_run", "tdcc_profile_step", "tdcc_checkpoint", "tdcc_step_info",
 )
 
@@ -48,6 +48,7 @@ def load():
         "tdcc_last_error": (ctypes.c_char_p, [vp]),
         "tdcc_set_system": (ctypes.c_int, [vp, _dp, _dp, _dp]),
         "tdcc_synth_tei": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_double]),
+        "tdcc_set_precision": (ctypes.c_int, [vp, ctypes.c_int]),
         "tdcc_set_state": (ctypes.c_int, [vp, _dp, _dp, _dp, _dp]),
         "tdcc_get_state": (ctypes.c_int, [vp, ctypes.c_int, _dp, _dp, _dp, _dp]),
         "tdcc_rhs": (ctypes.c_int, [vp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
@@ -130,6 +131,12 @@ class Context:
 
     def synth_tei(self, config_id, sigma):
         self._check(self.lib.tdcc_synth_tei(self._h, int(config_id), float(sigma)), "tdcc_synth_tei")
+
+    def set_precision(self, digits):
+        """0: FP64 contractions; 2..6: split precision with that many int8
+        digits per operand for the large contractions (see include/tdcc.h)."""
+        self._check(self.lib.tdcc_set_precision(self._h, int(digits)), "tdcc_set_precision")
+        self.digits = int(digits)
 
     def _shape_ok(self, t1, t2):
         if t1.shape != (self.o, self.v) or t2.shape != (self.o, self.o, self.v, self.v):

@@ -58,6 +58,8 @@ struct tdcc_ctx {
   double* arena = nullptr;
   int64_t arena_bytes = 0;
   double* vpack[2] = {nullptr, nullptr};   // packed <ab||ef>, <ef||ab> (may alias)
+  int8_t* vpack8[2] = {nullptr, nullptr};  // their digit planes in split precision (alias alike)
+  double* vpack8_scale[2] = {nullptr, nullptr};
   unsigned long long* counts = nullptr;    // kCountBlocks + 1 partial counts
   Runtime rt;
   std::map<std::string, Program> progs;
@@ -396,11 +398,71 @@ unsigned long long device_count(tdcc_ctx* c) {
   return n;
 }
 
+void free_vpack8(tdcc_ctx* c) {
+  for (int k : {1, 0}) {
+    if (k == 1 && c->vpack8[1] == c->vpack8[0]) {
+      c->vpack8[1] = nullptr;
+      c->vpack8_scale[1] = nullptr;
+    }
+    if (c->vpack8[k]) TDCC_HIP(hipFree(c->vpack8[k]));
+    if (c->vpack8_scale[k]) TDCC_HIP(hipFree(c->vpack8_scale[k]));
+    c->vpack8[k] = nullptr;
+    c->vpack8_scale[k] = nullptr;
+    c->rt.vpack8[k] = nullptr;
+    c->rt.vpack8_scale[k] = nullptr;
+  }
+}
+
+// Digit planes of the packed integrals for split precision: rows q(ab) of
+// vpack, split along q(ef), with the runtime's digit count.
+void build_vpack8(tdcc_ctx* c) {
+  free_vpack8(c);
+  const int S = c->rt.split_digits;
+  const int64_t P = (int64_t)c->v * (c->v - 1) / 2;
+  if (S <= 0 || P == 0 || !c->vpack[0]) return;
+  const int64_t Kp = (P + 63) / 64 * 64;
+  const int64_t plane = (P * Kp + 255) / 256 * 256;
+  double* pmax = nullptr;
+  TDCC_HIP(hipMalloc(&pmax, ozaki_split_workspace(P, Kp, false)));
+  for (int k = 0; k < 2; ++k) {
+    if (k == 1 && c->vpack[1] == c->vpack[0]) {
+      c->vpack8[1] = c->vpack8[0];
+      c->vpack8_scale[1] = c->vpack8_scale[0];
+      break;
+    }
+    TDCC_HIP(hipMalloc(&c->vpack8[k], S * plane));
+    TDCC_HIP(hipMalloc(&c->vpack8_scale[k], P * sizeof(double)));
+    SplitArgs a{};
+    a.src_re = c->vpack[k];
+    a.ldx = P;
+    a.Rb = P;
+    a.planes = 1;
+    a.K = P;
+    a.Kp = Kp;
+    a.S = S;
+    a.D = c->vpack8[k];
+    a.plane = plane;
+    a.scale = c->vpack8_scale[k];
+    a.pmax = pmax;
+    launch_ozaki_split(a, false, c->stream);
+    TDCC_HIP(hipGetLastError());
+  }
+  TDCC_HIP(hipStreamSynchronize(c->stream));
+  TDCC_HIP(hipFree(pmax));
+  for (int k = 0; k < 2; ++k) {
+    c->rt.vpack8[k] = c->vpack8[k];
+    c->rt.vpack8_scale[k] = c->vpack8_scale[k];
+  }
+  c->rt.vpack_Kp = Kp;
+  c->rt.vpack_plane = plane;
+}
+
 // Packed antisymmetric <ab||ef> for the particle-particle ladders.  The
 // packing is exact only for antisymmetrized integrals, which is checked here.
 void build_vpack(tdcc_ctx* c) {
   TDCC_HIP(hipStreamSynchronize(c->stream));
   for (auto& kv : c->progs) invalidate_exec(kv.second);
+  free_vpack8(c);
   if (c->vpack[1] && c->vpack[1] != c->vpack[0]) TDCC_HIP(hipFree(c->vpack[1]));
   if (c->vpack[0]) TDCC_HIP(hipFree(c->vpack[0]));
   c->vpack[0] = c->vpack[1] = nullptr;
@@ -425,6 +487,7 @@ void build_vpack(tdcc_ctx* c) {
   }
   c->rt.vpack[0] = c->vpack[0];
   c->rt.vpack[1] = c->vpack[1];
+  build_vpack8(c);
 }
 
 void require_system(tdcc_ctx* c) {
@@ -538,6 +601,11 @@ void tdcc_destroy(tdcc_ctx* c) {
     if (p) (void)hipFree(p);
   if (c->vpack[1] && c->vpack[1] != c->vpack[0]) (void)hipFree(c->vpack[1]);
   if (c->vpack[0]) (void)hipFree(c->vpack[0]);
+  for (int k : {1, 0}) {
+    if (k == 1 && c->vpack8[1] == c->vpack8[0]) continue;
+    if (c->vpack8[k]) (void)hipFree(c->vpack8[k]);
+    if (c->vpack8_scale[k]) (void)hipFree(c->vpack8_scale[k]);
+  }
   if (c->counts) (void)hipFree(c->counts);
   if (c->counter) (void)hipFree(c->counter);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -559,6 +627,19 @@ int tdcc_set_system(tdcc_ctx* c, const double* tei, const double* fock, const do
     TDCC_HIP(hipMemcpy(c->F, fock, n * n * sizeof(double), hipMemcpyHostToDevice));
     TDCC_HIP(hipMemcpy(c->mu, mu, 3 * n * n * sizeof(double), hipMemcpyHostToDevice));
     c->have_system = true;
+  });
+}
+
+int tdcc_set_precision(tdcc_ctx* c, int digits) {
+  return guarded(c, [&] {
+    if (digits != 0 && (digits < 2 || digits > kOzMaxSlices))
+      throw Error("digits must be 0 (FP64) or in [2, " + std::to_string(kOzMaxSlices) + "]");
+    if (digits == c->rt.split_digits) return;
+    TDCC_HIP(hipStreamSynchronize(c->stream));
+    for (auto& kv : c->progs) invalidate_exec(kv.second);
+    c->progs.clear();  // programs are planned for one precision
+    c->rt.split_digits = digits;
+    build_vpack8(c);
   });
 }
 

@@ -308,14 +308,17 @@ int CCBuilder::L2eq_rhs_CC2(int t1, int t2, int l1, int l2, int F) {  // CC2_Hel
 
 // ----------------------------------------------------------------- density
 int CCBuilder::Dij(int t1, int t2, int l1, int l2) {  // CCSD_Helper.py:984-988
+  Graph::Fp64Scope fp64(g_);  // the dipole stays FP64 in split precision
   return L(Terms().add(C("je,ie->ij", l1, t1), -1.0).add(C("jmea,imea->ij", l2, t2), -0.5));
 }
 
 int CCBuilder::Dab(int t1, int t2, int l1, int l2) {  // CCSD_Helper.py:991-995
+  Graph::Fp64Scope fp64(g_);  // the dipole stays FP64 in split precision
   return L(Terms().add(C("nb,na->ab", l1, t1)).add(C("mneb,mnea->ab", l2, t2), 0.5));
 }
 
 int CCBuilder::Dai(int t1, int t2, int l1, int l2) {  // CCSD_Helper.py:998-1006
+  Graph::Fp64Scope fp64(g_);  // the dipole stays FP64 in split precision
   return L(Terms()
                .addp(t1, {1, 0})
                .add(C("me,miea->ai", l1, t2))

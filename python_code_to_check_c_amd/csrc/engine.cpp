@@ -43,11 +43,17 @@ struct LadderArgs {
   int64_t rows = 0;
   int v = 0;
   int64_t P = 0;
+  // split precision (S > 0): digits of up, and the digit GEMM against vpack8
+  int S = 0;
+  SplitArgs su{};
+  OzGemmArgs oz{};
 };
 
 // A contraction lowered to dense GEMM form: gather A' (planes stacked along
 // rows, pa*M x K), gather B' (pb*N x K when bt, else K x pb*N), one DGEMM
 // (optionally split over K) into Z (pa*M x pb*N per split), scatter into C.
+// In split precision (S > 0) the gathers become digit splits of A' and of
+// B' (always pb*N x K), and the DGEMM the int8 digit GEMM.
 struct GemmPath {
   bool on = false;
   bool bt = false;
@@ -56,6 +62,11 @@ struct GemmPath {
   int64_t k_chunk = 0;
   int64_t a_off = 0, b_off = 0, z_off = 0;  // element offsets inside the node scratch
   int64_t ldb = 0;
+  // split precision: digits, split orientation, padded K, byte offsets
+  int S = 0;
+  bool a_cols = false, b_cols = false;
+  int64_t Kp = 0, a_plane = 0, b_plane = 0;
+  int64_t a8_off = 0, b8_off = 0, sa_off = 0, sb_off = 0, pa_off = 0, pb_off = 0, zb_off = 0;
 };
 
 struct Graph::Launch {
@@ -65,6 +76,8 @@ struct Graph::Launch {
   GemmPath gp{};
   Gather2dArgs ga{}, gb{};
   DgemmArgs dg{};
+  SplitArgs sa{}, sb{};
+  OzGemmArgs oz{};
   GemmScatterArgs sc{};
   GettArgs g{};
   bool ac = false, bc = false, akc = false, bkc = false, c_cplx = false;
@@ -100,8 +113,12 @@ static void contiguous_strides(const Graph& g, const Shape& s, int64_t st[4]) {
 int Graph::add(Node n) {
   if (!n.key.empty()) {
     auto it = cse_.find(n.key);
-    if (it != cse_.end()) return it->second;
+    if (it != cse_.end()) {
+      if (fp64_depth_ > 0) nodes_[it->second].fp64 = true;
+      return it->second;
+    }
   }
+  n.fp64 = fp64_depth_ > 0;
   const int id = (int)nodes_.size();
   if (n.key.empty()) n.key = "#" + std::to_string(id);
   cse_[n.key] = id;
@@ -425,6 +442,23 @@ static void inputs_of(const Node& n, std::vector<int>& in) {
   }
 }
 
+// Ladder scratch: packed input up and result rp (2 o^2 x P doubles each),
+// then, in split precision, the digit planes of up and its row scales.
+static int64_t al256(int64_t x) { return (x + 255) / 256 * 256; }
+static int64_t ladder_rows2(int o) { return 2 * (int64_t)o * o; }
+static int64_t ladder_P(int v) { return (int64_t)v * (v - 1) / 2; }
+static int64_t ladder_Kp(int v) { return (ladder_P(v) + 63) / 64 * 64; }
+static int64_t ladder_plane(int o, int v) { return al256(ladder_rows2(o) * ladder_Kp(v)); }
+static int64_t ladder_digits_off(int o, int v) { return al256(2 * ladder_rows2(o) * ladder_P(v) * (int64_t)sizeof(double)); }
+static int64_t ladder_scale_off(int o, int v, int S) { return ladder_digits_off(o, v) + S * ladder_plane(o, v); }
+static int64_t ladder_pmax_off(int o, int v, int S) {
+  return ladder_scale_off(o, v, S) + al256(ladder_rows2(o) * (int64_t)sizeof(double));
+}
+static int64_t ladder_scratch_bytes(int o, int v, int S) {
+  if (S <= 0) return ladder_digits_off(o, v);
+  return ladder_pmax_off(o, v, S) + al256(ozaki_split_workspace(ladder_rows2(o), ladder_Kp(v), false));
+}
+
 void Graph::plan() {
   const int N = (int)nodes_.size();
   // dead-node removal: keep what side-effect nodes reach
@@ -495,7 +529,7 @@ void Graph::plan() {
     if (!live[i]) continue;
     Node& n = nodes_[i];
     if (owns_buffer(n.op)) n.arena_off = alloc(size(n.shape) * (n.cplx ? 2 : 1) * (int64_t)sizeof(double));
-    if (n.op == Op::Ladder) n.scratch_len = 2 * (2 * (int64_t)o_ * o_ * ((int64_t)v_ * (v_ - 1) / 2)) * (int64_t)sizeof(double);
+    if (n.op == Op::Ladder) n.scratch_len = ladder_scratch_bytes(o_, v_, rt_->split_digits);
     if (n.op == Op::Contract || n.op == Op::Ladder) {
       if (n.op == Op::Contract) lower_contract(i);
       if (n.scratch_len > 0) {
@@ -726,6 +760,41 @@ void Graph::lower_contract(int id) {
     gp.pb = L->bc ? 2 : 1;
     gp.bt = bkc;
     const int64_t R = gp.pa * M, NB = gp.pb * Nn;
+    if (rt_->split_digits > 0 && !n.fp64) {
+      // digit planes of A' (R x Kp) and B' (NB x Kp); each operand is split
+      // along its contiguous index group (k, or its rows)
+      const int S = rt_->split_digits;
+      auto al256 = [](int64_t x) { return (x + 255) / 256 * 256; };
+      gp.S = S;
+      gp.a_cols = !akc;
+      gp.b_cols = !bkc;
+      gp.Kp = (K + 63) / 64 * 64;
+      gp.a_plane = al256(R * gp.Kp);
+      gp.b_plane = al256(NB * gp.Kp);
+      const int64_t bn = S <= 4 ? 128 : 64;
+      const int64_t tiles = ((R + 127) / 128) * ((NB + bn - 1) / bn);
+      int64_t sp = 1;
+      if (tiles < 240 && K >= 2048) sp = std::min<int64_t>((K + 1023) / 1024, (480 + tiles - 1) / tiles);
+      if (const char* fs = std::getenv("TDCC_GEMM_SPLITS")) sp = std::max<int64_t>(1, std::min<int64_t>(std::atol(fs), gp.Kp / 64));
+      const int64_t maxk = ozgemm_max_k(S);
+      sp = std::max<int64_t>(sp, (gp.Kp + maxk - 1) / maxk);
+      int64_t chunk = (gp.Kp + sp - 1) / sp;
+      chunk = (chunk + 63) / 64 * 64;
+      sp = (gp.Kp + chunk - 1) / chunk;
+      gp.splits = (int)sp;
+      gp.k_chunk = sp > 1 ? chunk : 0;
+      gp.a8_off = 0;
+      gp.b8_off = S * gp.a_plane;
+      gp.sa_off = gp.b8_off + S * gp.b_plane;
+      gp.sb_off = gp.sa_off + al256(8 * R);
+      gp.pa_off = gp.sb_off + al256(8 * NB);
+      gp.pb_off = gp.pa_off + al256(ozaki_split_workspace(R, gp.Kp, gp.a_cols));
+      gp.zb_off = gp.pb_off + al256(ozaki_split_workspace(NB, gp.Kp, gp.b_cols));
+      n.scratch_len = gp.zb_off + al256(8 * sp * R * NB);
+      L->node = id;
+      launches_.push_back(std::move(L));
+      return;
+    }
     const int64_t lda = K + (K & 1);
     gp.ldb = gp.bt ? lda : NB + (NB & 1);
     const int64_t tiles = ((R + 127) / 128) * ((NB + 127) / 128);
@@ -782,7 +851,70 @@ void Graph::bind(double* arena) {
         g.bb = L->tab[7] >= 0 ? dev_tables_ + L->tab[7] : nullptr;
         g.cb = L->tab[8] >= 0 ? dev_tables_ + L->tab[8] : nullptr;
         g.partial = n.scratch_len > 0 ? arena_ + n.scratch_off / (int64_t)sizeof(double) : nullptr;
-        if (L->gp.on) {
+        if (L->gp.on && L->gp.S > 0) {
+          const GemmPath& gp = L->gp;
+          char* scr = reinterpret_cast<char*>(arena_) + n.scratch_off;
+          SplitArgs& sa = L->sa;  // A' = [A_re; A_im] rows x K
+          sa.src_re = A.re;
+          sa.src_im = L->ac ? A.im : nullptr;
+          sa.rtab = g.am;
+          sa.ktab = g.ak;
+          sa.Rb = g.M;
+          sa.planes = gp.pa;
+          sa.K = g.K;
+          sa.Kp = gp.Kp;
+          sa.S = gp.S;
+          sa.D = reinterpret_cast<int8_t*>(scr + gp.a8_off);
+          sa.plane = gp.a_plane;
+          sa.scale = reinterpret_cast<double*>(scr + gp.sa_off);
+          sa.pmax = reinterpret_cast<double*>(scr + gp.pa_off);
+          SplitArgs& sb = L->sb;  // B' = [B_re; B_im] (pb*N) x K
+          sb.src_re = B.re;
+          sb.src_im = L->bc ? B.im : nullptr;
+          sb.rtab = g.bn;
+          sb.ktab = g.bk;
+          sb.Rb = g.N;
+          sb.planes = gp.pb;
+          sb.K = g.K;
+          sb.Kp = gp.Kp;
+          sb.S = gp.S;
+          sb.D = reinterpret_cast<int8_t*>(scr + gp.b8_off);
+          sb.plane = gp.b_plane;
+          sb.scale = reinterpret_cast<double*>(scr + gp.sb_off);
+          sb.pmax = reinterpret_cast<double*>(scr + gp.pb_off);
+          OzGemmArgs& oz = L->oz;
+          oz.A = sa.D;
+          oz.B = sb.D;
+          oz.a_plane = gp.a_plane;
+          oz.b_plane = gp.b_plane;
+          oz.sa = sa.scale;
+          oz.sb = sb.scale;
+          oz.C = reinterpret_cast<double*>(scr + gp.zb_off);
+          oz.M = (int)(gp.pa * g.M);
+          oz.N = (int)(gp.pb * g.N);
+          oz.ldc = oz.N;
+          oz.Kp = (int)gp.Kp;
+          oz.S = gp.S;
+          oz.k_chunk = (int)gp.k_chunk;
+          oz.splits = gp.splits;
+          oz.c_split_stride = (int64_t)oz.M * oz.N;
+          GemmScatterArgs& sc = L->sc;
+          sc.Z = oz.C;
+          sc.ldz = oz.ldc;
+          sc.splits = gp.splits;
+          sc.z_stride = oz.c_split_stride;
+          sc.M = g.M;
+          sc.N = g.N;
+          sc.kind = (L->ac && L->bc) ? 2 : (L->ac ? 1 : 0);
+          if (!L->ac && L->bc) throw Error("internal: complex operand not on the row side");
+          sc.cm = g.cm;
+          sc.cn = g.cn;
+          sc.out_re = C.re;
+          sc.out_im = n.cplx ? C.im : nullptr;
+          sc.alpha_re = n.ar;
+          sc.alpha_im = n.ai;
+          sc.accumulate = 0;
+        } else if (L->gp.on) {
           const GemmPath& gp = L->gp;
           double* scr = arena_ + n.scratch_off / (int64_t)sizeof(double);
           const int64_t M = g.M, Nn = g.N, K = g.K;
@@ -902,6 +1034,42 @@ void Graph::bind(double* arena) {
         a.rp = a.up + 2 * a.rows * a.P;
         a.vp = rt_->vpack[n.stage];
         if (!a.vp) throw Error("packed <ab||ef> not built");
+        a.S = rt_->split_digits;
+        if (a.S > 0) {
+          if (!rt_->vpack8[n.stage]) throw Error("digit planes of the packed <ab||ef> not built");
+          char* scr = reinterpret_cast<char*>(arena_) + n.scratch_off;
+          const int64_t Kp = ladder_Kp(v_), plane = ladder_plane(o_, v_);
+          if (Kp != rt_->vpack_Kp) throw Error("internal: ladder digit row length");
+          SplitArgs& su = a.su;  // rows of up, dense
+          su.src_re = a.up;
+          su.ldx = a.P;
+          su.Rb = 2 * a.rows;
+          su.planes = 1;
+          su.K = a.P;
+          su.Kp = Kp;
+          su.S = a.S;
+          su.D = reinterpret_cast<int8_t*>(scr + ladder_digits_off(o_, v_));
+          su.plane = plane;
+          su.scale = reinterpret_cast<double*>(scr + ladder_scale_off(o_, v_, a.S));
+          su.pmax = reinterpret_cast<double*>(scr + ladder_pmax_off(o_, v_, a.S));
+          OzGemmArgs& oz = a.oz;
+          oz.A = su.D;
+          oz.B = rt_->vpack8[n.stage];
+          oz.a_plane = plane;
+          oz.b_plane = rt_->vpack_plane;
+          oz.sa = su.scale;
+          oz.sb = rt_->vpack8_scale[n.stage];
+          oz.C = a.rp;
+          oz.ldc = a.P;
+          oz.M = (int)(2 * a.rows);
+          oz.N = (int)a.P;
+          oz.Kp = (int)Kp;
+          oz.S = a.S;
+          if (Kp > ozgemm_max_k(a.S)) throw Error("ladder K beyond the exact int32 range of the digit GEMM");
+          oz.k_chunk = 0;
+          oz.splits = 1;
+          oz.c_split_stride = 0;
+        }
         launches_.push_back(std::move(L));
         break;
       }
@@ -984,7 +1152,12 @@ static DgemmArgs ladder_gemm(const LadderArgs& a) {
 
 static void run_ladder(const LadderArgs& a, hipStream_t s) {
   launch_pack_pairs(a.u_re, a.u_im, a.up, a.rows, a.v, s);
-  launch_dgemm(ladder_gemm(a), true, s);
+  if (a.S > 0) {
+    launch_ozaki_split(a.su, false, s);
+    launch_ozgemm(a.oz, s);
+  } else {
+    launch_dgemm(ladder_gemm(a), true, s);
+  }
   launch_unpack_pairs(a.rp, a.out_re, a.out_im, a.rows, a.v, 1.0, s);
 }
 
@@ -993,6 +1166,13 @@ void Graph::run(hipStream_t s) const {
     switch (L->kind) {
       case Launch::GETT:
         if (L->g.M == 0 || L->g.N == 0) break;
+        if (L->gp.on && L->gp.S > 0) {
+          launch_ozaki_split(L->sa, L->gp.a_cols, s);
+          launch_ozaki_split(L->sb, L->gp.b_cols, s);
+          launch_ozgemm(L->oz, s);
+          launch_gemm_scatter(L->sc, s);
+          break;
+        }
         if (L->gp.on) {
           launch_gather2d(L->ga, s);
           launch_gather2d(L->gb, s);
@@ -1041,6 +1221,22 @@ void Graph::run_profiled(hipStream_t s, std::vector<LaunchRecord>& out) const {
           std::snprintf(nm, sizeof nm, "gett_kernel<%d,%d,%d,%d>%s", (int)L->ac, (int)L->bc, (int)L->akc, (int)L->bkc,
                         L->g.splits > 1 ? "+splitk" : "");
           char dt[160];
+          if (L->gp.on && L->gp.S > 0) {
+            const GemmPath& gp = L->gp;
+            const double abytes = (8.0 + gp.S) * L->sa.planes * (double)L->sa.Rb * L->sa.K;
+            timed(gp.a_cols ? "ozaki_split_cols_kernel" : "ozaki_split_rows_kernel", 0.0, abytes,
+                  [&] { launch_ozaki_split(L->sa, gp.a_cols, s); });
+            const double bbytes = (8.0 + gp.S) * L->sb.planes * (double)L->sb.Rb * L->sb.K;
+            timed(gp.b_cols ? "ozaki_split_cols_kernel" : "ozaki_split_rows_kernel", 0.0, bbytes,
+                  [&] { launch_ozaki_split(L->sb, gp.b_cols, s); });
+            std::snprintf(dt, sizeof dt, "%s rows=%d N=%d K=%d splits=%d digits=%d", nodes_[L->node].spec.c_str(),
+                          L->oz.M, L->oz.N, L->g.K, gp.splits, gp.S);
+            detail = dt;
+            timed("ozgemm_kernel", fl, 0.0, [&] { launch_ozgemm(L->oz, s); });
+            const double sbytes = 8.0 * L->sc.M * (double)L->sc.N * (L->sc.kind == 2 ? 4 : 2) * L->sc.splits;
+            timed("gemm_scatter_kernel", 0.0, sbytes, [&] { launch_gemm_scatter(L->sc, s); });
+            break;
+          }
           if (L->gp.on) {
             const double gbytes = 8.0 * ((double)L->ga.planes * L->ga.Rb * L->ga.Cb) * 2;
             timed("gather2d_kernel", 0.0, gbytes, [&] { launch_gather2d(L->ga, s); });
@@ -1076,9 +1272,17 @@ void Graph::run_profiled(hipStream_t s, std::vector<LaunchRecord>& out) const {
                 [&] { launch_pack_pairs(a.u_re, a.u_im, a.up, a.rows, a.v, s); });
           const DgemmArgs g = ladder_gemm(a);
           char dt[96];
-          std::snprintf(dt, sizeof dt, "packed ladder M=%d N=%d K=%d", g.M, g.N, g.K);
-          detail = dt;
-          timed("dgemm_kernel<BT>", 2.0 * g.M * (double)g.N * g.K, 0.0, [&] { launch_dgemm(g, true, s); });
+          if (a.S > 0) {
+            timed("ozaki_split_rows_kernel", 0.0, (8.0 + a.S) * 2.0 * a.rows * a.P,
+                  [&] { launch_ozaki_split(a.su, false, s); });
+            std::snprintf(dt, sizeof dt, "packed ladder M=%d N=%d K=%d digits=%d", g.M, g.N, g.K, a.S);
+            detail = dt;
+            timed("ozgemm_kernel", 2.0 * g.M * (double)g.N * g.K, 0.0, [&] { launch_ozgemm(a.oz, s); });
+          } else {
+            std::snprintf(dt, sizeof dt, "packed ladder M=%d N=%d K=%d", g.M, g.N, g.K);
+            detail = dt;
+            timed("dgemm_kernel<BT>", 2.0 * g.M * (double)g.N * g.K, 0.0, [&] { launch_dgemm(g, true, s); });
+          }
           timed("unpack_pairs_kernel", 0.0, 8.0 * (elems + 2.0 * a.rows * a.P),
                 [&] { launch_unpack_pairs(a.rp, a.out_re, a.out_im, a.rows, a.v, 1.0, s); });
           break;

@@ -74,6 +74,8 @@ struct Node {
   int64_t arena_off = -1;
   int64_t scratch_off = -1;  // split-K partials
   int64_t scratch_len = 0;
+  // stays FP64 when the runtime runs the large contractions in split precision
+  bool fp64 = false;
 };
 
 struct Handle {
@@ -97,6 +99,15 @@ struct Runtime {
   // packed antisymmetric <ab||ef>: vpack[0][q(ab)][q(ef)] = <ab||ef>,
   // vpack[1][q(ab)][q(ef)] = <ef||ab> (P = v(v-1)/2 pairs a < b)
   const double* vpack[2] = {nullptr, nullptr};
+  // Split precision: 0 = FP64 everywhere; 2..kOzMaxSlices = the dense-GEMM
+  // contractions and the ladders run on int8 digit planes with this many
+  // digits per operand (everything else stays FP64).
+  int split_digits = 0;
+  // digit planes of vpack[k]: split_digits planes of P x vpack_Kp bytes,
+  // vpack_plane bytes apart, and the P row scales
+  const int8_t* vpack8[2] = {nullptr, nullptr};
+  const double* vpack8_scale[2] = {nullptr, nullptr};
+  int64_t vpack_Kp = 0, vpack_plane = 0;
 };
 
 class Graph {
@@ -126,6 +137,13 @@ class Graph {
   // computed over pairs e < f and a < b (exact for antisymmetrized integrals).
   int ladder(int u, int ket);
   int ones_view(const Shape& s);                     // all-ones tensor (stride 0)
+  // While an Fp64Scope is open, every node created or shared is pinned to
+  // FP64 arithmetic regardless of the runtime's split precision.
+  struct Fp64Scope {
+    Graph& g;
+    explicit Fp64Scope(Graph& gr) : g(gr) { ++g.fp64_depth_; }
+    ~Fp64Scope() { --g.fp64_depth_; }
+  };
   void finalize_rk(int y_ext, int base_ext, const int k[4], bool lambda);
   void trace(int dij, int l1, int dai, int dab);
   void assign(int dst_ext, int src);
@@ -168,6 +186,7 @@ class Graph {
   int64_t* dev_tables_ = nullptr;
   std::vector<char> live_;
   int ones_id_ = -1;
+  int fp64_depth_ = 0;
   double* ones_buf_ = nullptr;
 };
 

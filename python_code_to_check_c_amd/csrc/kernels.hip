@@ -23,15 +23,6 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 constexpr int BM = 64, BN = 64, BKK = 16, NT = 256;
 constexpr int LDA = BM + 1, LDB = BN + 1;
 
-__device__ __forceinline__ int remap_tile(int bid, int tiles) {
-  // Workgroups are dispatched round-robin over the 8 XCDs (bid % 8).  Give
-  // every XCD a contiguous range of tiles so that tiles sharing a B panel
-  // (same tn, consecutive tm) run on one XCD and share its L2.
-  const int q = tiles >> 3, r = tiles & 7;
-  const int x = bid & 7, loc = bid >> 3;
-  return x * q + (x < r ? x : r) + loc;
-}
-
 template <bool AC, bool BC, bool AKC, bool BKC>
 __global__ __launch_bounds__(NT) void gett_kernel(GettArgs g) {
   constexpr int PA = AC ? 2 : 1;

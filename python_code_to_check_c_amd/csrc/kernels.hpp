@@ -10,6 +10,15 @@
 
 namespace tdcc {
 
+__device__ __forceinline__ int remap_tile(int bid, int tiles) {
+  // Workgroups are dispatched round-robin over the 8 XCDs (bid % 8).  Give
+  // every XCD a contiguous range of tiles so that tiles sharing a B panel
+  // (same tn, consecutive tm) run on one XCD and share its L2.
+  const int q = tiles >> 3, r = tiles & 7;
+  const int x = bid & 7, loc = bid >> 3;
+  return x * q + (x < r ? x : r) + loc;
+}
+
 // ---------------------------------------------------------------------------
 // Strided tensor contraction on FP64 MFMA ("GEMM-like tensor-tensor", GETT).
 // The M, N, K and batch index groups of an einsum are flattened on the host
@@ -176,6 +185,54 @@ struct GemmScatterArgs {
   int accumulate;
 };
 void launch_gemm_scatter(const GemmScatterArgs& g, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Split-precision GEMM on int8 MFMA (Ozaki scheme with exact accumulation).
+//
+// A row x of an FP64 matrix is written as x = 2^e * sum_{s=1..S} d_s 128^-s
+// (+ a remainder below 0.5 * 128^-S * 2^e), with the row exponent e chosen so
+// that |x| < 2^(e-1) and digits d_s = round-to-nearest in [-64, 64].  A product
+// of two split matrices keeps the digit pairs with s + t <= S + 1; pairs of
+// equal s + t share one int32 accumulator, which sums exactly while
+// pairs * 64 * 64 * k_chunk < 2^31.  The FP64 result is the weighted sum of the
+// accumulators times the two row scales.
+constexpr int kOzMaxSlices = 6;
+
+struct SplitArgs {
+  // Element (r, k) of the R x K source, R = planes * Rb, is
+  //   plane(r / Rb)[rtab[r % Rb] + ktab[k]]   (offset tables), or
+  //   plane(r / Rb)[(r % Rb) * ldx + k]       (rtab == ktab == nullptr);
+  // plane 0 is src_re, plane 1 src_im.
+  const double* src_re; const double* src_im;
+  const int64_t* rtab; const int64_t* ktab;
+  int64_t ldx;
+  int64_t Rb; int planes;
+  int64_t K, Kp;        // Kp: padded row length of the digit planes (multiple of 64)
+  int S;                // digits per element
+  int8_t* D;            // S planes of R x Kp, `plane` bytes apart
+  int64_t plane;
+  double* scale;        // R row scales 2^e
+  double* pmax;         // workspace: ozaki_split_workspace(R, Kp, by_columns) bytes
+};
+// Three launches: partial row maxima over k tiles, row scales, digits.
+// by_columns: threads run along r (for sources whose row index is the
+// contiguous one) instead of along k.
+void launch_ozaki_split(const SplitArgs& a, bool by_columns, hipStream_t s);
+int64_t ozaki_split_workspace(int64_t R, int64_t Kp, bool by_columns);
+
+struct OzGemmArgs {
+  const int8_t* A; const int8_t* B;   // digit planes: A[s * a_plane + m * Kp + k], B[s * b_plane + n * Kp + k]
+  int64_t a_plane, b_plane;
+  const double* sa; const double* sb; // row scales of A (M) and B (N)
+  double* C;                          // C[m * ldc + n] = sum_k A[m][k] * B[n][k]
+  int64_t ldc;
+  int M, N, Kp, S;
+  int k_chunk, splits;                // split-K over grid.y (k_chunk multiple of 64), partials
+  int64_t c_split_stride;             // c_split_stride apart
+};
+void launch_ozgemm(const OzGemmArgs& a, hipStream_t s);
+// Largest K range one int32 accumulator may sum exactly for S digits.
+int64_t ozgemm_max_k(int S);
 
 // Interleaved complex <-> two planes.
 void launch_split_complex(const double* inter, double* re, double* im, int64_t n, hipStream_t s);

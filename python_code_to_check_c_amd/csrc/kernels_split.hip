@@ -1,0 +1,337 @@
+// Split-precision GEMM for gfx950: FP64 operands are cut into int8 digit
+// planes (Ozaki scheme) and multiplied on v_mfma_i32_16x16x64_i8, whose int32
+// accumulation is exact, so the only error is the truncation of the digit
+// expansion.  The result is assembled in FP64.
+#include "kernels.hpp"
+
+#include <cmath>
+
+namespace tdcc {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// Row split in three launches:
+//   1. partial maxima of |x| over k tiles:  pmax[kt][r]
+//   2. row scales: e = ilogb(max_k |x_rk|) + 2, scale[r] = 2^e
+//   3. digits: y = x 2^-e (|y| < 1/2), d_s = rint(128 y), y <- 128 y - d_s
+// so every digit lies in [-64, 64] and every remainder in [-1/2, 1/2].  All
+// steps are exact in FP64 (power-of-two scaling, rint, subtraction of the
+// integer part).  A row with a non-finite entry gets a NaN scale and zero
+// digits, so its products come out NaN.
+// Tiles: along k (rows orientation) 16 rows x 256 k, a thread owning 4
+// consecutive k of 4 rows; along r (columns orientation) 64 rows x 64 k, a
+// thread owning 16 consecutive k of one row.  Offset tables are read once per
+// thread and reused across its elements.
+// ---------------------------------------------------------------------------
+constexpr int SR_TR = 16, SR_TK = 256;  // rows orientation tile
+constexpr int SC_TR = 64, SC_TK = 64;   // columns orientation tile
+
+__device__ __forceinline__ const double* split_row_base(const SplitArgs& a, int64_t r) {
+  const int64_t p = r / a.Rb, rb = r - p * a.Rb;
+  const double* src = p ? a.src_im : a.src_re;
+  return src + (a.rtab ? a.rtab[rb] : rb * a.ldx);
+}
+
+__device__ __forceinline__ int64_t split_koff(const SplitArgs& a, int64_t k) { return a.ktab ? a.ktab[k] : k; }
+
+__device__ __forceinline__ double abs_or_inf(double v) { return std::isfinite(v) ? fabs(v) : INFINITY; }
+
+
+// next digit of 4 values, packed little-endian into 4 bytes
+__device__ __forceinline__ uint32_t next_digit4(double v[4]) {
+  uint32_t packed = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const double y = v[j] * 128.0;
+    const double d = rint(y);
+    v[j] = y - d;
+    packed |= (uint32_t)(uint8_t)(int8_t)(int)d << (8 * j);
+  }
+  return packed;
+}
+
+__global__ __launch_bounds__(256) void split_max_rows_kernel(SplitArgs a) {
+  const int64_t R = (int64_t)a.planes * a.Rb;
+  const int q = threadIdx.x & 63, rg = threadIdx.x >> 6;
+  const int64_t k0 = (int64_t)blockIdx.y * SR_TK + 4 * q;
+  int64_t ko[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) ko[j] = k0 + j < a.K ? split_koff(a, k0 + j) : -1;
+#pragma unroll
+  for (int i = 0; i < SR_TR / 4; ++i) {
+    const int64_t r = (int64_t)blockIdx.x * SR_TR + rg + 4 * i;
+    double m = 0.0;
+    if (r < R) {
+      const double* base = split_row_base(a, r);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (ko[j] >= 0) m = fmax(m, abs_or_inf(base[ko[j]]));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    if (q == 0 && r < R) a.pmax[(int64_t)blockIdx.y * R + r] = m;
+  }
+}
+
+__global__ __launch_bounds__(256) void split_max_cols_kernel(SplitArgs a) {
+  __shared__ double red[4][SC_TR];
+  const int64_t R = (int64_t)a.planes * a.Rb;
+  const int rl = threadIdx.x & 63, kg = threadIdx.x >> 6;
+  const int64_t r = (int64_t)blockIdx.x * SC_TR + rl;
+  const int64_t k0 = (int64_t)blockIdx.y * SC_TK + 16 * kg;
+  double m = 0.0;
+  if (r < R) {
+    const double* base = split_row_base(a, r);
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+      if (k0 + j < a.K) m = fmax(m, abs_or_inf(base[split_koff(a, k0 + j)]));
+  }
+  red[kg][rl] = m;
+  __syncthreads();
+  if (kg == 0 && r < R)
+    a.pmax[(int64_t)blockIdx.y * R + r] = fmax(fmax(red[0][rl], red[1][rl]), fmax(red[2][rl], red[3][rl]));
+}
+
+__global__ __launch_bounds__(256) void split_scale_kernel(SplitArgs a, int64_t ktiles) {
+  const int64_t R = (int64_t)a.planes * a.Rb;
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r >= R) return;
+  double m = 0.0;
+  for (int64_t t = 0; t < ktiles; ++t) m = fmax(m, a.pmax[t * R + r]);
+  a.scale[r] = std::isfinite(m) ? ldexp(1.0, m > 0.0 ? ilogb(m) + 2 : 0) : NAN;
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void split_digits_rows_kernel(SplitArgs a) {
+  const int64_t R = (int64_t)a.planes * a.Rb;
+  const int q = threadIdx.x & 63, rg = threadIdx.x >> 6;
+  const int64_t k0 = (int64_t)blockIdx.y * SR_TK + 4 * q;
+  if (k0 >= a.Kp) return;
+  int64_t ko[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) ko[j] = k0 + j < a.K ? split_koff(a, k0 + j) : -1;
+#pragma unroll
+  for (int i = 0; i < SR_TR / 4; ++i) {
+    const int64_t r = (int64_t)blockIdx.x * SR_TR + rg + 4 * i;
+    if (r >= R) break;
+    const double sc = a.scale[r];
+    const bool finite = std::isfinite(sc);
+    const int e = finite ? ilogb(sc) : 0;
+    const double* base = split_row_base(a, r);
+    double v[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = (finite && ko[j] >= 0) ? ldexp(base[ko[j]], -e) : 0.0;
+    int8_t* d = a.D + r * a.Kp + k0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) *reinterpret_cast<uint32_t*>(d + s * a.plane) = next_digit4(v);
+  }
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void split_digits_cols_kernel(SplitArgs a) {
+  const int64_t R = (int64_t)a.planes * a.Rb;
+  const int rl = threadIdx.x & 63, kg = threadIdx.x >> 6;
+  const int64_t r = (int64_t)blockIdx.x * SC_TR + rl;
+  const int64_t k0 = (int64_t)blockIdx.y * SC_TK + 16 * kg;
+  if (r >= R) return;
+  const double sc = a.scale[r];
+  const bool finite = std::isfinite(sc);
+  const int e = finite ? ilogb(sc) : 0;
+  const double* base = split_row_base(a, r);
+  uint32_t out[4][S];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    double v[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int64_t k = k0 + 4 * q + j;
+      v[j] = (finite && k < a.K) ? ldexp(base[split_koff(a, k)], -e) : 0.0;
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) out[q][s] = next_digit4(v);
+  }
+  int8_t* d = a.D + r * a.Kp + k0;
+#pragma unroll
+  for (int s = 0; s < S; ++s)
+    *reinterpret_cast<uint4*>(d + s * a.plane) = make_uint4(out[0][s], out[1][s], out[2][s], out[3][s]);
+}
+
+int64_t ozaki_split_workspace(int64_t R, int64_t Kp, bool by_columns) {
+  const int64_t tk = by_columns ? SC_TK : SR_TK;
+  return ((Kp + tk - 1) / tk) * R * (int64_t)sizeof(double);
+}
+
+template <int S>
+static void split_launch(const SplitArgs& a, bool by_columns, hipStream_t s) {
+  const int64_t R = (int64_t)a.planes * a.Rb;
+  const int64_t tr = by_columns ? SC_TR : SR_TR, tk = by_columns ? SC_TK : SR_TK;
+  const int64_t kt = (a.Kp + tk - 1) / tk;
+  const dim3 grid((unsigned)((R + tr - 1) / tr), (unsigned)kt), block(256);
+  if (by_columns) hipLaunchKernelGGL(split_max_cols_kernel, grid, block, 0, s, a);
+  else hipLaunchKernelGGL(split_max_rows_kernel, grid, block, 0, s, a);
+  hipLaunchKernelGGL(split_scale_kernel, dim3((unsigned)((R + 255) / 256)), block, 0, s, a, kt);
+  if (by_columns) hipLaunchKernelGGL(split_digits_cols_kernel<S>, grid, block, 0, s, a);
+  else hipLaunchKernelGGL(split_digits_rows_kernel<S>, grid, block, 0, s, a);
+}
+
+void launch_ozaki_split(const SplitArgs& a, bool by_columns, hipStream_t s) {
+  if (a.planes * a.Rb <= 0) return;
+  switch (a.S) {
+    case 2: split_launch<2>(a, by_columns, s); break;
+    case 3: split_launch<3>(a, by_columns, s); break;
+    case 4: split_launch<4>(a, by_columns, s); break;
+    case 5: split_launch<5>(a, by_columns, s); break;
+    case 6: split_launch<6>(a, by_columns, s); break;
+    default: break;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Digit-pair GEMM.  Tile 128 x (32 JB) x 64, 256 threads = 4 waves (2 x 2);
+// a wave owns 64 x 16 JB = 4 x JB MFMA blocks of 16 x 16 per digit group.
+// K step = 64 = one v_mfma_i32_16x16x64_i8.  Lane l of a wave supplies
+// A[row l%16][k 16(l/16) .. +15] and B[col l%16][same k] as 16 bytes; the
+// int32 result sits at D[4(l/16) + r][l%16] (measured on gfx950).
+// LDS rows are 64 bytes with the 16-byte chunk index XOR-swizzled by
+// (row >> 1) & 3, so the 16-byte reads of 8 consecutive lanes hit 8 distinct
+// bank groups.  Global -> registers -> LDS double buffering, one barrier per
+// K step.
+// ---------------------------------------------------------------------------
+constexpr int OZ_BM = 128, OZ_BK = 64;
+
+__device__ __forceinline__ int oz_swz(int row, int kc) { return row * 4 + (kc ^ ((row >> 1) & 3)); }
+
+template <int S, int JB>
+__global__ __launch_bounds__(256) void ozgemm_kernel(OzGemmArgs g) {
+  constexpr int BN = 32 * JB;
+  constexpr int CA = OZ_BM * 4 / 256, CB = BN * 4 / 256;  // 16-byte chunks per thread and digit
+  __shared__ v4i As[2][S][OZ_BM * 4];
+  __shared__ v4i Bs[2][S][BN * 4];
+  const int tiles_m = (g.M + OZ_BM - 1) / OZ_BM, tiles_n = (g.N + BN - 1) / BN;
+  const int L = remap_tile(blockIdx.x, tiles_m * tiles_n);
+  const int tm = L % tiles_m, tn = L / tiles_m;
+  const int m0 = tm * OZ_BM, n0 = tn * BN;
+  const int tid = threadIdx.x;
+  const int kb = g.k_chunk > 0 ? (int)blockIdx.y * g.k_chunk : 0;
+  const int ke = g.k_chunk > 0 ? min(g.Kp, kb + g.k_chunk) : g.Kp;
+  double* const Cout = g.C + (int64_t)blockIdx.y * g.c_split_stride;
+
+  v4i ra[S][CA], rb[S][CB];
+  auto load_global = [&](int k0) {
+#pragma unroll
+    for (int h = 0; h < CA; ++h) {
+      const int c = tid + 256 * h, row = c >> 2, kc = c & 3;
+      const int m = m0 + row;
+#pragma unroll
+      for (int s = 0; s < S; ++s)
+        ra[s][h] = m < g.M ? *reinterpret_cast<const v4i*>(g.A + s * g.a_plane + (int64_t)m * g.Kp + k0 + kc * 16)
+                           : v4i{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int h = 0; h < CB; ++h) {
+      const int c = tid + 256 * h, row = c >> 2, kc = c & 3;
+      const int n = n0 + row;
+#pragma unroll
+      for (int s = 0; s < S; ++s)
+        rb[s][h] = n < g.N ? *reinterpret_cast<const v4i*>(g.B + s * g.b_plane + (int64_t)n * g.Kp + k0 + kc * 16)
+                           : v4i{0, 0, 0, 0};
+    }
+  };
+  auto store_lds = [&](int buf) {
+#pragma unroll
+    for (int h = 0; h < CA; ++h) {
+      const int c = tid + 256 * h;
+#pragma unroll
+      for (int s = 0; s < S; ++s) As[buf][s][oz_swz(c >> 2, c & 3)] = ra[s][h];
+    }
+#pragma unroll
+    for (int h = 0; h < CB; ++h) {
+      const int c = tid + 256 * h;
+#pragma unroll
+      for (int s = 0; s < S; ++s) Bs[buf][s][oz_swz(c >> 2, c & 3)] = rb[s][h];
+    }
+  };
+
+  v4i acc[S][4][JB];
+#pragma unroll
+  for (int q = 0; q < S; ++q)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < JB; ++j) acc[q][i][j] = v4i{0, 0, 0, 0};
+  const int lane = tid & 63, w = tid >> 6;
+  const int wm = (w & 1) * 64, wn = (w >> 1) * (16 * JB);
+  const int lr = lane & 15, lk = lane >> 4;
+
+  load_global(kb);
+  store_lds(0);
+  __syncthreads();
+  int buf = 0;
+  for (int k0 = kb; k0 < ke; k0 += OZ_BK) {
+    const bool more = k0 + OZ_BK < ke;
+    if (more) load_global(k0 + OZ_BK);
+    v4i a[S][4];
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[s][i] = As[buf][s][oz_swz(wm + i * 16 + lr, lk)];
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+      v4i b[JB];
+#pragma unroll
+      for (int j = 0; j < JB; ++j) b[j] = Bs[buf][t][oz_swz(wn + j * 16 + lr, lk)];
+#pragma unroll
+      for (int s = 0; s + t < S; ++s)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < JB; ++j)
+            acc[s + t][i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[s][i], b[j], acc[s + t][i][j], 0, 0, 0);
+    }
+    if (more) store_lds(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+  // digit group q (pairs with s + t = q + 2, 1-based digits) weighs 128^-(q+2)
+#pragma unroll
+  for (int j = 0; j < JB; ++j) {
+    const int col = n0 + wn + j * 16 + lr;
+    if (col >= g.N) continue;
+    const double sbc = g.sb[col];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm + i * 16 + 4 * lk + r;
+        if (row >= g.M) continue;
+        double v = 0.0;
+#pragma unroll
+        for (int q = S - 1; q >= 0; --q) v += ldexp((double)acc[q][i][j][r], -7 * (q + 2));
+        Cout[(int64_t)row * g.ldc + col] = v * (g.sa[row] * sbc);
+      }
+  }
+}
+
+int64_t ozgemm_max_k(int S) {
+  // S pairs share the largest group; |d| <= 64 for every digit
+  const int64_t lim = ((int64_t)1 << 31) - 1;
+  return lim / (4096LL * S) / OZ_BK * OZ_BK;
+}
+
+void launch_ozgemm(const OzGemmArgs& g, hipStream_t s) {
+  if (g.M <= 0 || g.N <= 0) return;
+  const int bn = g.S <= 4 ? 128 : 64;
+  const int tiles = ((g.M + OZ_BM - 1) / OZ_BM) * ((g.N + bn - 1) / bn);
+  const dim3 grid(tiles, g.k_chunk > 0 ? g.splits : 1), block(256);
+  switch (g.S) {
+    case 2: hipLaunchKernelGGL((ozgemm_kernel<2, 4>), grid, block, 0, s, g); break;
+    case 3: hipLaunchKernelGGL((ozgemm_kernel<3, 4>), grid, block, 0, s, g); break;
+    case 4: hipLaunchKernelGGL((ozgemm_kernel<4, 4>), grid, block, 0, s, g); break;
+    case 5: hipLaunchKernelGGL((ozgemm_kernel<5, 2>), grid, block, 0, s, g); break;
+    case 6: hipLaunchKernelGGL((ozgemm_kernel<6, 2>), grid, block, 0, s, g); break;
+    default: break;
+  }
+}
+
+}  // namespace tdcc

@@ -1,0 +1,99 @@
+"""Split-precision contractions (int8 digit planes, exact int32 accumulation) on the GPU.
+
+Every batch-free contraction is forced through the dense-GEMM path
+(TDCC_GEMM_MIN_FLOPS=0), so at these small shapes all rings, o^2 v^3 terms and
+the ladders run on the digit GEMM.  The reference's outputs (golden vectors)
+are the yardstick:
+  * each operand element is truncated to 0.5 * 128^-S of its row maximum, so
+    residuals, the dipole and RK4 trajectories must agree with the reference to
+    a relative error of at most 30 * 128^-S (the constant covers the sums of
+    up to a few hundred terms at these shapes; measured ratios are 2-8);
+  * int32 accumulation is exact and the FP64 assembly has a fixed order, so a
+    repeated evaluation is bitwise identical;
+  * switching back to FP64 gives the FP64 results (1e-11).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from python_code_to_check_c_amd import _lib
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+RHS = [("T1eq_rhs", _lib.RHS_T1), ("T2eq_rhs", _lib.RHS_T2), ("lam_1eq_rhs", _lib.RHS_L1),
+       ("lam2eq_rhs", _lib.RHS_L2), ("T2eq_rhs_CC2", _lib.RHS_T2_CC2), ("L1eq_rhs_CC2", _lib.RHS_L1_CC2),
+       ("L2eq_rhs_CC2", _lib.RHS_L2_CC2)]
+
+
+def bound(S):
+    return 30.0 * 128.0 ** -S
+
+
+@pytest.fixture(scope="module", params=["o4v6", "c1_o10v4"])
+def case(request):
+    g = dict(np.load(os.path.join(HERE, "golden", f"golden_{request.param}.npz")))
+    print(f"\n[split case {request.param}]", flush=True)
+    saved = os.environ.get("TDCC_GEMM_MIN_FLOPS")
+    os.environ["TDCC_GEMM_MIN_FLOPS"] = "0"
+    ctx = _lib.Context(int(g["o"]), int(g["v"]))
+    ctx.set_system(g["TEI"], g["F"], g["mu"])
+    try:
+        yield g, ctx
+    finally:
+        ctx.close()
+        if saved is None:
+            os.environ.pop("TDCC_GEMM_MIN_FLOPS", None)
+        else:
+            os.environ["TDCC_GEMM_MIN_FLOPS"] = saved
+
+
+@pytest.mark.parametrize("S", [2, 3, 4, 5, 6])
+def test_residuals_within_truncation_bound(case, S):
+    g, ctx = case
+    ctx.set_precision(S)
+    for name, kind in RHS:
+        got = ctx.rhs(kind, g["Fc"], g["t1"], g["t2"], g["lam1"], g["lam2"])
+        ref = g["out_" + name]
+        rel = np.abs(got - ref).max() / np.abs(ref).max()
+        assert rel <= bound(S), (name, rel)
+    got = ctx.dipole(g["t1"], g["t2"], g["lam1"], g["lam2"])
+    ref = g["out_dipole_moment"]
+    assert np.abs(got - ref).max() <= bound(S) * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("S", [3, 4, 6])
+@pytest.mark.parametrize("method", ["CCSD", "CC2"])
+def test_rk4_trajectory_within_bound(case, S, method):
+    g, ctx = case
+    ctx.set_precision(S)
+    ctx.set_state(g["t1"], g["t2"], g["lam1"], g["lam2"])
+    mu, _ = ctx.rk4_run(method, int(g["rk_steps"]), 0.0, float(g["rk_dt"]), float(g["rk_w0"]), float(g["rk_A"]))
+    ref = g[f"rk_{method}_mu"][1:]
+    assert np.abs(mu - ref).max() <= bound(S) * np.abs(ref).max()
+
+
+def test_split_is_deterministic(case):
+    g, ctx = case
+    ctx.set_precision(4)
+    a = [ctx.rhs(kind, g["Fc"], g["t1"], g["t2"], g["lam1"], g["lam2"]) for _, kind in RHS]
+    b = [ctx.rhs(kind, g["Fc"], g["t1"], g["t2"], g["lam1"], g["lam2"]) for _, kind in RHS]
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_back_to_fp64(case):
+    g, ctx = case
+    ctx.set_precision(5)
+    ctx.rhs(_lib.RHS_T2, g["Fc"], g["t1"], g["t2"], g["lam1"], g["lam2"])
+    ctx.set_precision(0)
+    for name, kind in RHS:
+        got = ctx.rhs(kind, g["Fc"], g["t1"], g["t2"], g["lam1"], g["lam2"])
+        assert np.abs(got - g["out_" + name]).max() <= 1e-11, name
+
+
+def test_rejects_bad_digit_count(case):
+    _, ctx = case
+    for bad in (1, 7, -1):
+        with pytest.raises(_lib.TdccError):
+            ctx.set_precision(bad)
