@@ -133,7 +133,14 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--dump-profile", default=None, help="write every profiled launch (tsv) to this file")
+    ap.add_argument("--split-only", action="store_true",
+                    help="time only the split-precision leg (profiling runs; no FP64 comparison)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy kernel over MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding of this "
               "path is a design, not a run.", file=sys.stderr)
@@ -169,13 +176,16 @@ def main():
         log(f"{'FP64' if digits == 0 else f'split({digits} digits)'}: timed {args.steps} steps in {el:.3f} s")
         return args.steps / el, el, mu, float(ts[-1])
 
-    sps64, el64, mu64, t_end = leg(0)
     split = args.digits > 0
-    if split:
+    if args.split_only and split:
         sps, el, mu, t_end = leg(args.digits)
-        err = mu[:, 2] - mu64[:, 2]
+        sps64, el64, mu64 = float("nan"), float("nan"), None
     else:
-        sps, el, mu = sps64, el64, mu64
+        sps64, el64, mu64, t_end = leg(0)
+        if split:
+            sps, el, mu, t_end = leg(args.digits)
+        else:
+            sps, el, mu = sps64, el64, mu64
 
     f_ref, f_alg = step_flops(o, v, args.method)
     prec = (f"split: int8 digit planes ({args.digits} digits, Ozaki scheme, exact int32 accumulation) for the "
@@ -198,7 +208,8 @@ def main():
                    "ref_equiv_TFLOPs": f_alg * sps / 1e12,
                    "exec_TFLOPs": flops_exec * sps / 1e12},
     }
-    if split:
+    if split and mu64 is not None:
+        err = mu[:, 2] - mu64[:, 2]
         out["config"]["dipole_z_err_vs_fp64"] = {"steps": args.steps, "re": float(np.abs(err.real).max()),
                                                  "im": float(np.abs(err.imag).max())}
     if split and args.error_steps > 0:
@@ -222,6 +233,20 @@ def main():
         out["cpu_baseline"] = cpu_baseline(args.method, o, v)
     ctx.close()
     print(json.dumps(out), flush=True)
+
+
+def calibrate_copy(mib):
+    """A plain device-to-device copy kernel of known size (y = x * 1.0 over
+    mib MiB of doubles: mib MiB read, mib MiB written), for scaling the HBM
+    traffic counters (FETCH_SIZE / WRITE_SIZE) of the other kernels."""
+    import torch
+    n = mib * 2**20 // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    log(f"copy calibration: 3 launches of y = x * 1.0, {mib} MiB read + {mib} MiB written each")
 
 
 def log(msg):

@@ -67,6 +67,11 @@ struct GemmPath {
   bool a_cols = false, b_cols = false;
   int64_t Kp = 0, a_plane = 0, b_plane = 0;
   int64_t a8_off = 0, b8_off = 0, sa_off = 0, sb_off = 0, pa_off = 0, pb_off = 0, zb_off = 0;
+  // operands that never change while the program is bound (integral blocks)
+  // are split once per bind into the graph's digit cache: entry index or -1
+  int a_cache = -1, b_cache = -1;
+  // one split: the digit GEMM scatters into C itself (planes interleaved by row)
+  bool fused = false;
 };
 
 struct Graph::Launch {
@@ -79,6 +84,9 @@ struct Graph::Launch {
   SplitArgs sa{}, sb{};
   OzGemmArgs oz{};
   GemmScatterArgs sc{};
+  // skinny path: one operand with at most kSkinnyMax rows (small_is_a: the A side)
+  bool skinny = false, small_is_a = true;
+  SkinnyArgs sk{};
   GettArgs g{};
   bool ac = false, bc = false, akc = false, bkc = false, c_cplx = false;
   int64_t tab[9] = {0, 0, 0, 0, 0, 0, -1, -1, -1};  // am ak bn bk cm cn ab bb cb offsets into tables
@@ -93,6 +101,7 @@ Graph::Graph(int o, int v, const Runtime* rt) : o_(o), v_(v), rt_(rt) {}
 Graph::~Graph() {
   if (dev_tables_) (void)hipFree(dev_tables_);
   if (ones_buf_) (void)hipFree(ones_buf_);
+  if (cache_buf_) (void)hipFree(cache_buf_);
 }
 
 int64_t Graph::size(const Shape& s) const {
@@ -211,6 +220,21 @@ static bool has(const std::string& s, char c) { return s.find(c) != std::string:
 static double gemm_min_flops() {
   const char* e = std::getenv("TDCC_GEMM_MIN_FLOPS");
   return e ? std::atof(e) : 2e9;
+}
+
+// TDCC_SPLIT_MULTIPASS=1 makes every digit split use the three-pass kernels.
+static bool split_multipass() {
+  const char* e = std::getenv("TDCC_SPLIT_MULTIPASS");
+  return e && std::atoi(e) != 0;
+}
+
+// Skinny contractions with at least this much big-operand work (rows x K) take
+// the VALU streaming kernel; TDCC_SKINNY_MIN overrides it (0: all skinny
+// shapes, a negative value: none).
+static double skinny_min_work() {
+  const char* e = std::getenv("TDCC_SKINNY_MIN");
+  const double v = e ? std::atof(e) : 65536.0;
+  return v < 0 ? 1e300 : v;
 }
 
 int Graph::contract(const std::string& spec, int a, int b, double ar, double ai) {
@@ -459,8 +483,17 @@ static int64_t ladder_scratch_bytes(int o, int v, int S) {
   return ladder_pmax_off(o, v, S) + al256(ozaki_split_workspace(ladder_rows2(o), ladder_Kp(v), false));
 }
 
+bool Graph::is_static(int id) const {
+  const Node& n = nodes_[id];
+  if (n.op == Op::View) return is_static(n.base);
+  return n.op == Op::External && (n.key.rfind("X|tei|", 0) == 0 || n.key.rfind("X|ones|", 0) == 0);
+}
+
 void Graph::plan() {
   const int N = (int)nodes_.size();
+  cache_.clear();
+  cache_index_.clear();
+  cache_bytes_ = 0;
   // dead-node removal: keep what side-effect nodes reach
   std::vector<char> live(N, 0);
   std::vector<int> in;
@@ -548,6 +581,11 @@ void Graph::plan() {
     TDCC_HIP(hipMalloc(&dev_tables_, host_tables_.size() * sizeof(int64_t)));
     TDCC_HIP(hipMemcpy(dev_tables_, host_tables_.data(), host_tables_.size() * sizeof(int64_t), hipMemcpyHostToDevice));
   }
+  if (cache_buf_) {
+    (void)hipFree(cache_buf_);
+    cache_buf_ = nullptr;
+  }
+  if (cache_bytes_ > 0) TDCC_HIP(hipMalloc(&cache_buf_, cache_bytes_));
 }
 
 Dev Graph::resolve(int id) const {
@@ -658,8 +696,12 @@ void Graph::lower_contract(int id) {
   const double mul0 = (nodes_[a].cplx && nodes_[b].cplx) ? 8 : ((nodes_[a].cplx || nodes_[b].cplx) ? 4 : 2);
   const double gflops = mul0 * gsize(gm) * (double)gsize(gn) * gsize(gk);
   const double min_flops = gemm_min_flops();
-  const bool use_gemm = gb.empty() && gflops >= min_flops &&
-                        (min_flops == 0.0 || (gsize(gm) >= 64 && gsize(gn) >= 64 && gsize(gk) >= 32));
+  // dot-product shapes (both free groups small, K long) also stream well
+  // through the GEMM path's split-K
+  const bool dot_shape = gsize(gm) >= 16 && gsize(gn) >= 16 && gsize(gk) >= 8192 && gflops >= 2.5e8;
+  const bool use_gemm = gb.empty() && (min_flops == 0.0 ||
+                                       (gflops >= min_flops && gsize(gm) >= 64 && gsize(gn) >= 64 && gsize(gk) >= 32) ||
+                                       (dot_shape && min_flops <= 2e9));
   if (use_gemm && !nodes_[a].cplx && nodes_[b].cplx) {
     std::swap(la, lb);
     std::swap(a, b);
@@ -753,6 +795,33 @@ void Graph::lower_contract(int id) {
   n.a = a;  // orientation used by the launch
   n.b = b;
   n.scratch_len = splits > 1 ? (int64_t)splits * Bt * M * Nn * (L->ac || L->bc ? 2 : 1) * (int64_t)sizeof(double) : 0;
+  // Skinny shapes (one side at most kSkinnyMax rows) stream the big operand
+  // once on the VALU kernel instead of padding 64 x 64 MFMA tiles.
+  // The big operand must be streamed along its rows (contiguous index not in
+  // K), except for matrix-vector shapes (at most 4 small rows).
+  const bool small_a = M <= Nn;
+  const bool big_kmajor = small_a ? bkc : akc;
+  if (!use_gemm && Bt == 1 && std::min(M, Nn) <= kSkinnyMax && (double)std::max(M, Nn) * K >= skinny_min_work() &&
+      (!big_kmajor || std::min(M, Nn) <= 4)) {
+    L->skinny = true;
+    L->small_is_a = small_a;
+    SkinnyArgs& k = L->sk;
+    k.I = (int)(L->small_is_a ? M : Nn);
+    k.J = L->small_is_a ? Nn : M;
+    k.K = K;
+    k.g_kmajor = L->small_is_a ? bkc : akc;
+    const int64_t per = k.g_kmajor ? 4 : 256;
+    const int64_t blocks = (k.J + per - 1) / per;
+    int64_t sp = std::max<int64_t>(1, std::min<int64_t>((1024 + blocks - 1) / blocks, (K + 255) / 256));
+    int64_t ch = (K + sp - 1) / sp;
+    ch = (ch + 63) / 64 * 64;
+    sp = (K + ch - 1) / ch;
+    k.splits = (int)sp;
+    k.k_chunk = ch;
+    k.alpha_re = n.ar;
+    k.alpha_im = n.ai;
+    n.scratch_len = sp > 1 ? sp * (L->ac || L->bc ? 2 : 1) * (int64_t)k.I * k.J * (int64_t)sizeof(double) : 0;
+  }
   if (use_gemm) {
     GemmPath& gp = L->gp;
     gp.on = true;
@@ -790,7 +859,25 @@ void Graph::lower_contract(int id) {
       gp.pa_off = gp.sb_off + al256(8 * NB);
       gp.pb_off = gp.pa_off + al256(ozaki_split_workspace(R, gp.Kp, gp.a_cols));
       gp.zb_off = gp.pb_off + al256(ozaki_split_workspace(NB, gp.Kp, gp.b_cols));
-      n.scratch_len = gp.zb_off + al256(8 * sp * R * NB);
+      gp.fused = sp == 1;
+      n.scratch_len = gp.zb_off + (gp.fused ? 0 : al256(8 * sp * R * NB));
+      auto cache_entry = [&](int x, const std::string& rows, bool cols, int64_t nrows, int64_t plane) {
+        if (!is_static(x)) return -1;
+        const std::string key = nodes_[x].key + "|" + rows + "|" + gk + "|" + (cols ? "c" : "r") + "|" +
+                                std::to_string(nrows) + "|" + std::to_string(S);
+        auto it = cache_index_.find(key);
+        if (it != cache_index_.end()) return it->second;
+        DigitCache e;
+        e.digits_off = cache_bytes_;
+        e.scale_off = cache_bytes_ + S * plane;
+        cache_bytes_ = e.scale_off + al256(8 * nrows);
+        cache_.push_back(e);
+        const int idx = (int)cache_.size() - 1;
+        cache_index_[key] = idx;
+        return idx;
+      };
+      gp.a_cache = cache_entry(a, gm, gp.a_cols, R, gp.a_plane);
+      gp.b_cache = cache_entry(b, gn, gp.b_cols, NB, gp.b_plane);
       L->node = id;
       launches_.push_back(std::move(L));
       return;
@@ -818,6 +905,8 @@ void Graph::lower_contract(int id) {
 
 void Graph::bind(double* arena) {
   arena_ = arena;
+  for (auto& e : cache_) e.filled = false;
+  bool cache_launched = false;
   // rebuild every launch record with device pointers
   std::vector<std::unique_ptr<Launch>> gett;
   for (auto& L : launches_)
@@ -851,6 +940,24 @@ void Graph::bind(double* arena) {
         g.bb = L->tab[7] >= 0 ? dev_tables_ + L->tab[7] : nullptr;
         g.cb = L->tab[8] >= 0 ? dev_tables_ + L->tab[8] : nullptr;
         g.partial = n.scratch_len > 0 ? arena_ + n.scratch_off / (int64_t)sizeof(double) : nullptr;
+        if (L->skinny) {
+          SkinnyArgs& k = L->sk;
+          const bool sa_ = L->small_is_a;
+          k.s_re = sa_ ? g.a_re : g.b_re;
+          k.s_im = sa_ ? g.a_im : g.b_im;
+          k.g_re = sa_ ? g.b_re : g.a_re;
+          k.g_im = sa_ ? g.b_im : g.a_im;
+          k.sr = sa_ ? g.am : g.bn;
+          k.sk = sa_ ? g.ak : g.bk;
+          k.gr = sa_ ? g.bn : g.am;
+          k.gk = sa_ ? g.bk : g.ak;
+          k.cs = sa_ ? g.cm : g.cn;
+          k.cb = sa_ ? g.cn : g.cm;
+          k.c_re = C.re;
+          k.c_im = n.cplx ? C.im : nullptr;
+          k.partial = g.partial;
+          if ((k.s_im || k.g_im) && !k.c_im) throw Error("internal: complex product into a real output");
+        }
         if (L->gp.on && L->gp.S > 0) {
           const GemmPath& gp = L->gp;
           char* scr = reinterpret_cast<char*>(arena_) + n.scratch_off;
@@ -868,6 +975,8 @@ void Graph::bind(double* arena) {
           sa.plane = gp.a_plane;
           sa.scale = reinterpret_cast<double*>(scr + gp.sa_off);
           sa.pmax = reinterpret_cast<double*>(scr + gp.pa_off);
+          sa.multipass = split_multipass();
+          sa.interleave = gp.fused;
           SplitArgs& sb = L->sb;  // B' = [B_re; B_im] (pb*N) x K
           sb.src_re = B.re;
           sb.src_im = L->bc ? B.im : nullptr;
@@ -882,6 +991,22 @@ void Graph::bind(double* arena) {
           sb.plane = gp.b_plane;
           sb.scale = reinterpret_cast<double*>(scr + gp.sb_off);
           sb.pmax = reinterpret_cast<double*>(scr + gp.pb_off);
+          sb.multipass = split_multipass();
+          sb.interleave = gp.fused;
+          // static operands: digits live in the cache, split here once per bind
+          // (the node's scratch serves as the split workspace)
+          for (auto [idx, sp, cols] : {std::make_tuple(gp.a_cache, &sa, gp.a_cols),
+                                       std::make_tuple(gp.b_cache, &sb, gp.b_cols)}) {
+            if (idx < 0) continue;
+            DigitCache& e = cache_[idx];
+            sp->D = reinterpret_cast<int8_t*>(cache_buf_ + e.digits_off);
+            sp->scale = reinterpret_cast<double*>(cache_buf_ + e.scale_off);
+            if (!e.filled) {
+              launch_ozaki_split(*sp, cols, rt_->stream);
+              e.filled = true;
+              cache_launched = true;
+            }
+          }
           OzGemmArgs& oz = L->oz;
           oz.A = sa.D;
           oz.B = sb.D;
@@ -898,6 +1023,17 @@ void Graph::bind(double* arena) {
           oz.k_chunk = (int)gp.k_chunk;
           oz.splits = gp.splits;
           oz.c_split_stride = (int64_t)oz.M * oz.N;
+          if (gp.fused) {
+            oz.C = nullptr;
+            oz.epi = (L->ac && L->bc) ? 2 : (L->ac ? 1 : 0);
+            oz.cm = g.cm;
+            oz.cn = g.cn;
+            oz.out_re = C.re;
+            oz.out_im = n.cplx ? C.im : nullptr;
+            oz.alpha_re = n.ar;
+            oz.alpha_im = n.ai;
+            if (!L->ac && L->bc) throw Error("internal: complex operand not on the row side");
+          }
           GemmScatterArgs& sc = L->sc;
           sc.Z = oz.C;
           sc.ldz = oz.ldc;
@@ -1052,6 +1188,7 @@ void Graph::bind(double* arena) {
           su.plane = plane;
           su.scale = reinterpret_cast<double*>(scr + ladder_scale_off(o_, v_, a.S));
           su.pmax = reinterpret_cast<double*>(scr + ladder_pmax_off(o_, v_, a.S));
+          su.multipass = split_multipass();
           OzGemmArgs& oz = a.oz;
           oz.A = su.D;
           oz.B = rt_->vpack8[n.stage];
@@ -1134,6 +1271,10 @@ void Graph::bind(double* arena) {
       default: break;
     }
   }
+  if (cache_launched) {
+    TDCC_HIP(hipGetLastError());
+    TDCC_HIP(hipStreamSynchronize(rt_->stream));
+  }
 }
 
 static DgemmArgs ladder_gemm(const LadderArgs& a) {
@@ -1167,10 +1308,10 @@ void Graph::run(hipStream_t s) const {
       case Launch::GETT:
         if (L->g.M == 0 || L->g.N == 0) break;
         if (L->gp.on && L->gp.S > 0) {
-          launch_ozaki_split(L->sa, L->gp.a_cols, s);
-          launch_ozaki_split(L->sb, L->gp.b_cols, s);
+          if (L->gp.a_cache < 0) launch_ozaki_split(L->sa, L->gp.a_cols, s);
+          if (L->gp.b_cache < 0) launch_ozaki_split(L->sb, L->gp.b_cols, s);
           launch_ozgemm(L->oz, s);
-          launch_gemm_scatter(L->sc, s);
+          if (!L->gp.fused) launch_gemm_scatter(L->sc, s);
           break;
         }
         if (L->gp.on) {
@@ -1178,6 +1319,10 @@ void Graph::run(hipStream_t s) const {
           launch_gather2d(L->gb, s);
           launch_dgemm(L->dg, L->gp.bt, s);
           launch_gemm_scatter(L->sc, s);
+          break;
+        }
+        if (L->skinny) {
+          launch_skinny(L->sk, s);
           break;
         }
         launch_gett(L->g, L->ac, L->bc, L->akc, L->bkc, s);
@@ -1224,17 +1369,28 @@ void Graph::run_profiled(hipStream_t s, std::vector<LaunchRecord>& out) const {
           if (L->gp.on && L->gp.S > 0) {
             const GemmPath& gp = L->gp;
             const double abytes = (8.0 + gp.S) * L->sa.planes * (double)L->sa.Rb * L->sa.K;
-            timed(gp.a_cols ? "ozaki_split_cols_kernel" : "ozaki_split_rows_kernel", 0.0, abytes,
-                  [&] { launch_ozaki_split(L->sa, gp.a_cols, s); });
+            auto split_detail = [&](const SplitArgs& x, const char* side) {
+              std::snprintf(dt, sizeof dt, "%s %s R=%lld K=%lld", nodes_[L->node].spec.c_str(), side,
+                            (long long)(x.planes * x.Rb), (long long)x.K);
+              detail = dt;
+            };
+            if (gp.a_cache < 0) {
+              split_detail(L->sa, "A");
+              timed(gp.a_cols ? "ozaki_split_cols_kernel" : "ozaki_split_rows_kernel", 0.0, abytes,
+                    [&] { launch_ozaki_split(L->sa, gp.a_cols, s); });
+            }
             const double bbytes = (8.0 + gp.S) * L->sb.planes * (double)L->sb.Rb * L->sb.K;
-            timed(gp.b_cols ? "ozaki_split_cols_kernel" : "ozaki_split_rows_kernel", 0.0, bbytes,
-                  [&] { launch_ozaki_split(L->sb, gp.b_cols, s); });
+            if (gp.b_cache < 0) {
+              split_detail(L->sb, "B");
+              timed(gp.b_cols ? "ozaki_split_cols_kernel" : "ozaki_split_rows_kernel", 0.0, bbytes,
+                    [&] { launch_ozaki_split(L->sb, gp.b_cols, s); });
+            }
             std::snprintf(dt, sizeof dt, "%s rows=%d N=%d K=%d splits=%d digits=%d", nodes_[L->node].spec.c_str(),
                           L->oz.M, L->oz.N, L->g.K, gp.splits, gp.S);
             detail = dt;
             timed("ozgemm_kernel", fl, 0.0, [&] { launch_ozgemm(L->oz, s); });
             const double sbytes = 8.0 * L->sc.M * (double)L->sc.N * (L->sc.kind == 2 ? 4 : 2) * L->sc.splits;
-            timed("gemm_scatter_kernel", 0.0, sbytes, [&] { launch_gemm_scatter(L->sc, s); });
+            if (!gp.fused) timed("gemm_scatter_kernel", 0.0, sbytes, [&] { launch_gemm_scatter(L->sc, s); });
             break;
           }
           if (L->gp.on) {
@@ -1249,6 +1405,18 @@ void Graph::run_profiled(hipStream_t s, std::vector<LaunchRecord>& out) const {
                   [&] { launch_dgemm(L->dg, L->gp.bt, s); });
             const double sbytes = 8.0 * L->sc.M * (double)L->sc.N * (L->sc.kind == 2 ? 4 : 2) * L->sc.splits;
             timed("gemm_scatter_kernel", 0.0, sbytes, [&] { launch_gemm_scatter(L->sc, s); });
+            break;
+          }
+          if (L->skinny) {
+            const SkinnyArgs& k = L->sk;
+            std::snprintf(dt, sizeof dt, "%s I=%d J=%lld K=%lld kmajor=%d S=%d", nodes_[L->node].spec.c_str(), k.I,
+                          (long long)k.J, (long long)k.K, (int)k.g_kmajor, k.splits);
+            detail = dt;
+            const double pl = (L->ac || L->bc) ? 2.0 : 1.0;
+            const double bytes = 8.0 * ((L->small_is_a ? (L->ac ? 2 : 1) : (L->bc ? 2 : 1)) * (double)k.I * k.K +
+                                        (L->small_is_a ? (L->bc ? 2 : 1) : (L->ac ? 2 : 1)) * (double)k.J * k.K +
+                                        pl * k.I * (double)k.J);
+            timed(k.g_kmajor ? "skinny_k_kernel" : "skinny_rows_kernel", fl, bytes, [&] { launch_skinny(k, s); });
             break;
           }
           std::snprintf(dt, sizeof dt, "%s M=%d N=%d K=%d B=%d S=%d", nodes_[L->node].spec.c_str(), L->g.M, L->g.N,

@@ -187,6 +187,16 @@ class Graph {
   std::vector<char> live_;
   int ones_id_ = -1;
   int fp64_depth_ = 0;
+  // digit planes of static operands (split precision), filled at bind
+  bool is_static(int id) const;
+  struct DigitCache {
+    int64_t digits_off = 0, scale_off = 0;
+    bool filled = false;
+  };
+  std::vector<DigitCache> cache_;
+  std::map<std::string, int> cache_index_;
+  int64_t cache_bytes_ = 0;
+  char* cache_buf_ = nullptr;
   double* ones_buf_ = nullptr;
 };
 

@@ -213,6 +213,8 @@ struct SplitArgs {
   int64_t plane;
   double* scale;        // R row scales 2^e
   double* pmax;         // workspace: ozaki_split_workspace(R, Kp, by_columns) bytes
+  bool multipass;       // always use the three-pass kernels (else single-pass when rows fit)
+  bool interleave;      // with 2 planes: row r is plane r & 1 of base row r >> 1 (else plane r / Rb)
 };
 // Three launches: partial row maxima over k tiles, row scales, digits.
 // by_columns: threads run along r (for sources whose row index is the
@@ -229,10 +231,43 @@ struct OzGemmArgs {
   int M, N, Kp, S;
   int k_chunk, splits;                // split-K over grid.y (k_chunk multiple of 64), partials
   int64_t c_split_stride;             // c_split_stride apart
+  // Fused scatter epilogue (epi >= 0, one split only): instead of C, write
+  //   out[cm[m] + cn[n]] = alpha * value(m, n)
+  // with the planes of A (and B) interleaved by row:
+  //   epi 0 (real x real):       value = Z[m][n]
+  //   epi 1 (complex x real):    value = Z[2m][n] + i Z[2m+1][n]
+  //   epi 2 (complex x complex): value = (Z[2m][2n] - Z[2m+1][2n+1]) + i (Z[2m][2n+1] + Z[2m+1][2n])
+  int epi = -1;
+  const int64_t* cm = nullptr; const int64_t* cn = nullptr;
+  double* out_re = nullptr; double* out_im = nullptr;
+  double alpha_re = 1.0, alpha_im = 0.0;
 };
 void launch_ozgemm(const OzGemmArgs& a, hipStream_t s);
 // Largest K range one int32 accumulator may sum exactly for S digits.
 int64_t ozgemm_max_k(int S);
+
+// ---------------------------------------------------------------------------
+// Skinny contraction (one operand with at most kSkinnyMax rows), FP64 VALU:
+//   C[cs[i] + cb[j]] = alpha * sum_k S[i,k] G[j,k],
+//   S[i,k] = s[sr[i] + sk[k]] (I <= kSkinnyMax), G[j,k] = g[gr[j] + gk[k]].
+// g_kmajor: G's contiguous index lies in k.  s_im / g_im null for real
+// operands.  splits > 1: split s covers k in [s k_chunk, (s+1) k_chunk) and
+// writes partial[s][plane][i][j], reduced in split order.
+constexpr int kSkinnyMax = 32;
+struct SkinnyArgs {
+  const double* s_re; const double* s_im;
+  const double* g_re; const double* g_im;
+  const int64_t* sr; const int64_t* sk;
+  const int64_t* gr; const int64_t* gk;
+  const int64_t* cs; const int64_t* cb;
+  double* c_re; double* c_im;
+  int I; int64_t J, K;
+  int64_t k_chunk; int splits;
+  double* partial;
+  double alpha_re, alpha_im;
+  bool g_kmajor;
+};
+void launch_skinny(const SkinnyArgs& a, hipStream_t s);
 
 // Interleaved complex <-> two planes.
 void launch_split_complex(const double* inter, double* re, double* im, int64_t n, hipStream_t s);

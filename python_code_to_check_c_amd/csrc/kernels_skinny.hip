@@ -1,0 +1,190 @@
+// Skinny tensor contractions on gfx950: one operand has at most 32 rows.
+//
+//   C[cs[i] + cb[j]] = alpha * sum_k S[i, k] * G[j, k]
+//   S[i, k] = s_plane[sr[i] + sk[k]]   (small operand, i < I <= 32)
+//   G[j, k] = g_plane[gr[j] + gk[k]]   (big operand, j < J)
+//
+// These shapes (matrix-vector products with t1, outer products over a short
+// index, and dot products over o^2 v^2 or o v^3) are HBM-bound: the big
+// operand is streamed once and the small one sits in LDS.  FP64 VALU, fixed
+// summation order, optional split over K with the partial sums reduced in
+// split order.
+#include "kernels.hpp"
+
+namespace tdcc {
+
+constexpr int SK_KC = 64;  // k chunk staged in LDS
+
+// Small-operand chunk in LDS, [i][k] with a padded row so that lanes reading
+// consecutive k hit distinct banks.
+template <bool SC, int IT>
+struct SmallChunk {
+  double re[IT][SK_KC + 1];
+  double im[SC ? IT : 1][SK_KC + 1];
+};
+
+template <bool SC, int IT>
+__device__ __forceinline__ void load_small(SmallChunk<SC, IT>& sm, const SkinnyArgs& a, int64_t k0, int64_t k1) {
+  for (int e = threadIdx.x; e < IT * SK_KC; e += blockDim.x) {
+    const int i = e / SK_KC, kk = e - i * SK_KC;
+    const int64_t k = k0 + kk;
+    const bool ok = i < a.I && k < k1;
+    const int64_t off = ok ? a.sr[i] + a.sk[k] : 0;
+    sm.re[i][kk] = ok ? a.s_re[off] : 0.0;
+    if (SC) sm.im[i][kk] = ok ? a.s_im[off] : 0.0;
+  }
+}
+
+// Store (or write a partial of) the IT accumulators of big row j.
+template <bool SC, bool GC, int IT>
+__device__ __forceinline__ void store_row(const SkinnyArgs& a, int64_t j, const double* acc_re, const double* acc_im) {
+  constexpr bool CC = SC || GC;
+  if (a.splits > 1) {
+    double* p = a.partial + (int64_t)blockIdx.y * (CC ? 2 : 1) * a.I * a.J;
+#pragma unroll
+    for (int i = 0; i < IT; ++i)
+      if (i < a.I) {
+        p[(int64_t)i * a.J + j] = acc_re[i];
+        if (CC) p[(int64_t)(a.I + i) * a.J + j] = acc_im[i];
+      }
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < IT; ++i)
+    if (i < a.I) {
+      const int64_t o = a.cs[i] + a.cb[j];
+      const double yr = a.alpha_re * acc_re[i] - (CC ? a.alpha_im * acc_im[i] : 0.0);
+      a.c_re[o] = yr;
+      if (CC) a.c_im[o] = a.alpha_re * acc_im[i] + a.alpha_im * acc_re[i];
+    }
+}
+
+// The big operand's contiguous index is a row index: one thread per big row,
+// 256 rows per block, loop over k (reads coalesced across threads).
+template <bool SC, bool GC, int IT>
+__global__ __launch_bounds__(256) void skinny_rows_kernel(SkinnyArgs a) {
+  __shared__ SmallChunk<SC, IT> sm;
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const bool live = j < a.J;
+  const int64_t kb = (int64_t)blockIdx.y * a.k_chunk, ke = min(a.K, kb + a.k_chunk);
+  const double* gre = a.g_re + (live ? a.gr[j] : 0);
+  const double* gim = GC ? a.g_im + (live ? a.gr[j] : 0) : nullptr;
+  double acc_re[IT], acc_im[IT];
+#pragma unroll
+  for (int i = 0; i < IT; ++i) acc_re[i] = acc_im[i] = 0.0;
+  for (int64_t k0 = kb; k0 < ke; k0 += SK_KC) {
+    __syncthreads();
+    load_small<SC, IT>(sm, a, k0, ke);
+    __syncthreads();
+    const int kn = (int)min((int64_t)SK_KC, ke - k0);
+    if (!live) continue;
+    for (int kk = 0; kk < kn; ++kk) {
+      const int64_t go = a.gk[k0 + kk];
+      const double br = gre[go];
+      const double bi = GC ? gim[go] : 0.0;
+#pragma unroll
+      for (int i = 0; i < IT; ++i) {
+        const double sr = sm.re[i][kk];
+        const double si = SC ? sm.im[i][kk] : 0.0;
+        acc_re[i] = fma(sr, br, acc_re[i]);
+        if (SC && GC) acc_re[i] = fma(-si, bi, acc_re[i]);
+        if (GC) acc_im[i] = fma(sr, bi, acc_im[i]);
+        if (SC) acc_im[i] = fma(si, br, acc_im[i]);
+      }
+    }
+  }
+  if (live) store_row<SC, GC, IT>(a, j, acc_re, acc_im);
+}
+
+// The big operand's contiguous index lies in k: one wave per big row, lanes
+// over k, then a lane reduction.  4 rows per block share the LDS chunk.
+template <bool SC, bool GC, int IT>
+__global__ __launch_bounds__(256) void skinny_k_kernel(SkinnyArgs a) {
+  __shared__ SmallChunk<SC, IT> sm;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t j = (int64_t)blockIdx.x * 4 + w;
+  const bool live = j < a.J;
+  const int64_t kb = (int64_t)blockIdx.y * a.k_chunk, ke = min(a.K, kb + a.k_chunk);
+  const double* gre = a.g_re + (live ? a.gr[j] : 0);
+  const double* gim = GC ? a.g_im + (live ? a.gr[j] : 0) : nullptr;
+  double acc_re[IT], acc_im[IT];
+#pragma unroll
+  for (int i = 0; i < IT; ++i) acc_re[i] = acc_im[i] = 0.0;
+  for (int64_t k0 = kb; k0 < ke; k0 += SK_KC) {
+    __syncthreads();
+    load_small<SC, IT>(sm, a, k0, ke);
+    __syncthreads();
+    const int64_t k = k0 + lane;
+    if (!live || k >= ke) continue;
+    const int64_t go = a.gk[k];
+    const double br = gre[go];
+    const double bi = GC ? gim[go] : 0.0;
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+      const double sr = sm.re[i][lane];
+      const double si = SC ? sm.im[i][lane] : 0.0;
+      acc_re[i] = fma(sr, br, acc_re[i]);
+      if (SC && GC) acc_re[i] = fma(-si, bi, acc_re[i]);
+      if (GC) acc_im[i] = fma(sr, bi, acc_im[i]);
+      if (SC) acc_im[i] = fma(si, br, acc_im[i]);
+    }
+  }
+  // butterfly reduction: every lane ends with the same, order-fixed sum
+#pragma unroll
+  for (int i = 0; i < IT; ++i)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      acc_re[i] += __shfl_xor(acc_re[i], off);
+      if (SC || GC) acc_im[i] += __shfl_xor(acc_im[i], off);
+    }
+  if (live && lane == 0) store_row<SC, GC, IT>(a, j, acc_re, acc_im);
+}
+
+__global__ void skinny_reduce_kernel(SkinnyArgs a, int cplx) {
+  const int64_t total = (int64_t)a.I * a.J;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e / a.J, j = e - i * a.J;
+    double vr = 0.0, vi = 0.0;
+    for (int s = 0; s < a.splits; ++s) {
+      const double* p = a.partial + (int64_t)s * (cplx ? 2 : 1) * total;
+      vr += p[e];
+      if (cplx) vi += p[total + e];
+    }
+    const int64_t o = a.cs[i] + a.cb[j];
+    a.c_re[o] = a.alpha_re * vr - a.alpha_im * vi;
+    if (cplx) a.c_im[o] = a.alpha_re * vi + a.alpha_im * vr;
+  }
+}
+
+template <bool SC, bool GC, int IT>
+static void skinny_launch(const SkinnyArgs& a, hipStream_t s) {
+  const int64_t per = a.g_kmajor ? 4 : 256;
+  const dim3 grid((unsigned)((a.J + per - 1) / per), (unsigned)a.splits), block(256);
+  if (a.g_kmajor) hipLaunchKernelGGL((skinny_k_kernel<SC, GC, IT>), grid, block, 0, s, a);
+  else hipLaunchKernelGGL((skinny_rows_kernel<SC, GC, IT>), grid, block, 0, s, a);
+}
+
+template <bool SC, bool GC>
+static void skinny_dispatch(const SkinnyArgs& a, hipStream_t s) {
+  if (a.I <= 4) skinny_launch<SC, GC, 4>(a, s);
+  else if (a.I <= 8) skinny_launch<SC, GC, 8>(a, s);
+  else if (a.I <= 16) skinny_launch<SC, GC, 16>(a, s);
+  else skinny_launch<SC, GC, 32>(a, s);
+}
+
+void launch_skinny(const SkinnyArgs& a, hipStream_t s) {
+  if (a.I <= 0 || a.J <= 0) return;
+  const bool sc = a.s_im != nullptr, gc = a.g_im != nullptr;
+  if (sc && gc) skinny_dispatch<true, true>(a, s);
+  else if (sc) skinny_dispatch<true, false>(a, s);
+  else if (gc) skinny_dispatch<false, true>(a, s);
+  else skinny_dispatch<false, false>(a, s);
+  if (a.splits > 1) {
+    const int64_t total = (int64_t)a.I * a.J;
+    int64_t b = (total + 255) / 256;
+    if (b > 4096) b = 4096;
+    hipLaunchKernelGGL(skinny_reduce_kernel, dim3((unsigned)b), dim3(256), 0, s, a, (int)(sc || gc));
+  }
+}
+
+}  // namespace tdcc

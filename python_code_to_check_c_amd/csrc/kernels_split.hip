@@ -28,7 +28,8 @@ constexpr int SR_TR = 16, SR_TK = 256;  // rows orientation tile
 constexpr int SC_TR = 64, SC_TK = 64;   // columns orientation tile
 
 __device__ __forceinline__ const double* split_row_base(const SplitArgs& a, int64_t r) {
-  const int64_t p = r / a.Rb, rb = r - p * a.Rb;
+  const bool il = a.interleave && a.planes == 2;
+  const int64_t p = il ? (r & 1) : r / a.Rb, rb = il ? (r >> 1) : r - p * a.Rb;
   const double* src = p ? a.src_im : a.src_re;
   return src + (a.rtab ? a.rtab[rb] : rb * a.ldx);
 }
@@ -157,6 +158,92 @@ __global__ __launch_bounds__(256) void split_digits_cols_kernel(SplitArgs a) {
     *reinterpret_cast<uint4*>(d + s * a.plane) = make_uint4(out[0][s], out[1][s], out[2][s], out[3][s]);
 }
 
+// Single-pass splits for rows that fit in registers: the block reads its
+// rows once, reduces the maximum, and writes the digits from registers.
+// Rows orientation: one row per block, thread t holds k = 1024 j + 4 t + (0..3)
+// for j < NJ (Kp <= 1024 NJ).  Columns orientation: 64 rows per block, thread
+// (row r, phase g) holds k = 64 j + 16 g + (0..15) for j < NJ (Kp <= 64 NJ).
+template <int S, int NJ>
+__global__ __launch_bounds__(256) void split_rows_resident_kernel(SplitArgs a) {
+  __shared__ double red[4];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int64_t r = blockIdx.x;
+  const double* base = split_row_base(a, r);
+  double v[NJ][4];
+  double m = 0.0;
+#pragma unroll
+  for (int j = 0; j < NJ; ++j)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int64_t k = 1024 * (int64_t)j + 4 * tid + q;
+      v[j][q] = k < a.K ? base[split_koff(a, k)] : 0.0;
+      m = fmax(m, abs_or_inf(v[j][q]));
+    }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  if (lane == 0) red[w] = m;
+  __syncthreads();
+  m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+  const bool finite = std::isfinite(m);
+  const int e = finite ? (m > 0.0 ? ilogb(m) + 2 : 0) : 0;
+  if (tid == 0) a.scale[r] = finite ? ldexp(1.0, e) : NAN;
+  int8_t* drow = a.D + r * a.Kp;
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const int64_t k = 1024 * (int64_t)j + 4 * tid;
+    if (k >= a.Kp) break;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[j][q] = finite ? ldexp(v[j][q], -e) : 0.0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) *reinterpret_cast<uint32_t*>(drow + s * a.plane + k) = next_digit4(v[j]);
+  }
+}
+
+template <int S, int NJ>
+__global__ __launch_bounds__(256) void split_cols_resident_kernel(SplitArgs a) {
+  __shared__ double red[4][SC_TR];
+  const int64_t R = (int64_t)a.planes * a.Rb;
+  const int rl = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int64_t r = (int64_t)blockIdx.x * SC_TR + rl;
+  const bool live = r < R;
+  const double* base = live ? split_row_base(a, r) : nullptr;
+  double v[NJ][16];
+  double m = 0.0;
+#pragma unroll
+  for (int j = 0; j < NJ; ++j)
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int64_t k = 64 * (int64_t)j + 16 * g + q;
+      v[j][q] = (live && k < a.K) ? base[split_koff(a, k)] : 0.0;
+      m = fmax(m, abs_or_inf(v[j][q]));
+    }
+  red[g][rl] = m;
+  __syncthreads();
+  if (!live) return;
+  m = fmax(fmax(red[0][rl], red[1][rl]), fmax(red[2][rl], red[3][rl]));
+  const bool finite = std::isfinite(m);
+  const int e = finite ? (m > 0.0 ? ilogb(m) + 2 : 0) : 0;
+  if (g == 0) a.scale[r] = finite ? ldexp(1.0, e) : NAN;
+  int8_t* drow = a.D + r * a.Kp;
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const int64_t k0 = 64 * (int64_t)j + 16 * g;
+    if (k0 >= a.Kp) break;
+    uint32_t out[4][S];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      double x[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) x[i] = finite ? ldexp(v[j][4 * q + i], -e) : 0.0;
+#pragma unroll
+      for (int s = 0; s < S; ++s) out[q][s] = next_digit4(x);
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+      *reinterpret_cast<uint4*>(drow + s * a.plane + k0) = make_uint4(out[0][s], out[1][s], out[2][s], out[3][s]);
+  }
+}
+
 int64_t ozaki_split_workspace(int64_t R, int64_t Kp, bool by_columns) {
   const int64_t tk = by_columns ? SC_TK : SR_TK;
   return ((Kp + tk - 1) / tk) * R * (int64_t)sizeof(double);
@@ -165,9 +252,26 @@ int64_t ozaki_split_workspace(int64_t R, int64_t Kp, bool by_columns) {
 template <int S>
 static void split_launch(const SplitArgs& a, bool by_columns, hipStream_t s) {
   const int64_t R = (int64_t)a.planes * a.Rb;
+  const dim3 block(256);
+  if (!a.multipass && !by_columns && a.Kp <= 16384) {
+    const dim3 grid((unsigned)R);
+    if (a.Kp <= 1024) hipLaunchKernelGGL((split_rows_resident_kernel<S, 1>), grid, block, 0, s, a);
+    else if (a.Kp <= 2048) hipLaunchKernelGGL((split_rows_resident_kernel<S, 2>), grid, block, 0, s, a);
+    else if (a.Kp <= 4096) hipLaunchKernelGGL((split_rows_resident_kernel<S, 4>), grid, block, 0, s, a);
+    else if (a.Kp <= 8192) hipLaunchKernelGGL((split_rows_resident_kernel<S, 8>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((split_rows_resident_kernel<S, 16>), grid, block, 0, s, a);
+    return;
+  }
+  if (!a.multipass && by_columns && a.Kp <= 256) {
+    const dim3 grid((unsigned)((R + SC_TR - 1) / SC_TR));
+    if (a.Kp <= 64) hipLaunchKernelGGL((split_cols_resident_kernel<S, 1>), grid, block, 0, s, a);
+    else if (a.Kp <= 128) hipLaunchKernelGGL((split_cols_resident_kernel<S, 2>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((split_cols_resident_kernel<S, 4>), grid, block, 0, s, a);
+    return;
+  }
   const int64_t tr = by_columns ? SC_TR : SR_TR, tk = by_columns ? SC_TK : SR_TK;
   const int64_t kt = (a.Kp + tk - 1) / tk;
-  const dim3 grid((unsigned)((R + tr - 1) / tr), (unsigned)kt), block(256);
+  const dim3 grid((unsigned)((R + tr - 1) / tr), (unsigned)kt);
   if (by_columns) hipLaunchKernelGGL(split_max_cols_kernel, grid, block, 0, s, a);
   else hipLaunchKernelGGL(split_max_rows_kernel, grid, block, 0, s, a);
   hipLaunchKernelGGL(split_scale_kernel, dim3((unsigned)((R + 255) / 256)), block, 0, s, a, kt);
@@ -294,22 +398,76 @@ __global__ __launch_bounds__(256) void ozgemm_kernel(OzGemmArgs g) {
     buf ^= 1;
   }
   // digit group q (pairs with s + t = q + 2, 1-based digits) weighs 128^-(q+2)
+  if (g.epi < 0) {
+#pragma unroll
+    for (int j = 0; j < JB; ++j) {
+      const int col = n0 + wn + j * 16 + lr;
+      if (col >= g.N) continue;
+      const double sbc = g.sb[col];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wm + i * 16 + 4 * lk + r;
+          if (row >= g.M) continue;
+          double v = 0.0;
+#pragma unroll
+          for (int q = S - 1; q >= 0; --q) v += ldexp((double)acc[q][i][j][r], -7 * (q + 2));
+          Cout[(int64_t)row * g.ldc + col] = v * (g.sa[row] * sbc);
+        }
+    }
+    return;
+  }
+  // Fused scatter.  A lane's four rows 4 lk + r start at an even row, so the
+  // interleaved (re, im) rows 2m, 2m+1 of A sit in one lane (r = 0,1 and
+  // 2,3); the interleaved columns 2n, 2n+1 of B sit in lanes lr and lr ^ 1.
+  const double ar = g.alpha_re, ai = g.alpha_im;
 #pragma unroll
   for (int j = 0; j < JB; ++j) {
     const int col = n0 + wn + j * 16 + lr;
-    if (col >= g.N) continue;
-    const double sbc = g.sb[col];
+    const bool colok = col < g.N;
+    const double sbc = colok ? g.sb[col] : 0.0;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 4; ++i) {
+      const int row0 = m0 + wm + i * 16 + 4 * lk;
+      double z[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm + i * 16 + 4 * lk + r;
-        if (row >= g.M) continue;
         double v = 0.0;
 #pragma unroll
         for (int q = S - 1; q >= 0; --q) v += ldexp((double)acc[q][i][j][r], -7 * (q + 2));
-        Cout[(int64_t)row * g.ldc + col] = v * (g.sa[row] * sbc);
+        z[r] = (colok && row0 + r < g.M) ? v * (g.sa[row0 + r] * sbc) : 0.0;
       }
+      if (g.epi == 0) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (colok && row0 + r < g.M) g.out_re[g.cm[row0 + r] + g.cn[col]] = ar * z[r];
+      } else if (g.epi == 1) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          if (!colok || row0 + 2 * h >= g.M) continue;
+          const double re = z[2 * h], im = z[2 * h + 1];
+          const int64_t o = g.cm[(row0 >> 1) + h] + g.cn[col];
+          g.out_re[o] = ar * re - ai * im;
+          g.out_im[o] = ar * im + ai * re;
+        }
+      } else {
+        double w[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) w[r] = __shfl_xor(z[r], 1);
+        if ((lr & 1) == 0 && colok) {
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            if (row0 + 2 * h >= g.M) continue;
+            const double re = z[2 * h] - w[2 * h + 1];   // rr - ii
+            const double im = w[2 * h] + z[2 * h + 1];   // ri + ir
+            const int64_t o = g.cm[(row0 >> 1) + h] + g.cn[col >> 1];
+            g.out_re[o] = ar * re - ai * im;
+            g.out_im[o] = ar * im + ai * re;
+          }
+        }
+      }
+    }
   }
 }
 

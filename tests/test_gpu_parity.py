@@ -23,11 +23,15 @@ RHS = [("T1eq_rhs", _lib.RHS_T1), ("T2eq_rhs", _lib.RHS_T2), ("lam_1eq_rhs", _li
        ("L2eq_rhs_CC2", _lib.RHS_L2_CC2)]
 
 
-# Engine modes: the default lowering (small contractions on the strided MFMA
-# kernel), every contraction through the dense-GEMM path, and the dense-GEMM
-# path with its K dimension split over 3 partial products.
-MODES = {"default": {}, "gemm": {"TDCC_GEMM_MIN_FLOPS": "0"},
+# Engine modes: the default lowering; every skinny contraction (one side with
+# at most 32 rows) on the streaming VALU kernel; no skinny path (everything
+# small on the strided MFMA kernel); every batch-free contraction through the
+# dense-GEMM path; the dense-GEMM path with its K dimension split over 3
+# partial products.
+MODES = {"default": {}, "skinny": {"TDCC_SKINNY_MIN": "0"}, "gett": {"TDCC_SKINNY_MIN": "-1"},
+         "gemm": {"TDCC_GEMM_MIN_FLOPS": "0"},
          "gemm_splitk": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_GEMM_SPLITS": "3"}}
+ENV_KEYS = ("TDCC_GEMM_MIN_FLOPS", "TDCC_GEMM_SPLITS", "TDCC_SKINNY_MIN")
 
 
 @pytest.fixture(scope="module", params=[(n, m) for m in MODES for n in ("o4v6", "c1_o10v4")],
@@ -36,7 +40,7 @@ def case(request):
     name, mode = request.param
     g = dict(np.load(os.path.join(HERE, "golden", f"golden_{name}.npz")))
     print(f"\n[case {name} engine={mode}]", flush=True)
-    saved = {k: os.environ.get(k) for k in ("TDCC_GEMM_MIN_FLOPS", "TDCC_GEMM_SPLITS")}
+    saved = {k: os.environ.get(k) for k in ENV_KEYS}
     for k in saved:
         os.environ.pop(k, None)
     os.environ.update(MODES[mode])

@@ -30,22 +30,35 @@ def bound(S):
     return 30.0 * 128.0 ** -S
 
 
-@pytest.fixture(scope="module", params=["o4v6", "c1_o10v4"])
+# The digit split has single-pass kernels (rows held in registers) and
+# three-pass kernels (partial maxima, scales, digits); "multipass" forces the
+# latter everywhere.
+MODES = {"default": {"TDCC_GEMM_MIN_FLOPS": "0"},
+         "multipass": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_SPLIT_MULTIPASS": "1"}}
+
+
+@pytest.fixture(scope="module", params=[(n, m) for m in MODES for n in ("o4v6", "c1_o10v4")],
+                ids=lambda p: f"{p[0]}-{p[1]}")
 def case(request):
-    g = dict(np.load(os.path.join(HERE, "golden", f"golden_{request.param}.npz")))
-    print(f"\n[split case {request.param}]", flush=True)
-    saved = os.environ.get("TDCC_GEMM_MIN_FLOPS")
-    os.environ["TDCC_GEMM_MIN_FLOPS"] = "0"
+    name, mode = request.param
+    g = dict(np.load(os.path.join(HERE, "golden", f"golden_{name}.npz")))
+    print(f"\n[split case {name} {mode}]", flush=True)
+    keys = ("TDCC_GEMM_MIN_FLOPS", "TDCC_SPLIT_MULTIPASS")
+    saved = {k: os.environ.get(k) for k in keys}
+    for k in keys:
+        os.environ.pop(k, None)
+    os.environ.update(MODES[mode])
     ctx = _lib.Context(int(g["o"]), int(g["v"]))
     ctx.set_system(g["TEI"], g["F"], g["mu"])
     try:
         yield g, ctx
     finally:
         ctx.close()
-        if saved is None:
-            os.environ.pop("TDCC_GEMM_MIN_FLOPS", None)
-        else:
-            os.environ["TDCC_GEMM_MIN_FLOPS"] = saved
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
 
 
 @pytest.mark.parametrize("S", [2, 3, 4, 5, 6])
