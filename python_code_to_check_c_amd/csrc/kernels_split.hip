@@ -94,6 +94,8 @@ __global__ __launch_bounds__(256) void split_max_cols_kernel(SplitArgs a) {
     a.pmax[(int64_t)blockIdx.y * R + r] = fmax(fmax(red[0][rl], red[1][rl]), fmax(red[2][rl], red[3][rl]));
 }
 
+// Row scales from the partial maxima.  Few k tiles: one thread per row.
+// Many k tiles (long rows): one 256-thread block per row.
 __global__ __launch_bounds__(256) void split_scale_kernel(SplitArgs a, int64_t ktiles) {
   const int64_t R = (int64_t)a.planes * a.Rb;
   const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -101,6 +103,53 @@ __global__ __launch_bounds__(256) void split_scale_kernel(SplitArgs a, int64_t k
   double m = 0.0;
   for (int64_t t = 0; t < ktiles; ++t) m = fmax(m, a.pmax[t * R + r]);
   a.scale[r] = std::isfinite(m) ? ldexp(1.0, m > 0.0 ? ilogb(m) + 2 : 0) : NAN;
+}
+
+__global__ __launch_bounds__(256) void split_scale_long_kernel(SplitArgs a, int64_t ktiles) {
+  __shared__ double red[4];
+  const int64_t R = (int64_t)a.planes * a.Rb;
+  const int64_t r = blockIdx.x;
+  double m = 0.0;
+  for (int64_t t = threadIdx.x; t < ktiles; t += 256) m = fmax(m, a.pmax[t * R + r]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    a.scale[r] = std::isfinite(m) ? ldexp(1.0, m > 0.0 ? ilogb(m) + 2 : 0) : NAN;
+  }
+}
+
+// Rows of at most 256 digits: one wave per row, 4 rows per block, thread
+// (lane l) holds k = 4 l + (0..3).
+template <int S>
+__global__ __launch_bounds__(256) void split_rows_wave_kernel(SplitArgs a) {
+  const int64_t R = (int64_t)a.planes * a.Rb;
+  const int lane = threadIdx.x & 63;
+  const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= R) return;  // whole wave leaves together
+  const double* base = split_row_base(a, r);
+  double v[4];
+  double m = 0.0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int64_t k = 4 * lane + q;
+    v[q] = k < a.K ? base[split_koff(a, k)] : 0.0;
+    m = fmax(m, abs_or_inf(v[q]));
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  const bool finite = std::isfinite(m);
+  const int e = finite ? (m > 0.0 ? ilogb(m) + 2 : 0) : 0;
+  if (lane == 0) a.scale[r] = finite ? ldexp(1.0, e) : NAN;
+  const int64_t k = 4 * lane;
+  if (k >= a.Kp) return;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) v[q] = finite ? ldexp(v[q], -e) : 0.0;
+  int8_t* d = a.D + r * a.Kp + k;
+#pragma unroll
+  for (int s = 0; s < S; ++s) *reinterpret_cast<uint32_t*>(d + s * a.plane) = next_digit4(v);
 }
 
 template <int S>
@@ -253,6 +302,10 @@ template <int S>
 static void split_launch(const SplitArgs& a, bool by_columns, hipStream_t s) {
   const int64_t R = (int64_t)a.planes * a.Rb;
   const dim3 block(256);
+  if (!a.multipass && !by_columns && a.Kp <= 256) {
+    hipLaunchKernelGGL(split_rows_wave_kernel<S>, dim3((unsigned)((R + 3) / 4)), block, 0, s, a);
+    return;
+  }
   if (!a.multipass && !by_columns && a.Kp <= 16384) {
     const dim3 grid((unsigned)R);
     if (a.Kp <= 1024) hipLaunchKernelGGL((split_rows_resident_kernel<S, 1>), grid, block, 0, s, a);
@@ -274,7 +327,8 @@ static void split_launch(const SplitArgs& a, bool by_columns, hipStream_t s) {
   const dim3 grid((unsigned)((R + tr - 1) / tr), (unsigned)kt);
   if (by_columns) hipLaunchKernelGGL(split_max_cols_kernel, grid, block, 0, s, a);
   else hipLaunchKernelGGL(split_max_rows_kernel, grid, block, 0, s, a);
-  hipLaunchKernelGGL(split_scale_kernel, dim3((unsigned)((R + 255) / 256)), block, 0, s, a, kt);
+  if (kt > 64) hipLaunchKernelGGL(split_scale_long_kernel, dim3((unsigned)R), block, 0, s, a, kt);
+  else hipLaunchKernelGGL(split_scale_kernel, dim3((unsigned)((R + 255) / 256)), block, 0, s, a, kt);
   if (by_columns) hipLaunchKernelGGL(split_digits_cols_kernel<S>, grid, block, 0, s, a);
   else hipLaunchKernelGGL(split_digits_rows_kernel<S>, grid, block, 0, s, a);
 }
