@@ -137,9 +137,16 @@ def main():
                     help="time only the split-precision leg (profiling runs; no FP64 comparison)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy kernel over MIB MiB (HBM counter calibration) and exit")
+    ap.add_argument("--pmc-summary", nargs=5, default=None,
+                    metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV", "COPY_MIB"),
+                    help="only turn rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE counter files (workload and copy "
+                         "calibration) into the per-kernel HBM traffic table on stdout, and exit")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+        return
+    if args.pmc_summary:
+        pmc_summary(*args.pmc_summary)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding of this "
@@ -223,6 +230,9 @@ def main():
                 for r in sorted(recs, key=lambda r: -r[3]):
                     f.write("\t".join(str(x) for x in r) + "\n")
         name, roof, groups = roofline(recs, args.digits)
+        roof["traffic"] = traffic_of(name)
+        if roof["traffic"] is not None:
+            roof["traffic_source"] = os.path.relpath(TRAFFIC_TABLE, ROOT)
         out["roofline"] = roof
         top = sorted(groups.items(), key=lambda kv: -kv[1][2])[:6]
         out["config"]["profile_top"] = {k: {"ms_per_step": g[2], "calls": g[3],
@@ -247,6 +257,55 @@ def calibrate_copy(mib):
         torch.mul(x, 1.0, out=y)
     torch.cuda.synchronize()
     log(f"copy calibration: 3 launches of y = x * 1.0, {mib} MiB read + {mib} MiB written each")
+
+
+TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "hbm_traffic_tdccsd_c3_o20_v160.txt")
+
+
+def _counter_rows(path):
+    import csv
+    with open(path) as f:
+        return [(r["Kernel_Name"].split("(")[0].replace("void ", ""), float(r["Counter_Value"]))
+                for r in csv.DictReader(f)]
+
+
+def pmc_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, copy_mib):
+    """Per-kernel HBM bytes per launch from FETCH_SIZE / WRITE_SIZE (KiB per
+    dispatch), each scaled so that the copy kernel's counter equals the bytes
+    it moves (copy_mib MiB each way)."""
+    true_kib = float(copy_mib) * 1024.0
+    cf = [v for n, v in _counter_rows(copy_fetch_csv) if "MulFunctor" in n]
+    cw = [v for n, v in _counter_rows(copy_write_csv) if "MulFunctor" in n]
+    f_scale, w_scale = true_kib / np.mean(cf), true_kib / np.mean(cw)
+    acc = {}
+    for (n, fv), (n2, wv) in zip(_counter_rows(fetch_csv), _counter_rows(write_csv)):
+        if n != n2:
+            raise RuntimeError("the FETCH_SIZE and WRITE_SIZE runs dispatched different kernels")
+        a = acc.setdefault(n, [0, 0.0, 0.0])
+        a[0] += 1
+        a[1] += fv * f_scale * 1024.0
+        a[2] += wv * w_scale * 1024.0
+    print(f"# FETCH_SIZE scale {f_scale:.4f}, WRITE_SIZE scale {w_scale:.4f} (copy kernel, {copy_mib} MiB each way)")
+    print("# kernel\tlaunches\tfetch_bytes_per_launch\twrite_bytes_per_launch")
+    for n, (c, fb, wb) in sorted(acc.items(), key=lambda kv: -(kv[1][1] + kv[1][2])):
+        print(f"{n}\t{c}\t{fb / c:.6e}\t{wb / c:.6e}")
+
+
+def traffic_of(kernel_name):
+    """HBM bytes per launch of a kernel from the committed counter table (the
+    base name, e.g. ozgemm_kernel, matches tdcc::ozgemm_kernel<4, 4>), or None."""
+    if not os.path.exists(TRAFFIC_TABLE):
+        return None
+    base = kernel_name.split("<")[0]
+    best = None
+    with open(TRAFFIC_TABLE) as f:
+        for line in f:
+            if line.startswith("#") or not line.strip():
+                continue
+            n, c, fb, wb = line.rstrip("\n").split("\t")
+            if n.split("::")[-1].split("<")[0] == base and (best is None or int(c) > best[0]):
+                best = (int(c), float(fb) + float(wb))
+    return best[1] if best else None
 
 
 def log(msg):

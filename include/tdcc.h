@@ -107,6 +107,22 @@ int tdcc_dipole(tdcc_ctx* ctx, const double* t1, const double* t2, const double*
 int tdcc_rk4_run(tdcc_ctx* ctx, int method, int nsteps, double t0, double dt, double w0,
                  double A, int pol, int precs, double* mu_out, double* t_out);
 
+/* Batched trajectories of this system (many field polarisations, frequencies
+ * and amplitudes; the reference runs one trajectory per Runge_Kutta_solver
+ * call, CCSD_Helper.py:1191).  tdcc_set_batch allocates nb state slots
+ * (0 frees them); slots are filled and read like the single state. */
+int tdcc_set_batch(tdcc_ctx* ctx, int nb);
+int tdcc_set_state_slot(tdcc_ctx* ctx, int slot, const double* t1, const double* t2, const double* l1,
+                        const double* l2);
+int tdcc_get_state_slot(tdcc_ctx* ctx, int slot, int which, double* t1, double* t2, double* l1, double* l2);
+
+/* The loop body of tdcc_rk4_run for every slot at once: slot b uses the field
+ * -A[b] mu[pol[b]] exp(i 2 pi w0[b] t); all slots share t0, dt and precs.  One
+ * captured graph advances all slots by one step.  mu_out: nb x nsteps x 3
+ * complex (slot-major); t_out: nsteps times. */
+int tdcc_rk4_run_batch(tdcc_ctx* ctx, int method, int nsteps, double t0, double dt, const int* pol,
+                       const double* w0, const double* A, int precs, double* mu_out, double* t_out);
+
 /* Take one step of the same program eagerly with a HIP event pair around
  * every kernel launch; `report` receives one line per launch:
  * "<kernel name>\t<algorithmic flops>\t<algorithmic bytes>\t<milliseconds>\n". */

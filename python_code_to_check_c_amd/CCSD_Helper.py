@@ -27,7 +27,10 @@ class CCSD_Helper(object):
     method = "CCSD"
     time_digits = 4
 
-    def __init__(self, psi, device=0):
+    def __init__(self, psi, device=0, digits=0):
+        """`digits` > 0 runs the large contractions in split precision with that
+        many int8 digits per operand (include/tdcc.h, tdcc_set_precision); the
+        default 0 keeps the reference's FP64 arithmetic throughout."""
         self.system = psi
         o, v = int(psi.nocc), int(psi.nvir)
         self.nocc, self.nvir = o, v
@@ -45,6 +48,8 @@ class CCSD_Helper(object):
             cid, sigma = psi.tei_config
             self.ctx.synth_tei(cid, sigma)
         self._F_loaded = np.asarray(psi.F, dtype=float)
+        self.digits = int(digits)
+        self.ctx.set_precision(self.digits)
 
     # ------------------------------------------------------------ system
     def F_MO(self, H=None, C=None):

@@ -16,9 +16,10 @@ class CC_Calculator(object):
 
     def __init__(self, psi, **kwargs):
         self.psi = psi
-        self.mol = CCSD_Helper(psi)
-        self.ndocc = self.mol.ndocc
         self.tol = kwargs.pop("tol", 1e-10)
+        self.digits = int(kwargs.pop("digits", 0))  # split-precision propagation (0: FP64)
+        self.mol = CCSD_Helper(psi, digits=self.digits)
+        self.ndocc = self.mol.ndocc
         try:
             self.w0 = kwargs['w0']
             self.A = kwargs['A']
@@ -43,7 +44,7 @@ class CC_Calculator(object):
         """CC_Calculator.py:70-145: resume from the files a stopped run saved."""
         r = restart_io.read_restart()
         if r["method"] == 'CC2':
-            mol = CC2_Helper(self.psi)
+            mol = CC2_Helper(self.psi, digits=self.digits)
             mol.Runge_Kutta_solver_CC2(r["F"], r["t1"], r["t2"], r["lam1"], r["lam2"], r["w0"], r["A"],
                                        r["t0"], r["tf"], r["dt"], timeout, r["precs"], 'restart')
         elif r["method"] == 'CCSD':
@@ -53,10 +54,16 @@ class CC_Calculator(object):
         self.last = mol
 
     def _ground_state(self, mol):
-        F = mol.F_MO()
-        t1, t2, _ = ground_state.solve_t(mol, F, tol=self.tol, verbose=True)
-        print("E_corr =", mol.CCSD_Corr_E(t1, t2, F))
-        lam1, lam2, _ = ground_state.solve_lambda(mol, F, t1, t2, tol=self.tol, verbose=True)
+        # the ground state is always converged in FP64; only the propagation
+        # uses the split precision
+        mol.ctx.set_precision(0)
+        try:
+            F = mol.F_MO()
+            t1, t2, _ = ground_state.solve_t(mol, F, tol=self.tol, verbose=True)
+            print("E_corr =", mol.CCSD_Corr_E(t1, t2, F))
+            lam1, lam2, _ = ground_state.solve_lambda(mol, F, t1, t2, tol=self.tol, verbose=True)
+        finally:
+            mol.ctx.set_precision(mol.digits)
         return F, t1, t2, lam1, lam2
 
     def TDCCSD(self, timeout):
@@ -69,7 +76,7 @@ class CC_Calculator(object):
 
     def TDCC2(self, timeout):
         """CC_Calculator.py:223-267"""
-        mol = CC2_Helper(self.psi)
+        mol = CC2_Helper(self.psi, digits=self.digits)
         F, t1, t2, lam1, lam2 = self._ground_state(mol)
         mol.Runge_Kutta_solver_CC2(F, t1, t2, lam1, lam2, self.w0, self.A, self.t0, self.tf, self.dt,
                                    timeout, self.precs)

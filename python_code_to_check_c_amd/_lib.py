@@ -17,6 +17,7 @@ EXPORTS = (
     "tdcc_create", "tdcc_destroy", "tdcc_last_error", "tdcc_set_system", "tdcc_synth_tei",
     "tdcc_set_precision", "tdcc_set_state", "tdcc_get_state", "tdcc_rhs", "tdcc_intermediate", "tdcc_dipole",
     "tdcc_rk4_run", "tdcc_profile_step", "tdcc_checkpoint", "tdcc_step_info",
+    "tdcc_set_batch", "tdcc_set_state_slot", "tdcc_get_state_slot", "tdcc_rk4_run_batch",
 )
 
 RHS_T1, RHS_T2, RHS_L1, RHS_L2, RHS_T2_CC2, RHS_L1_CC2, RHS_L2_CC2 = range(7)
@@ -61,6 +62,11 @@ def load():
                                              ctypes.c_int64]),
         "tdcc_checkpoint": (ctypes.c_int, [vp, ctypes.c_int]),
         "tdcc_step_info": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, i64p, i64p]),
+        "tdcc_set_batch": (ctypes.c_int, [vp, ctypes.c_int]),
+        "tdcc_set_state_slot": (ctypes.c_int, [vp, ctypes.c_int, _dp, _dp, _dp, _dp]),
+        "tdcc_get_state_slot": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]),
+        "tdcc_rk4_run_batch": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                              ctypes.POINTER(ctypes.c_int), _dp, _dp, ctypes.c_int, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -216,6 +222,43 @@ class Context:
             name, fl, by, ms, detail = line.split("\t")
             out.append((name, float(fl), float(by), float(ms), detail))
         return out
+
+    # ------------------------------------------------------------- batches
+    def set_batch(self, nb):
+        self._check(self.lib.tdcc_set_batch(self._h, int(nb)), "tdcc_set_batch")
+        self.nb = int(nb)
+
+    def set_state_slot(self, slot, t1, t2, lam1, lam2):
+        a, pa = _c(t1)
+        b, pb = _c(t2)
+        c, pc = _c(lam1)
+        d, pd = _c(lam2)
+        self._shape_ok(a, b)
+        self._shape_ok(c, d)
+        self._check(self.lib.tdcc_set_state_slot(self._h, int(slot), pa, pb, pc, pd), "tdcc_set_state_slot")
+
+    def get_state_slot(self, slot, which=0):
+        o, v = self.o, self.v
+        out = [np.empty((o, v), complex), np.empty((o, o, v, v), complex),
+               np.empty((o, v), complex), np.empty((o, o, v, v), complex)]
+        self._check(self.lib.tdcc_get_state_slot(self._h, int(slot), int(which),
+                                                 *[x.ctypes.data_as(_dp) for x in out]), "tdcc_get_state_slot")
+        return out
+
+    def rk4_run_batch(self, method, nsteps, t0, dt, pol, w0, A, precs=15):
+        """Advance every batch slot by nsteps steps; slot b uses pol[b], w0[b], A[b].
+        Returns mu (nb, nsteps, 3) complex and the nsteps times."""
+        m = METHODS[method] if isinstance(method, str) else int(method)
+        nb = self.nb
+        pol_ = (ctypes.c_int * nb)(*[int(x) for x in pol])
+        w0_, pw0 = _r(np.asarray(w0, dtype=float).reshape(nb))
+        A_, pA = _r(np.asarray(A, dtype=float).reshape(nb))
+        mu = np.empty((nb, max(nsteps, 1), 3), complex)
+        ts = np.empty(max(nsteps, 1))
+        self._check(self.lib.tdcc_rk4_run_batch(self._h, m, int(nsteps), float(t0), float(dt), pol_, pw0, pA,
+                                                int(precs), mu.ctypes.data_as(_dp), ts.ctypes.data_as(_dp)),
+                    "tdcc_rk4_run_batch")
+        return mu[:, :nsteps], ts[:nsteps]
 
     def checkpoint(self, op):
         self._check(self.lib.tdcc_checkpoint(self._h, int(op)), "tdcc_checkpoint")

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <array>
 #include <map>
 #include <memory>
 #include <string>
@@ -35,6 +36,8 @@ struct Program {
   int pol = -1;
   const void* table = nullptr;
   const void* mu_out = nullptr;
+  bool batched = false;
+  std::vector<Runtime::Traj> batch;  // trajectory parameters the records were bound with
 };
 
 }  // namespace
@@ -63,6 +66,13 @@ struct tdcc_ctx {
   unsigned long long* counts = nullptr;    // kCountBlocks + 1 partial counts
   Runtime rt;
   std::map<std::string, Program> progs;
+  // batched trajectories of this system (tdcc_set_batch): per-slot state and
+  // rounded base, per-slot field tables and dipole rows ([slot][cap][3][2])
+  int nb = 0;
+  std::vector<std::array<Planes, 4>> bstate, bbase;
+  double* btable = nullptr;
+  double* bmu = nullptr;
+  int64_t bcap = 0;
 };
 
 namespace {
@@ -170,7 +180,8 @@ void ensure_arena(tdcc_ctx* c, int64_t bytes) {
 
 void bind(tdcc_ctx* c, Program& p) {
   const bool stale = !p.bound || p.negA != c->rt.negA || p.dt6 != c->rt.dt6 || p.scale != c->rt.round_scale ||
-                     p.pol != c->rt.pol || p.table != c->table || p.mu_out != c->mu_out;
+                     p.pol != c->rt.pol || p.table != c->table || p.mu_out != c->mu_out ||
+                     (p.batched && p.batch != c->rt.batch);
   if (!stale) return;
   invalidate_exec(p);
   p.g->bind(c->arena);
@@ -181,6 +192,7 @@ void bind(tdcc_ctx* c, Program& p) {
   p.pol = c->rt.pol;
   p.table = c->table;
   p.mu_out = c->mu_out;
+  if (p.batched) p.batch = c->rt.batch;
 }
 
 Program& finish_program(tdcc_ctx* c, const std::string& key, std::unique_ptr<Graph> g, bool step) {
@@ -332,6 +344,31 @@ Program& step_program(tdcc_ctx* c, int method, double dt) {
   for (int k = 0; k < 4; ++k) io.ynew[k] = ext_planes(*g, c->state[k], shape_of_block(k), "ynew");
   build_rk4_step(*g, io, method, dt);
   return finish_program(c, key, std::move(g), true);
+}
+
+// One program advancing every batch slot by one step: the slots' step
+// graphs side by side (integral-only nodes shared), one device counter.
+Program& batch_step_program(tdcc_ctx* c, int method, double dt) {
+  const std::string key = "bstep:" + std::to_string(method) + ":" + hexd(dt) + ":" + std::to_string(c->nb);
+  auto it = c->progs.find(key);
+  if (it != c->progs.end()) return it->second;
+  auto g = std::make_unique<Graph>(c->o, c->v, &c->rt);
+  const int tei = ext_tei(c, *g);
+  for (int b = 0; b < c->nb; ++b) {
+    g->set_traj(b);
+    StepIO io;
+    io.tei = tei;
+    for (int k = 0; k < 4; ++k) {
+      io.y[k] = ext_planes(*g, c->bstate[b][k], shape_of_block(k), "y");
+      io.base[k] = ext_planes(*g, c->bbase[b][k], shape_of_block(k), "base");
+    }
+    for (int k = 0; k < 4; ++k) io.ynew[k] = ext_planes(*g, c->bstate[b][k], shape_of_block(k), "ynew");
+    build_rk4_step(*g, io, method, dt);
+  }
+  g->set_traj(-1);
+  Program& p = finish_program(c, key, std::move(g), true);
+  p.batched = true;
+  return p;
 }
 
 void run_program(tdcc_ctx* c, Program& p, bool use_graph) {
@@ -494,19 +531,10 @@ void require_system(tdcc_ctx* c) {
   if (!c->have_system || !c->have_tei) throw Error("system not set: call tdcc_set_system (and tdcc_synth_tei when tei was NULL)");
 }
 
-Program* prepare_run(tdcc_ctx* c, int method, int nsteps, double t0, double dt, double w0, double A, int pol,
-                     int precs, std::vector<double>& times) {
-  require_system(c);
-  if (method != TDCC_CCSD && method != TDCC_CC2) throw Error("method must be TDCC_CCSD or TDCC_CC2");
-  if (nsteps < 0) throw Error("nsteps must be >= 0");
-  if (pol < 0 || pol > 2) throw Error("pol must be 0, 1 or 2");
-  if (precs < 0 || precs > 22) throw Error("precs must be in [0, 22]");
-  if (!(dt > 0.0) || !std::isfinite(dt)) throw Error("dt must be positive");
-  if (nsteps == 0) return nullptr;
-  ensure_rows(c, nsteps);
-  // Field table, evaluated as the reference does: t advances by dt before
-  // the step; stage times t, t + dt/2, t + dt; exp(i * ((w0*2)*pi) * t).
-  std::vector<double> tab((size_t)nsteps * 6);
+// Field table, evaluated as the reference does: t advances by dt before the
+// step; stage times t, t + dt/2, t + dt; exp(i * ((w0*2)*pi) * t).
+void field_table(int nsteps, double t0, double dt, double w0, std::vector<double>& tab, std::vector<double>& times) {
+  tab.assign((size_t)nsteps * 6, 0.0);
   times.assign(nsteps, 0.0);
   double t = t0;
   const double wpi = (w0 * 2.0) * M_PI;
@@ -520,6 +548,39 @@ Program* prepare_run(tdcc_ctx* c, int method, int nsteps, double t0, double dt, 
       tab[(size_t)i * 6 + 2 * s + 1] = std::sin(th);
     }
   }
+}
+
+void check_run_args(int method, int nsteps, int pol, int precs, double dt) {
+  if (method != TDCC_CCSD && method != TDCC_CC2) throw Error("method must be TDCC_CCSD or TDCC_CC2");
+  if (nsteps < 0) throw Error("nsteps must be >= 0");
+  if (pol < 0 || pol > 2) throw Error("pol must be 0, 1 or 2");
+  if (precs < 0 || precs > 22) throw Error("precs must be in [0, 22]");
+  if (!(dt > 0.0) || !std::isfinite(dt)) throw Error("dt must be positive");
+}
+
+void free_batch(tdcc_ctx* c) {
+  for (auto& s : c->bstate)
+    for (auto& p : s) free_planes(p);
+  for (auto& s : c->bbase)
+    for (auto& p : s) free_planes(p);
+  c->bstate.clear();
+  c->bbase.clear();
+  if (c->btable) (void)hipFree(c->btable);
+  if (c->bmu) (void)hipFree(c->bmu);
+  c->btable = c->bmu = nullptr;
+  c->bcap = 0;
+  c->nb = 0;
+  c->rt.batch.clear();
+}
+
+Program* prepare_run(tdcc_ctx* c, int method, int nsteps, double t0, double dt, double w0, double A, int pol,
+                     int precs, std::vector<double>& times) {
+  require_system(c);
+  check_run_args(method, nsteps, pol, precs, dt);
+  if (nsteps == 0) return nullptr;
+  ensure_rows(c, nsteps);
+  std::vector<double> tab;
+  field_table(nsteps, t0, dt, w0, tab, times);
   TDCC_HIP(hipMemcpyAsync(c->table, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   c->rt.negA = -A;
   c->rt.pol = pol;
@@ -597,6 +658,7 @@ void tdcc_destroy(tdcc_ctx* c) {
   free_planes(c->io_out);
   free_planes(c->io_F);
   free_planes(c->int_out);
+  free_batch(c);
   for (double* p : {c->tei, c->F, c->mu, c->staging, c->table, c->mu_out, c->arena})
     if (p) (void)hipFree(p);
   if (c->vpack[1] && c->vpack[1] != c->vpack[0]) (void)hipFree(c->vpack[1]);
@@ -670,6 +732,101 @@ int tdcc_get_state(tdcc_ctx* c, int which, double* t1, double* t2, double* l1, d
     double* h[4] = {t1, t2, l1, l2};
     for (int k = 0; k < 4; ++k)
       if (h[k]) download_complex(c, which == 0 ? c->state[k] : c->base[k], h[k]);
+  });
+}
+
+int tdcc_set_batch(tdcc_ctx* c, int nb) {
+  return guarded(c, [&] {
+    if (nb < 0) throw Error("nb must be >= 0");
+    TDCC_HIP(hipStreamSynchronize(c->stream));
+    for (auto it = c->progs.begin(); it != c->progs.end();) {
+      if (it->second.batched) {
+        invalidate_exec(it->second);
+        it = c->progs.erase(it);
+      } else {
+        ++it;
+      }
+    }
+    free_batch(c);
+    c->bstate.resize(nb);
+    c->bbase.resize(nb);
+    for (int b = 0; b < nb; ++b)
+      for (int k = 0; k < 4; ++k) {
+        alloc_planes(c->bstate[b][k], block_size(c, k));
+        alloc_planes(c->bbase[b][k], block_size(c, k));
+      }
+    c->nb = nb;
+    c->rt.batch.assign(nb, Runtime::Traj{});
+  });
+}
+
+int tdcc_set_state_slot(tdcc_ctx* c, int slot, const double* t1, const double* t2, const double* l1,
+                        const double* l2) {
+  return guarded(c, [&] {
+    if (slot < 0 || slot >= c->nb) throw Error("no such batch slot");
+    const double* h[4] = {t1, t2, l1, l2};
+    for (int k = 0; k < 4; ++k) {
+      if (!h[k]) throw Error("all four amplitude arrays are required");
+      upload_complex(c, h[k], c->bstate[slot][k]);
+    }
+  });
+}
+
+int tdcc_get_state_slot(tdcc_ctx* c, int slot, int which, double* t1, double* t2, double* l1, double* l2) {
+  return guarded(c, [&] {
+    if (slot < 0 || slot >= c->nb) throw Error("no such batch slot");
+    if (which != 0 && which != 1) throw Error("which must be 0 (state) or 1 (rounded base)");
+    double* h[4] = {t1, t2, l1, l2};
+    for (int k = 0; k < 4; ++k)
+      if (h[k]) download_complex(c, which == 0 ? c->bstate[slot][k] : c->bbase[slot][k], h[k]);
+  });
+}
+
+int tdcc_rk4_run_batch(tdcc_ctx* c, int method, int nsteps, double t0, double dt, const int* pol,
+                       const double* w0, const double* A, int precs, double* mu_out, double* t_out) {
+  return guarded(c, [&] {
+    require_system(c);
+    if (c->nb == 0) throw Error("no batch slots: call tdcc_set_batch");
+    if (!pol || !w0 || !A) throw Error("pol, w0 and A arrays are required");
+    for (int b = 0; b < c->nb; ++b) check_run_args(method, nsteps, pol[b], precs, dt);
+    if (nsteps == 0) return;
+    if (nsteps > c->bcap) {
+      TDCC_HIP(hipStreamSynchronize(c->stream));
+      if (c->btable) TDCC_HIP(hipFree(c->btable));
+      if (c->bmu) TDCC_HIP(hipFree(c->bmu));
+      c->btable = c->bmu = nullptr;
+      c->bcap = 0;
+      const int64_t cap = std::max<int64_t>(nsteps, 1024);
+      TDCC_HIP(hipMalloc(&c->btable, (size_t)c->nb * cap * 6 * sizeof(double)));
+      TDCC_HIP(hipMalloc(&c->bmu, (size_t)c->nb * cap * 6 * sizeof(double)));
+      c->bcap = cap;
+    }
+    std::vector<double> tab, times;
+    for (int b = 0; b < c->nb; ++b) {
+      field_table(nsteps, t0, dt, w0[b], tab, times);
+      double* dst = c->btable + (size_t)b * c->bcap * 6;
+      TDCC_HIP(hipMemcpyAsync(dst, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      Runtime::Traj& tr = c->rt.batch[b];
+      tr.pol = pol[b];
+      tr.negA = -A[b];
+      tr.field_table = dst;
+      tr.mu_out = c->bmu + (size_t)b * c->bcap * 6;
+    }
+    c->rt.dt6 = dt / 6.0;
+    double scale = 1.0;
+    for (int k = 0; k < precs; ++k) scale *= 10.0;
+    c->rt.round_scale = scale;
+    Program& p = batch_step_program(c, method, dt);
+    launch_counter_set(c->counter, 0, c->stream);
+    const bool use_graph = graphs_enabled();
+    for (int i = 0; i < nsteps; ++i) run_program(c, p, use_graph);
+    TDCC_HIP(hipGetLastError());
+    if (mu_out)
+      for (int b = 0; b < c->nb; ++b)
+        TDCC_HIP(hipMemcpyAsync(mu_out + (size_t)b * nsteps * 6, c->bmu + (size_t)b * c->bcap * 6,
+                                (size_t)nsteps * 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    TDCC_HIP(hipStreamSynchronize(c->stream));
+    if (t_out) std::memcpy(t_out, times.data(), nsteps * sizeof(double));
   });
 }
 

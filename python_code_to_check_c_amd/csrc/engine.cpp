@@ -396,7 +396,8 @@ int Graph::field(int stage) {
   n.shape.cls[0] = n.shape.cls[1] = Cls::N;
   n.cplx = true;
   n.stage = stage;
-  n.key = "F|" + std::to_string(stage);
+  n.traj = traj_;
+  n.key = "F|" + std::to_string(stage) + "|" + std::to_string(traj_);
   return add(std::move(n));
 }
 
@@ -429,6 +430,7 @@ void Graph::finalize_rk(int y_ext, int base_ext, const int k[4], bool lambda) {
 void Graph::trace(int dij, int l1, int dai, int dab) {
   Node n;
   n.op = Op::Trace;
+  n.traj = traj_;
   n.k[0] = dij;
   n.k[1] = l1;
   n.k[2] = dai;
@@ -1128,6 +1130,7 @@ void Graph::bind(double* arena) {
         for (size_t t0 = 0; t0 < terms.size(); t0 += kMaxTerms) {
           auto L = std::make_unique<Launch>();
           L->kind = Launch::LIN;
+          L->node = i;
           LincombArgs& l = L->l;
           l.out_re = O.re;
           l.out_im = out_cplx ? O.im : nullptr;
@@ -1214,13 +1217,15 @@ void Graph::bind(double* arena) {
         auto L = std::make_unique<Launch>();
         L->kind = Launch::FIELD;
         const Dev O = resolve(i);
+        if (n.traj >= (int)rt_->batch.size()) throw Error("internal: trajectory without parameters");
+        const bool bt = n.traj >= 0;
         L->f.F = rt_->F;
-        L->f.mu_pol = rt_->mu + (int64_t)rt_->pol * n2;
+        L->f.mu_pol = rt_->mu + (int64_t)(bt ? rt_->batch[n.traj].pol : rt_->pol) * n2;
         L->f.out_re = O.re;
         L->f.out_im = O.im;
         L->f.n2 = n2;
-        L->f.negA = rt_->negA;
-        L->f.table = rt_->field_table;
+        L->f.negA = bt ? rt_->batch[n.traj].negA : rt_->negA;
+        L->f.table = bt ? rt_->batch[n.traj].field_table : rt_->field_table;
         L->f.counter = rt_->counter;
         L->f.stage = n.stage;
         launches_.push_back(std::move(L));
@@ -1263,7 +1268,8 @@ void Graph::bind(double* arena) {
         L->tr.mu = rt_->mu;
         L->tr.o = o_;
         L->tr.v = v_;
-        L->tr.out = rt_->mu_out;
+        if (n.traj >= (int)rt_->batch.size()) throw Error("internal: trajectory without parameters");
+        L->tr.out = n.traj >= 0 ? rt_->batch[n.traj].mu_out : rt_->mu_out;
         L->tr.counter = rt_->counter;
         launches_.push_back(std::move(L));
         break;
@@ -1430,6 +1436,17 @@ void Graph::run_profiled(hipStream_t s, std::vector<LaunchRecord>& out) const {
         }
         case Launch::LIN: {
           const double bytes = 8.0 * L->l.total * ((L->l.out_im ? 2 : 1) * (1 + L->l.accumulate) + 2 * L->l.nterms);
+          char dt[160];
+          int perm = 0;  // terms read through a non-identity permutation
+          for (int k = 0; k < L->l.nterms; ++k)
+            for (int d = 0; d + 1 < L->l.rank; ++d)
+              if (L->l.t[k].st[d] != 0 && L->l.t[k].st[d] < L->l.t[k].st[d + 1]) {
+                ++perm;
+                break;
+              }
+          std::snprintf(dt, sizeof dt, "node %d terms=%d permuted=%d n=%lld acc=%d", L->node, L->l.nterms, perm,
+                        (long long)L->l.total, L->l.accumulate);
+          detail = dt;
           timed("lincomb_kernel", 0.0, bytes, [&] { launch_lincomb(L->l, s); });
           break;
         }

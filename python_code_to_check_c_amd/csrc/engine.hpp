@@ -62,6 +62,9 @@ struct Node {
   int64_t vst[4] = {0, 0, 0, 0};
   // Field
   int stage = -1;
+  // Field / Trace of a batched program: trajectory index into Runtime::batch
+  // (-1: the runtime's single-trajectory parameters)
+  int traj = -1;
   // External
   Dev ext;
   // Finalize: inputs k[4], target external `a`, base external `b`
@@ -96,6 +99,17 @@ struct Runtime {
   int64_t* counter = nullptr;
   double* mu_out = nullptr;         // [steps][3][2]
   double dt6 = 0.0, round_scale = 1e15;
+  // per-trajectory field and dipole parameters of batched programs
+  struct Traj {
+    int pol = 2;
+    double negA = 0.0;
+    const double* field_table = nullptr;  // [steps][3][2]
+    double* mu_out = nullptr;             // [steps][3][2]
+    bool operator==(const Traj& o) const {
+      return pol == o.pol && negA == o.negA && field_table == o.field_table && mu_out == o.mu_out;
+    }
+  };
+  std::vector<Traj> batch;
   // packed antisymmetric <ab||ef>: vpack[0][q(ab)][q(ef)] = <ab||ef>,
   // vpack[1][q(ab)][q(ef)] = <ef||ab> (P = v(v-1)/2 pairs a < b)
   const double* vpack[2] = {nullptr, nullptr};
@@ -131,6 +145,9 @@ class Graph {
   int contract3(const std::string& spec, int a, int b, int c, double ar = 1.0);
   int lincomb(std::vector<Term> terms);
   int field(int stage);
+  // Field and Trace nodes created from now on belong to trajectory b of a
+  // batched program (-1: not batched).
+  void set_traj(int b) { traj_ = b; }
   // Particle-particle ladder over the antisymmetric integral block:
   //   ket == 0: out_ijab = sum_ef <ab||ef> u_ijef   ('abef,ijef->ijab')
   //   ket == 1: out_ijab = sum_ef <ef||ab> u_ijef   ('efab,ijef->ijab')
@@ -187,6 +204,7 @@ class Graph {
   std::vector<char> live_;
   int ones_id_ = -1;
   int fp64_depth_ = 0;
+  int traj_ = -1;
   // digit planes of static operands (split precision), filled at bind
   bool is_static(int id) const;
   struct DigitCache {

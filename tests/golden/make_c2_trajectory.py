@@ -10,8 +10,9 @@ w0 = 0.923, A = 0.002, precs = 15 with the oracle propagator (which matches the
 reference's Runge_Kutta_solver, see tests/test_oracle_golden.py).
 
 Stored (compressed): the initial amplitudes, the dipole (x, y, z) after every
-step, the exact times, and the final amplitudes.  Antisymmetric (o,o,v,v)
-arrays are stored as their i<j, a<b elements.
+step, the exact times, and the final t1 / lam1 (the final doubles are summarized
+by their antisymmetry defect).  Antisymmetric (o,o,v,v) arrays are stored as
+their i<j, a<b elements.
 """
 from __future__ import annotations
 
@@ -76,8 +77,6 @@ def main():
         ft1, ft2, fl1, fl2 = tr.state
         rec[f"{method}_t1"] = ft1
         rec[f"{method}_l1"] = fl1
-        rec[f"{method}_t2p"] = pack4(ft2)
-        rec[f"{method}_l2p"] = pack4(fl2)
         rec[f"{method}_t2_asym"] = np.abs(ft2 + ft2.transpose(1, 0, 2, 3)).max()
     path = os.path.join(HERE, "c2_trajectory.npz")
     np.savez_compressed(path, **rec)

@@ -57,3 +57,31 @@ def test_rk4_trajectory(name, method):
     assert np.abs(tr.mu - ref_mu[1:]).max() <= TOL
     for k, key in enumerate(("t1", "t2", "lam1", "lam2")):
         assert np.abs(tr.state[k] - g[f"rk_{method}_{key}"][-1]).max() <= TOL, key
+
+
+def test_c2_trajectory_fixture_matches_oracle():
+    """The first steps of the committed o=10, v=38 trajectories are reproduced
+    by the oracle from the stored initial amplitudes (the full 1000 steps take
+    about an hour on the CPU; tests/golden/make_c2_trajectory.py)."""
+    from python_code_to_check_c_amd import synthetic
+    from oracle import rk4
+    g = dict(np.load(os.path.join(HERE, "golden", "c2_trajectory.npz")))
+    o, v = int(g["o"]), int(g["v"])
+    s = synthetic.SyntheticSystem(int(g["config_id"]), o, v, tei_sigma=float(g["sigma"]))
+    eq = CCEquations(s.TEI, o)
+    i, j = np.triu_indices(o, 1)
+    a, b = np.triu_indices(v, 1)
+
+    def unpack(p):
+        x = np.zeros((o, o, v, v), complex)
+        x[i[:, None], j[:, None], a[None, :], b[None, :]] = p
+        x[j[:, None], i[:, None], a[None, :], b[None, :]] = -p
+        x[i[:, None], j[:, None], b[None, :], a[None, :]] = -p
+        x[j[:, None], i[:, None], b[None, :], a[None, :]] = p
+        return x
+
+    for method in ("CCSD", "CC2"):
+        tr = rk4.propagate(eq, s.F, g[f"{method}_t10"], unpack(g[f"{method}_t2p0"]), g[f"{method}_l10"],
+                           unpack(g[f"{method}_l2p0"]), s.mu, float(g["w0"]), float(g["A"]), 0.0,
+                           float(g["dt"]), method=method, max_steps=2)
+        assert np.abs(np.array(tr.mu) - g[f"{method}_mu"][:2]).max() <= 1e-13
