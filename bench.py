@@ -60,7 +60,7 @@ def cpu_baseline(method, o_full, v_full, budget_s=20.0):
     from oracle.cc_equations import CCEquations
     from oracle.flops import step_flops
     from python_code_to_check_c_amd import synthetic
-    o, v = min(o_full, 20), 24
+    o, v = min(o_full, 20), min(v_full, 24)
     s = synthetic.SyntheticSystem(90, o, v, tei_sigma=0.02)
     eq = CCEquations(s.TEI, o)
     t1, t2, l1, l2 = initial_state(s, o, v)
@@ -141,7 +141,16 @@ def main():
                     metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV", "COPY_MIB"),
                     help="only turn rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE counter files (workload and copy "
                          "calibration) into the per-kernel HBM traffic table on stdout, and exit")
+    ap.add_argument("--batched", action="store_true",
+                    help="the batched workload instead: 4 molecule shapes x 3 polarisations x 8 frequencies, "
+                         "one context and one captured step per molecule (prints its own JSON line)")
     args = ap.parse_args()
+    if args.batched:
+        if args.gpus != 1:
+            print("bench.py runs on one GPU only", file=sys.stderr)
+            sys.exit(2)
+        batched(args)
+        return
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
         return
@@ -243,6 +252,64 @@ def main():
         out["cpu_baseline"] = cpu_baseline(args.method, o, v)
     ctx.close()
     print(json.dumps(out), flush=True)
+
+
+BATCH_SHAPES = [(8, 24), (10, 38), (14, 58), (20, 96)]
+
+
+def batched(args):
+    """Batched TDCCSD: for each molecule shape (config id 40 + index) 24
+    trajectories (x/y/z field x 8 frequencies w0 ~ U[0.3, 1.2] from the seeded
+    generator, A = 0.002) advanced together; split precision as --digits.
+    value = trajectory-steps/s summed over the molecules (each molecule's
+    batch timed over --steps steps after --warmup, synchronised)."""
+    from python_code_to_check_c_amd import _lib, synthetic
+    from oracle.flops import step_flops
+    dt, A, precs = 0.001, 0.002, 15
+    per_mol, total_time, total_traj_steps, flops = [], 0.0, 0, 0.0
+    for k, (o, v) in enumerate(BATCH_SHAPES):
+        cid = 40 + k
+        sysm = synthetic.SyntheticSystem(cid, o, v, tei_sigma=args.tei_sigma, tei_on_device=True)
+        ctx = _lib.Context(o, v)
+        ctx.set_system(None, sysm.F, sysm.mu)
+        ctx.synth_tei(cid, args.tei_sigma)
+        ctx.set_precision(args.digits)
+        w0s = synthetic.w0_draws(cid, 8)
+        pol = [p for p in (0, 1, 2) for _ in range(8)]
+        w0 = [float(w) for _ in (0, 1, 2) for w in w0s]
+        amp = [A] * len(pol)
+        state = initial_state(sysm, o, v)
+        ctx.set_batch(len(pol))
+        for b in range(len(pol)):
+            ctx.set_state_slot(b, *state)
+        if args.warmup:
+            ctx.rk4_run_batch(args.method, args.warmup, 0.0, dt, pol, w0, amp, precs)
+            for b in range(len(pol)):
+                ctx.set_state_slot(b, *state)
+        t0 = time.perf_counter()
+        mu, _ = ctx.rk4_run_batch(args.method, args.steps, 0.0, dt, pol, w0, amp, precs)
+        el = time.perf_counter() - t0
+        if not np.all(np.isfinite(mu)):
+            raise RuntimeError("non-finite dipole in the batched run")
+        n_ts = len(pol) * args.steps
+        per_mol.append({"o": o, "v": v, "trajectories": len(pol), "seconds": el,
+                        "trajectory_steps_per_s": n_ts / el})
+        total_time += el
+        total_traj_steps += n_ts
+        flops += len(pol) * step_flops(o, v, args.method)[1]
+        log(f"batch o={o} v={v}: {len(pol)} trajectories x {args.steps} steps in {el:.3f} s")
+        ctx.close()
+    value = total_traj_steps / total_time
+    print(json.dumps({
+        "metric": "batched TDCCSD trajectory-steps/s (4 shapes x 24 trajectories)", "value": value,
+        "unit": "trajectory-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * total_time / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64+int8-split" if args.digits else "f64", "data": "synthetic",
+        "config": {"workload": "batched TDCCSD, shapes (8,24),(10,38),(14,58),(20,96), 3 polarisations x 8 "
+                               "frequencies each, A=0.002, dt=0.001",
+                   "digits": args.digits, "per_molecule": per_mol,
+                   "F_alg_flop_per_batched_step": flops,
+                   "ref_equiv_TFLOPs": flops * args.steps / total_time / 1e12}}), flush=True)
 
 
 def calibrate_copy(mib):
