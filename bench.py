@@ -144,7 +144,16 @@ def main():
     ap.add_argument("--batched", action="store_true",
                     help="the batched workload instead: 4 molecule shapes x 3 polarisations x 8 frequencies, "
                          "one context and one captured step per molecule (prints its own JSON line)")
+    ap.add_argument("--gradient", action="store_true",
+                    help="the differentiable workload instead: o=10, v=38, 500 steps, gradient of the Hann-windowed "
+                         "mu_z with respect to A, w0 and the z dipole matrix (prints its own JSON line)")
     args = ap.parse_args()
+    if args.gradient:
+        if args.gpus != 1:
+            print("bench.py runs on one GPU only", file=sys.stderr)
+            sys.exit(2)
+        gradient(args)
+        return
     if args.batched:
         if args.gpus != 1:
             print("bench.py runs on one GPU only", file=sys.stderr)
@@ -310,6 +319,54 @@ def batched(args):
                    "digits": args.digits, "per_molecule": per_mol,
                    "F_alg_flop_per_batched_step": flops,
                    "ref_equiv_TFLOPs": flops * args.steps / total_time / 1e12}}), flush=True)
+
+
+def gradient(args, o=10, v=38, nsteps=500, ckpt=50):
+    """Windowed-dipole gradient: L = sum_n hann_n dt mu_z(t_n) over 500 steps
+    at o=10, v=38 (config id 2, TEI sigma 0.02, MP2 start), FP64, checkpoints
+    every 50 steps.  Times the forward + adjoint call against one plain
+    propagation, and checks dL/dA and dL/dw0 by central differences
+    (h = 1e-5 relative) of the propagation itself."""
+    from python_code_to_check_c_amd import _lib, synthetic
+    dt, w0, A, precs = 0.001, 0.923, 0.002, 15
+    sysm = synthetic.SyntheticSystem(2, o, v, tei_sigma=0.02)
+    ctx = _lib.Context(o, v)
+    ctx.set_system(sysm.TEI, sysm.F, sysm.mu)
+    X0 = initial_state(sysm, o, v)
+    w = np.hanning(nsteps + 2)[1:-1] * dt
+
+    def loss(A_, w0_):
+        ctx.set_state(*X0)
+        mu, _ = ctx.rk4_run("CCSD", nsteps, 0.0, dt, w0_, A_, 2, precs)
+        return np.sum(w * mu[:, 2])
+
+    loss(A, w0)  # warm the programs
+    ctx.set_state(*X0)
+    ctx.dipole_gradient("CCSD", 2, 0.0, dt, w0, A, w[:2], pol=2, precs=precs, checkpoint_every=ckpt)
+    t0 = time.perf_counter()
+    L0 = loss(A, w0)
+    t_fwd = time.perf_counter() - t0
+    ctx.set_state(*X0)
+    t0 = time.perf_counter()
+    L, dA, dw, dmu = ctx.dipole_gradient("CCSD", nsteps, 0.0, dt, w0, A, w, pol=2, precs=precs,
+                                         checkpoint_every=ckpt)
+    t_grad = time.perf_counter() - t0
+    hA, hw = 1e-5 * A, 1e-5 * w0
+    fdA = (loss(A + hA, w0) - loss(A - hA, w0)) / (2 * hA)
+    fdw = (loss(A, w0 + hw) - loss(A, w0 - hw)) / (2 * hw)
+    ctx.close()
+    c = lambda z: [float(np.real(z)), float(np.imag(z))]
+    print(json.dumps({
+        "metric": "windowed-dipole gradient wall time (forward + adjoint, 500 RK4 steps)", "value": t_grad,
+        "unit": "s", "n_gpus": 1, "steps": nsteps, "warmup": 1, "ms_per_step": 1e3 * t_grad / nsteps,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"TDCCSD o={o}, v={v}, L = sum hann*dt*mu_z over {nsteps} steps, dL/d(A, w0, mu_z), "
+                               f"checkpoints every {ckpt} steps",
+                   "forward_only_s": t_fwd, "gradient_over_forward": t_grad / t_fwd,
+                   "loss": c(L), "loss_forward": c(L0), "dL_dA": c(dA), "dL_dw0": c(dw),
+                   "fd_dL_dA": c(fdA), "fd_dL_dw0": c(fdw),
+                   "rel_err_dA": float(abs(dA - fdA) / abs(fdA)), "rel_err_dw0": float(abs(dw - fdw) / abs(fdw)),
+                   "dL_dmu_max_abs": float(np.abs(dmu).max())}}), flush=True)
 
 
 def calibrate_copy(mib):

@@ -123,6 +123,20 @@ int tdcc_get_state_slot(tdcc_ctx* ctx, int slot, int which, double* t1, double* 
 int tdcc_rk4_run_batch(tdcc_ctx* ctx, int method, int nsteps, double t0, double dt, const int* pol,
                        const double* w0, const double* A, int precs, double* mu_out, double* t_out);
 
+/* Gradient of the windowed dipole L = sum_n weights[n] mu_pol(t_n) over nsteps
+ * RK4 steps from the current state (the reference's loop, CCSD_Helper.py:
+ * 1206-1240, without stop rules), with respect to the field amplitude A, the
+ * frequency w0 and the dipole matrix mu_pol (which enters both the field
+ * -A mu_pol exp(i 2 pi w0 t) and the read-out).  Hand-written adjoint of the
+ * step (transposed contractions; the rounding to precs decimals is treated as
+ * the identity), checkpoints every checkpoint_every steps with recompute of
+ * each segment.  Outputs (complex, interleaved): loss[1], dA[1], dw0[1],
+ * dmu[n*n] (n = o + v).  The state is left at the end of the propagation.
+ * There is no reference counterpart (the reference has no derivatives). */
+int tdcc_dipole_gradient(tdcc_ctx* ctx, int method, int nsteps, double t0, double dt, double w0, double A,
+                         int pol, int precs, const double* weights, int checkpoint_every, double* loss,
+                         double* dA, double* dw0, double* dmu);
+
 /* Take one step of the same program eagerly with a HIP event pair around
  * every kernel launch; `report` receives one line per launch:
  * "<kernel name>\t<algorithmic flops>\t<algorithmic bytes>\t<milliseconds>\n". */

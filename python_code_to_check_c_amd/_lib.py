@@ -18,6 +18,7 @@ EXPORTS = (
     "tdcc_set_precision", "tdcc_set_state", "tdcc_get_state", "tdcc_rhs", "tdcc_intermediate", "tdcc_dipole",
     "tdcc_rk4_run", "tdcc_profile_step", "tdcc_checkpoint", "tdcc_step_info",
     "tdcc_set_batch", "tdcc_set_state_slot", "tdcc_get_state_slot", "tdcc_rk4_run_batch",
+    "tdcc_dipole_gradient",
 )
 
 RHS_T1, RHS_T2, RHS_L1, RHS_L2, RHS_T2_CC2, RHS_L1_CC2, RHS_L2_CC2 = range(7)
@@ -62,6 +63,9 @@ def load():
                                              ctypes.c_int64]),
         "tdcc_checkpoint": (ctypes.c_int, [vp, ctypes.c_int]),
         "tdcc_step_info": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, i64p, i64p]),
+        "tdcc_dipole_gradient": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp,
+                                                ctypes.c_int, _dp, _dp, _dp, _dp]),
         "tdcc_set_batch": (ctypes.c_int, [vp, ctypes.c_int]),
         "tdcc_set_state_slot": (ctypes.c_int, [vp, ctypes.c_int, _dp, _dp, _dp, _dp]),
         "tdcc_get_state_slot": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]),
@@ -259,6 +263,24 @@ class Context:
                                                 int(precs), mu.ctypes.data_as(_dp), ts.ctypes.data_as(_dp)),
                     "tdcc_rk4_run_batch")
         return mu[:, :nsteps], ts[:nsteps]
+
+    # ----------------------------------------------------------- gradients
+    def dipole_gradient(self, method, nsteps, t0, dt, w0, A, weights, pol=2, precs=15, checkpoint_every=50):
+        """L = sum_n weights[n] mu_pol(t_n) over nsteps steps from the current
+        state, and dL/dA, dL/dw0 (complex scalars), dL/dmu_pol (n, n complex).
+        The state is left at the end of the propagation."""
+        m = METHODS[method] if isinstance(method, str) else int(method)
+        w_, pw = _r(np.asarray(weights, dtype=float).reshape(int(nsteps)))
+        loss = np.empty(1, complex)
+        dA = np.empty(1, complex)
+        dw = np.empty(1, complex)
+        dmu = np.empty((self.n, self.n), complex)
+        self._check(self.lib.tdcc_dipole_gradient(self._h, m, int(nsteps), float(t0), float(dt), float(w0), float(A),
+                                                  int(pol), int(precs), pw, int(checkpoint_every),
+                                                  loss.ctypes.data_as(_dp), dA.ctypes.data_as(_dp),
+                                                  dw.ctypes.data_as(_dp), dmu.ctypes.data_as(_dp)),
+                    "tdcc_dipole_gradient")
+        return loss[0], dA[0], dw[0], dmu
 
     def checkpoint(self, op):
         self._check(self.lib.tdcc_checkpoint(self._h, int(op)), "tdcc_checkpoint")

@@ -38,6 +38,9 @@ struct Program {
   const void* mu_out = nullptr;
   bool batched = false;
   std::vector<Runtime::Traj> batch;  // trajectory parameters the records were bound with
+  const void* wtab = nullptr;
+  const void* ttab = nullptr;
+  const void* grad = nullptr;
 };
 
 }  // namespace
@@ -73,6 +76,13 @@ struct tdcc_ctx {
   double* btable = nullptr;
   double* bmu = nullptr;
   int64_t bcap = 0;
+  // adjoint buffers (tdcc_dipole_gradient): state before a step, adjoint in /
+  // out, per-step weights and stage times, gradient accumulators
+  std::array<Planes, 4> adj_y, adj_a, adj_b;
+  double* wtab = nullptr;
+  double* ttab = nullptr;
+  int64_t vcap = 0;
+  double* grad = nullptr;  // [4] dL/dA, dL/dw0 (complex), then (n,n) re and im planes of dL/dmu_pol
 };
 
 namespace {
@@ -181,7 +191,8 @@ void ensure_arena(tdcc_ctx* c, int64_t bytes) {
 void bind(tdcc_ctx* c, Program& p) {
   const bool stale = !p.bound || p.negA != c->rt.negA || p.dt6 != c->rt.dt6 || p.scale != c->rt.round_scale ||
                      p.pol != c->rt.pol || p.table != c->table || p.mu_out != c->mu_out ||
-                     (p.batched && p.batch != c->rt.batch);
+                     (p.batched && p.batch != c->rt.batch) || p.wtab != c->rt.wtab || p.ttab != c->rt.ttab ||
+                     p.grad != c->rt.g;
   if (!stale) return;
   invalidate_exec(p);
   p.g->bind(c->arena);
@@ -193,6 +204,9 @@ void bind(tdcc_ctx* c, Program& p) {
   p.table = c->table;
   p.mu_out = c->mu_out;
   if (p.batched) p.batch = c->rt.batch;
+  p.wtab = c->rt.wtab;
+  p.ttab = c->rt.ttab;
+  p.grad = c->rt.g;
 }
 
 Program& finish_program(tdcc_ctx* c, const std::string& key, std::unique_ptr<Graph> g, bool step) {
@@ -369,6 +383,44 @@ Program& batch_step_program(tdcc_ctx* c, int method, double dt) {
   Program& p = finish_program(c, key, std::move(g), true);
   p.batched = true;
   return p;
+}
+
+// Adjoint programs of the windowed-dipole gradient (tdcc_dipole_gradient):
+// their externals are the context's adjoint buffers.
+Program& vjp_step_program(tdcc_ctx* c, int method, double dt) {
+  const std::string key = "vstep:" + std::to_string(method) + ":" + hexd(dt);
+  auto it = c->progs.find(key);
+  if (it != c->progs.end()) return it->second;
+  auto g = std::make_unique<Graph>(c->o, c->v, &c->rt);
+  StepVjpIO io;
+  io.tei = ext_tei(c, *g);
+  for (int k = 0; k < 4; ++k) {
+    io.y[k] = ext_planes(*g, c->adj_y[k], shape_of_block(k), "y");
+    io.ybar_in[k] = ext_planes(*g, c->adj_a[k], shape_of_block(k), "ybar_in");
+    io.ybar_out[k] = ext_planes(*g, c->adj_b[k], shape_of_block(k), "ybar_out");
+  }
+  build_rk4_step_vjp(*g, io, method, dt);
+  return finish_program(c, key, std::move(g), false);
+}
+
+Program& vjp_readout_program(tdcc_ctx* c) {
+  const std::string key = "vread";
+  auto it = c->progs.find(key);
+  if (it != c->progs.end()) return it->second;
+  auto g = std::make_unique<Graph>(c->o, c->v, &c->rt);
+  ReadoutVjpIO io;
+  io.tei = ext_tei(c, *g);
+  for (int k = 0; k < 4; ++k) {
+    io.y[k] = ext_planes(*g, c->adj_y[k], shape_of_block(k), "y");
+    io.ybar[k] = ext_planes(*g, c->adj_a[k], shape_of_block(k), "ybar");
+  }
+  build_readout_vjp(*g, io);
+  return finish_program(c, key, std::move(g), false);
+}
+
+void copy_state(tdcc_ctx* c, const std::array<Planes, 4>& dst, const Planes* src) {
+  for (int k = 0; k < 4; ++k)
+    TDCC_HIP(hipMemcpyAsync(dst[k].d, src[k].d, 2 * src[k].n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
 }
 
 void run_program(tdcc_ctx* c, Program& p, bool use_graph) {
@@ -659,6 +711,13 @@ void tdcc_destroy(tdcc_ctx* c) {
   free_planes(c->io_F);
   free_planes(c->int_out);
   free_batch(c);
+  for (int k = 0; k < 4; ++k) {
+    free_planes(c->adj_y[k]);
+    free_planes(c->adj_a[k]);
+    free_planes(c->adj_b[k]);
+  }
+  for (double* p : {c->wtab, c->ttab, c->grad})
+    if (p) (void)hipFree(p);
   for (double* p : {c->tei, c->F, c->mu, c->staging, c->table, c->mu_out, c->arena})
     if (p) (void)hipFree(p);
   if (c->vpack[1] && c->vpack[1] != c->vpack[0]) (void)hipFree(c->vpack[1]);
@@ -827,6 +886,140 @@ int tdcc_rk4_run_batch(tdcc_ctx* c, int method, int nsteps, double t0, double dt
                                 (size_t)nsteps * 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     TDCC_HIP(hipStreamSynchronize(c->stream));
     if (t_out) std::memcpy(t_out, times.data(), nsteps * sizeof(double));
+  });
+}
+
+int tdcc_dipole_gradient(tdcc_ctx* c, int method, int nsteps, double t0, double dt, double w0, double A, int pol,
+                         int precs, const double* weights, int checkpoint_every, double* loss, double* dA,
+                         double* dw0, double* dmu) {
+  return guarded(c, [&] {
+    require_system(c);
+    check_run_args(method, nsteps, pol, precs, dt);
+    if (nsteps < 1) throw Error("nsteps must be >= 1");
+    if (!weights) throw Error("weights are required");
+    const int K = checkpoint_every > 0 ? checkpoint_every : nsteps;
+    const int64_t n = c->n, n2 = n * n;
+    // tables: field (as the forward run), stage times, weights
+    std::vector<double> times;
+    Program* fwd = prepare_run(c, method, nsteps, t0, dt, w0, A, pol, precs, times);
+    if (nsteps > c->vcap) {
+      if (c->wtab) TDCC_HIP(hipFree(c->wtab));
+      if (c->ttab) TDCC_HIP(hipFree(c->ttab));
+      c->wtab = c->ttab = nullptr;
+      c->vcap = 0;
+      TDCC_HIP(hipMalloc(&c->wtab, (size_t)nsteps * sizeof(double)));
+      TDCC_HIP(hipMalloc(&c->ttab, (size_t)nsteps * 3 * sizeof(double)));
+      c->vcap = nsteps;
+    }
+    std::vector<double> tt((size_t)nsteps * 3);
+    for (int i = 0; i < nsteps; ++i) {
+      tt[3 * i] = times[i];
+      tt[3 * i + 1] = times[i] + dt / 2.0;
+      tt[3 * i + 2] = times[i] + dt;
+    }
+    TDCC_HIP(hipMemcpyAsync(c->ttab, tt.data(), tt.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    TDCC_HIP(hipMemcpyAsync(c->wtab, weights, (size_t)nsteps * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (!c->grad) TDCC_HIP(hipMalloc(&c->grad, (4 + 2 * n2) * sizeof(double)));
+    for (int k = 0; k < 4; ++k) {
+      if (!c->adj_y[k].d) {
+        alloc_planes(c->adj_y[k], block_size(c, k));
+        alloc_planes(c->adj_a[k], block_size(c, k));
+        alloc_planes(c->adj_b[k], block_size(c, k));
+      }
+    }
+    c->rt.wtab = c->wtab;
+    c->rt.ttab = c->ttab;
+    c->rt.g = c->grad;
+    c->rt.gmu_re = c->grad + 4;
+    c->rt.gmu_im = c->grad + 4 + n2;
+    // checkpoints (every K steps) and one segment of states
+    const int nseg = (nsteps + K - 1) / K;
+    std::vector<std::array<Planes, 4>> ck(nseg), stack(K);
+    auto free_all = [&] {
+      for (auto& s4 : ck)
+        for (auto& p : s4) free_planes(p);
+      for (auto& s4 : stack)
+        for (auto& p : s4) free_planes(p);
+    };
+    try {
+      for (auto* v : {&ck, &stack})
+        for (auto& s4 : *v)
+          for (int k = 0; k < 4; ++k) alloc_planes(s4[k], block_size(c, k));
+      const bool use_graph = graphs_enabled();
+      // forward, keeping the state at every segment start
+      copy_state(c, ck[0], c->state);
+      for (int i = 1; i <= nsteps; ++i) {
+        run_program(c, *fwd, use_graph);
+        if (i % K == 0 && i < nsteps) copy_state(c, ck[i / K], c->state);
+      }
+      std::vector<double> mu((size_t)nsteps * 6);
+      TDCC_HIP(hipMemcpyAsync(mu.data(), c->mu_out, mu.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      std::array<Planes, 4> last;
+      for (int k = 0; k < 4; ++k) alloc_planes(last[k], block_size(c, k));
+      copy_state(c, last, c->state);
+      TDCC_HIP(hipStreamSynchronize(c->stream));
+      double lr = 0.0, li = 0.0;
+      for (int i = 0; i < nsteps; ++i) {
+        lr += weights[i] * mu[(size_t)i * 6 + 2 * pol];
+        li += weights[i] * mu[(size_t)i * 6 + 2 * pol + 1];
+      }
+      // backward
+      TDCC_HIP(hipMemsetAsync(c->grad, 0, (4 + 2 * n2) * sizeof(double), c->stream));
+      for (int k = 0; k < 4; ++k) TDCC_HIP(hipMemsetAsync(c->adj_a[k].d, 0, 2 * c->adj_a[k].n * sizeof(double), c->stream));
+      Program& vstep = vjp_step_program(c, method, dt);
+      Program& vread = vjp_readout_program(c);
+      copy_state(c, c->adj_y, c->state);
+      launch_counter_set(c->counter, nsteps, c->stream);
+      run_program(c, vread, use_graph);
+      for (int j = nseg - 1; j >= 0; --j) {
+        const int n0 = j * K, n1 = std::min(nsteps, n0 + K);
+        copy_state(c, stack[0], ck[j].data());
+        copy_state(c, std::array<Planes, 4>{c->state[0], c->state[1], c->state[2], c->state[3]}, ck[j].data());
+        launch_counter_set(c->counter, n0, c->stream);
+        for (int m = 1; m < n1 - n0; ++m) {
+          run_program(c, *fwd, use_graph);
+          copy_state(c, stack[m], c->state);
+        }
+        for (int s = n1; s > n0; --s) {
+          copy_state(c, c->adj_y, stack[s - 1 - n0].data());
+          launch_counter_set(c->counter, s, c->stream);
+          run_program(c, vstep, use_graph);
+          copy_state(c, c->adj_a, c->adj_b.data());
+          if (s - 1 >= 1) {
+            launch_counter_set(c->counter, s - 1, c->stream);
+            run_program(c, vread, use_graph);
+          }
+        }
+      }
+      TDCC_HIP(hipGetLastError());
+      std::vector<double> gr(4 + 2 * n2);
+      TDCC_HIP(hipMemcpyAsync(gr.data(), c->grad, gr.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      // leave the state at the end of the propagation, as tdcc_rk4_run does
+      copy_state(c, std::array<Planes, 4>{c->state[0], c->state[1], c->state[2], c->state[3]}, last.data());
+      TDCC_HIP(hipStreamSynchronize(c->stream));
+      for (auto& p : last) free_planes(p);
+      if (loss) {
+        loss[0] = lr;
+        loss[1] = li;
+      }
+      if (dA) {
+        dA[0] = gr[0];
+        dA[1] = gr[1];
+      }
+      if (dw0) {
+        dw0[0] = gr[2];
+        dw0[1] = gr[3];
+      }
+      if (dmu)
+        for (int64_t e = 0; e < n2; ++e) {
+          dmu[2 * e] = gr[4 + e];
+          dmu[2 * e + 1] = gr[4 + n2 + e];
+        }
+    } catch (...) {
+      free_all();
+      throw;
+    }
+    free_all();
   });
 }
 

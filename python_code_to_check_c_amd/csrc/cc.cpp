@@ -415,15 +415,16 @@ int CCBuilder::L1_wefam_term(int t1, int t2, int l2, int F) {
 }
 
 // ------------------------------------------------------------ the RK4 step
-void build_rk4_step(Graph& g, const StepIO& io, int method, double dt) {
-  CCBuilder b(g, io.tei);
+// The 16 stage derivatives k[chain][s] of one step of the four decoupled
+// chains (CCSD_Helper.py:1158-1187, CC2_Helper.py:412-433): chain c evaluates
+// its right-hand side at y_c + c_s k[c][s-1] with the other blocks at y.
+static void stage_derivatives(Graph& g, CCBuilder& b, const int y[4], int method, double dt, int k[4][4]) {
   const int Fs[4] = {g.field(0), g.field(1), g.field(1), g.field(2)};
   const double cs[4] = {0.0, dt / 2.0, dt / 2.0, dt};
-  const int t1 = io.y[0], t2 = io.y[1], l1 = io.y[2], l2 = io.y[3];
-  int k[4][4];
+  const int t1 = y[0], t2 = y[1], l1 = y[2], l2 = y[3];
   for (int chain = 0; chain < 4; ++chain) {
     for (int s = 0; s < 4; ++s) {
-      const int y0 = io.y[chain];
+      const int y0 = y[chain];
       const int ys = s == 0 ? y0 : g.lincomb({tm(y0), tm(k[chain][s - 1], cs[s])});
       const int F = Fs[s];
       int r = -1;
@@ -445,9 +446,61 @@ void build_rk4_step(Graph& g, const StepIO& io, int method, double dt) {
       k[chain][s] = r;
     }
   }
+}
+
+void build_rk4_step(Graph& g, const StepIO& io, int method, double dt) {
+  CCBuilder b(g, io.tei);
+  int k[4][4];
+  stage_derivatives(g, b, io.y, method, dt, k);
   for (int chain = 0; chain < 4; ++chain) g.finalize_rk(io.y[chain], io.base[chain], k[chain], chain >= 2);
   const int n1 = io.ynew[0], n2 = io.ynew[1], m1 = io.ynew[2], m2 = io.ynew[3];
   g.trace(b.Dij(n1, n2, m1, m2), m1, b.Dai(n1, n2, m1, m2), b.Dab(n1, n2, m1, m2));
+}
+
+// Adjoint of one step with the rounding taken as the identity: the update
+// y_c' = y_c + sigma_c dt/6 (k1 + 2 k2 + 2 k3 + k4), sigma = -i (T), +i (Lambda),
+// gives ybar_c = ybar'_c + (adjoint of y_c through the 16 stage evaluations
+// seeded with kbar[c][s] = sigma_c dt/6 w_s ybar'_c), and the stage fields'
+// adjoints go to the runtime's gradient accumulators.
+void build_rk4_step_vjp(Graph& g, const StepVjpIO& io, int method, double dt) {
+  CCBuilder b(g, io.tei);
+  int k[4][4];
+  stage_derivatives(g, b, io.y, method, dt, k);
+  const double w[4] = {1.0, 2.0, 2.0, 1.0};
+  std::vector<std::pair<int, Term>> seeds;
+  for (int chain = 0; chain < 4; ++chain) {
+    const double sig = chain < 2 ? -1.0 : 1.0;
+    for (int s = 0; s < 4; ++s) seeds.push_back({k[chain][s], Term{io.ybar_in[chain], {0, 1, 2, 3}, 0.0, sig * dt / 6.0 * w[s]}});
+  }
+  const Graph::VjpResult r = g.vjp(seeds);
+  for (int chain = 0; chain < 4; ++chain) {
+    Terms t;
+    t.add(io.ybar_in[chain]);
+    auto it = r.ext.find(io.y[chain]);
+    if (it != r.ext.end()) t.add(it->second);
+    g.assign(io.ybar_out[chain], b.L(t));
+  }
+  for (const auto& fv : r.field_views) g.field_vjp(fv.second, fv.first);
+}
+
+// Adjoint of the weighted read-out w mu_pol . D(y) (CCSD_Helper.py:1066-1092)
+// added into ybar, and its direct term w D into the dipole-matrix gradient.
+void build_readout_vjp(Graph& g, const ReadoutVjpIO& io) {
+  CCBuilder b(g, io.tei);
+  const int t1 = io.y[0], t2 = io.y[1], l1 = io.y[2], l2 = io.y[3];
+  const int dij = b.Dij(t1, t2, l1, l2), dai = b.Dai(t1, t2, l1, l2), dab = b.Dab(t1, t2, l1, l2);
+  g.trace_vjp(dij, l1, dai, dab);
+  const int sij = g.seed(Cls::O, Cls::O), sov = g.seed(Cls::O, Cls::V), sai = g.seed(Cls::V, Cls::O),
+            sab = g.seed(Cls::V, Cls::V);
+  const Graph::VjpResult r = g.vjp({{dij, tm(sij)}, {dai, tm(sai)}, {dab, tm(sab)}});
+  for (int c = 0; c < 4; ++c) {
+    Terms t;
+    t.add(io.ybar[c]);
+    auto it = r.ext.find(io.y[c]);
+    if (it != r.ext.end()) t.add(it->second);
+    if (c == 2) t.add(sov);  // D[o][v] = lam1
+    g.assign(io.ybar[c], b.L(t));
+  }
 }
 
 }  // namespace tdcc

@@ -104,4 +104,20 @@ struct StepIO {
 // chains, fused finalize (rounding + update) per block, then the dipole trace.
 void build_rk4_step(Graph& g, const StepIO& io, int method, double dt);
 
+// Adjoint programs of the windowed-dipole gradient.
+struct StepVjpIO {
+  int tei;
+  int y[4];          // state before the step
+  int ybar_in[4];    // adjoint of the state after the step
+  int ybar_out[4];   // written: adjoint of the state before the step
+};
+void build_rk4_step_vjp(Graph& g, const StepVjpIO& io, int method, double dt);
+
+struct ReadoutVjpIO {
+  int tei;
+  int y[4];       // state whose dipole is read out
+  int ybar[4];    // adjoint accumulated in place
+};
+void build_readout_vjp(Graph& g, const ReadoutVjpIO& io);
+
 }  // namespace tdcc

@@ -75,7 +75,7 @@ struct GemmPath {
 };
 
 struct Graph::Launch {
-  enum Kind { GETT, LIN, FIELD, FIN, TRACE, LADDER } kind;
+  enum Kind { GETT, LIN, FIELD, FIN, TRACE, LADDER, FVJP, SEED, TVJP } kind;
   int node = -1;
   LadderArgs lad{};
   GemmPath gp{};
@@ -94,6 +94,9 @@ struct Graph::Launch {
   FieldArgs f{};
   FinalizeArgs fin{};
   TraceArgs tr{};
+  FieldVjpArgs fv{};
+  SeedArgs sd{};
+  TraceVjpArgs tv{};
 };
 
 Graph::Graph(int o, int v, const Runtime* rt) : o_(o), v_(v), rt_(rt) {}
@@ -427,6 +430,117 @@ void Graph::finalize_rk(int y_ext, int base_ext, const int k[4], bool lambda) {
   add(std::move(n));
 }
 
+// ---------------------------------------------------------------------------
+// Reverse mode.
+Graph::VjpResult Graph::vjp(const std::vector<std::pair<int, Term>>& seeds) {
+  const int top = (int)nodes_.size();
+  // nodes that depend on a non-constant input (state externals or fields)
+  std::vector<char> var(top, 0);
+  for (int i = 0; i < top; ++i) {
+    const Node& n = nodes_[i];
+    switch (n.op) {
+      case Op::External: var[i] = !is_static(i); break;
+      case Op::View: var[i] = var[n.base]; break;
+      case Op::Field: var[i] = 1; break;
+      case Op::Contract: var[i] = var[n.a] || var[n.b]; break;
+      case Op::Lincomb:
+        for (const Term& t : n.terms) var[i] = var[i] || var[t.src];
+        break;
+      case Op::Ladder: var[i] = var[n.a]; break;
+      default: var[i] = 0; break;
+    }
+  }
+  std::map<int, std::vector<Term>> acc;
+  for (const auto& s : seeds) {
+    if (s.first < 0 || s.first >= top) throw Error("vjp seed out of range");
+    if (var[s.first]) acc[s.first].push_back(s.second);
+  }
+  auto letters_of = [](const std::string& spec, std::string& la, std::string& lb, std::string& lc) {
+    const auto comma = spec.find(','), arrow = spec.find("->");
+    la = spec.substr(0, comma);
+    lb = spec.substr(comma + 1, arrow - comma - 1);
+    lc = spec.substr(arrow + 2);
+  };
+  VjpResult res;
+  for (int id = top - 1; id >= 0; --id) {
+    auto it = acc.find(id);
+    if (it == acc.end()) continue;
+    std::vector<Term> terms = std::move(it->second);
+    acc.erase(it);
+    const Node n = nodes_[id];  // copy: adding nodes may move the vector
+    const bool ident = terms.size() == 1 && terms[0].cr == 1.0 && terms[0].ci == 0.0 &&
+                       terms[0].perm[0] == 0 && terms[0].perm[1] == 1 && terms[0].perm[2] == 2 && terms[0].perm[3] == 3;
+    const int bar = ident ? terms[0].src : lincomb(terms);
+    switch (n.op) {
+      case Op::External:
+        res.ext[id] = bar;
+        break;
+      case Op::View:
+        if (nodes_[n.base].op == Op::Field) res.field_views.push_back({id, bar});
+        else throw Error("vjp: view of a non-constant, non-field tensor");
+        break;
+      case Op::Lincomb:
+        for (const Term& t : n.terms) {
+          if (!var[t.src]) continue;
+          Term b{bar, {0, 1, 2, 3}, t.cr, t.ci};
+          for (int d = 0; d < n.shape.rank; ++d) b.perm[t.perm[d]] = d;
+          acc[t.src].push_back(b);
+        }
+        break;
+      case Op::Contract: {
+        std::string la, lb, lc;
+        letters_of(n.spec, la, lb, lc);
+        if (var[n.a]) acc[n.a].push_back(tm(contract(lc + "," + lb + "->" + la, bar, n.b, n.ar, n.ai)));
+        if (var[n.b]) acc[n.b].push_back(tm(contract(la + "," + lc + "->" + lb, n.a, bar, n.ar, n.ai)));
+        break;
+      }
+      case Op::Ladder:
+        // out = sum_ef <ab||ef> u_ijef (ket 0) has the transpose
+        // ubar_ijef = sum_ab <ab||ef> outbar_ijab, the ket-1 ladder (and back)
+        acc[n.a].push_back(tm(ladder(bar, 1 - n.stage)));
+        break;
+      case Op::Field:
+        throw Error("vjp: a stage field used other than through its blocks");
+      default:
+        throw Error("vjp: node type without an adjoint");
+    }
+  }
+  return res;
+}
+
+void Graph::field_vjp(int adj, int view) {
+  const Node& v = nodes_[view];
+  if (v.op != Op::View || nodes_[v.base].op != Op::Field || v.shape.rank != 2)
+    throw Error("field_vjp needs a block view of a stage field");
+  Node n;
+  n.op = Op::FieldVjp;
+  n.a = adj;
+  n.b = view;
+  n.stage = nodes_[v.base].stage;
+  n.traj = nodes_[v.base].traj;
+  add(std::move(n));
+}
+
+int Graph::seed(Cls rows, Cls cols) {
+  Node n;
+  n.op = Op::Seed;
+  n.shape.rank = 2;
+  n.shape.cls[0] = rows;
+  n.shape.cls[1] = cols;
+  n.cplx = true;
+  return add(std::move(n));
+}
+
+void Graph::trace_vjp(int dij, int l1, int dai, int dab) {
+  Node n;
+  n.op = Op::TraceVjp;
+  n.k[0] = dij;
+  n.k[1] = l1;
+  n.k[2] = dai;
+  n.k[3] = dab;
+  add(std::move(n));
+}
+
 void Graph::trace(int dij, int l1, int dai, int dab) {
   Node n;
   n.op = Op::Trace;
@@ -449,7 +563,7 @@ void Graph::assign(int dst, int src) {
 
 // ---------------------------------------------------------------------------
 static bool owns_buffer(Op op) {
-  return op == Op::Contract || op == Op::Lincomb || op == Op::Field || op == Op::Ladder;
+  return op == Op::Contract || op == Op::Lincomb || op == Op::Field || op == Op::Ladder || op == Op::Seed;
 }
 
 static void inputs_of(const Node& n, std::vector<int>& in) {
@@ -464,6 +578,8 @@ static void inputs_of(const Node& n, std::vector<int>& in) {
     case Op::Trace: in = {n.k[0], n.k[1], n.k[2], n.k[3]}; break;
     case Op::Assign: in = {n.a, n.b}; break;
     case Op::Ladder: in = {n.a}; break;
+    case Op::FieldVjp: in = {n.a}; break;
+    case Op::TraceVjp: in = {n.k[0], n.k[1], n.k[2], n.k[3]}; break;
     default: break;
   }
 }
@@ -501,7 +617,8 @@ void Graph::plan() {
   std::vector<int> in;
   for (int i = N - 1; i >= 0; --i) {
     const Op op = nodes_[i].op;
-    if (op == Op::Finalize || op == Op::Trace || op == Op::Assign) live[i] = 1;
+    if (op == Op::Finalize || op == Op::Trace || op == Op::Assign || op == Op::FieldVjp || op == Op::TraceVjp)
+      live[i] = 1;
     if (!live[i]) continue;
     inputs_of(nodes_[i], in);
     for (int x : in) live[x] = 1;
@@ -1274,6 +1391,79 @@ void Graph::bind(double* arena) {
         launches_.push_back(std::move(L));
         break;
       }
+      case Op::FieldVjp: {
+        auto L = std::make_unique<Launch>();
+        L->kind = Launch::FVJP;
+        L->node = i;
+        if (!rt_->g || !rt_->ttab) throw Error("internal: field adjoint without gradient buffers");
+        if (n.traj >= 0) throw Error("adjoint of batched programs is not supported");
+        const Node& v = nodes_[n.b];
+        const Dev A = resolve(n.a);
+        FieldVjpArgs& f = L->fv;
+        f.ab_re = A.re;
+        f.ab_im = A.im;
+        f.rows = (int)extent(v.shape.cls[0]);
+        f.cols = (int)extent(v.shape.cls[1]);
+        f.p0 = v.shape.cls[0] == Cls::V ? o_ : 0;
+        f.q0 = v.shape.cls[1] == Cls::V ? o_ : 0;
+        f.n = o_ + v_;
+        f.mu_pol = rt_->mu + (int64_t)rt_->pol * n2;
+        f.negA = rt_->negA;
+        f.table = rt_->field_table;
+        f.ttab = rt_->ttab;
+        f.counter = rt_->counter;
+        f.stage = n.stage;
+        f.g = rt_->g;
+        f.gmu_re = rt_->gmu_re;
+        f.gmu_im = rt_->gmu_im;
+        if (!nodes_[n.a].cplx) throw Error("internal: real field adjoint");
+        launches_.push_back(std::move(L));
+        break;
+      }
+      case Op::Seed: {
+        auto L = std::make_unique<Launch>();
+        L->kind = Launch::SEED;
+        L->node = i;
+        if (!rt_->wtab) throw Error("internal: seed without a weight table");
+        const Dev O = resolve(i);
+        SeedArgs& d = L->sd;
+        d.out_re = O.re;
+        d.out_im = O.im;
+        d.rows = (int)extent(n.shape.cls[0]);
+        d.cols = (int)extent(n.shape.cls[1]);
+        d.p0 = n.shape.cls[0] == Cls::V ? o_ : 0;
+        d.q0 = n.shape.cls[1] == Cls::V ? o_ : 0;
+        d.n = o_ + v_;
+        d.mu_pol = rt_->mu + (int64_t)rt_->pol * n2;
+        d.wtab = rt_->wtab;
+        d.counter = rt_->counter;
+        launches_.push_back(std::move(L));
+        break;
+      }
+      case Op::TraceVjp: {
+        auto L = std::make_unique<Launch>();
+        L->kind = Launch::TVJP;
+        L->node = i;
+        if (!rt_->gmu_re || !rt_->wtab) throw Error("internal: read-out adjoint without gradient buffers");
+        const Dev d0 = resolve(n.k[0]), d1 = resolve(n.k[1]), d2 = resolve(n.k[2]), d3 = resolve(n.k[3]);
+        TraceVjpArgs& t = L->tv;
+        t.dij_re = d0.re;
+        t.dij_im = d0.im;
+        t.l1_re = d1.re;
+        t.l1_im = d1.im;
+        t.dai_re = d2.re;
+        t.dai_im = d2.im;
+        t.dab_re = d3.re;
+        t.dab_im = d3.im;
+        t.o = o_;
+        t.v = v_;
+        t.wtab = rt_->wtab;
+        t.counter = rt_->counter;
+        t.gmu_re = rt_->gmu_re;
+        t.gmu_im = rt_->gmu_im;
+        launches_.push_back(std::move(L));
+        break;
+      }
       default: break;
     }
   }
@@ -1339,6 +1529,9 @@ void Graph::run(hipStream_t s) const {
       case Launch::FIELD: launch_field(L->f, s); break;
       case Launch::FIN: launch_finalize(L->fin, s); break;
       case Launch::TRACE: launch_trace(L->tr, s); break;
+      case Launch::FVJP: launch_field_vjp(L->fv, s); break;
+      case Launch::SEED: launch_seed(L->sd, s); break;
+      case Launch::TVJP: launch_trace_vjp(L->tv, s); break;
     }
   }
   TDCC_HIP(hipGetLastError());
@@ -1480,6 +1673,15 @@ void Graph::run_profiled(hipStream_t s, std::vector<LaunchRecord>& out) const {
           break;
         case Launch::TRACE:
           timed("trace_kernel", 0.0, 0.0, [&] { launch_trace(L->tr, s); });
+          break;
+        case Launch::FVJP:
+          timed("field_vjp_kernel", 0.0, 0.0, [&] { launch_field_vjp(L->fv, s); });
+          break;
+        case Launch::SEED:
+          timed("seed_kernel", 0.0, 0.0, [&] { launch_seed(L->sd, s); });
+          break;
+        case Launch::TVJP:
+          timed("trace_vjp_kernel", 0.0, 0.0, [&] { launch_trace_vjp(L->tv, s); });
           break;
       }
     }

@@ -37,7 +37,7 @@ struct Dev {  // device data of a tensor: planes + element strides
   int64_t st[4] = {0, 0, 0, 0};
 };
 
-enum class Op { External, View, Contract, Lincomb, Field, Finalize, Trace, Assign, Ladder };
+enum class Op { External, View, Contract, Lincomb, Field, Finalize, Trace, Assign, Ladder, FieldVjp, Seed, TraceVjp };
 
 struct Term {
   int src;
@@ -110,6 +110,13 @@ struct Runtime {
     }
   };
   std::vector<Traj> batch;
+  // adjoint programs: per-step loss weights [steps], stage times [steps][3],
+  // gradient accumulators g[4] = dL/dA, dL/dw0 (complex) and dL/dmu_pol (n,n)
+  const double* wtab = nullptr;
+  const double* ttab = nullptr;
+  double* g = nullptr;
+  double* gmu_re = nullptr;
+  double* gmu_im = nullptr;
   // packed antisymmetric <ab||ef>: vpack[0][q(ab)][q(ef)] = <ab||ef>,
   // vpack[1][q(ab)][q(ef)] = <ef||ab> (P = v(v-1)/2 pairs a < b)
   const double* vpack[2] = {nullptr, nullptr};
@@ -162,6 +169,25 @@ class Graph {
     ~Fp64Scope() { --g.fp64_depth_; }
   };
   void finalize_rk(int y_ext, int base_ext, const int k[4], bool lambda);
+
+  // ---- reverse mode ----
+  // Adjoints of the nodes built so far (holomorphic maps: transposed
+  // contractions and permutations, complex coefficients as they are).
+  // seeds: (node, term over an adjoint node) pairs.  Returns the summed
+  // adjoint of every non-constant External reached, and the adjoints of the
+  // views of Field nodes (the field's blocks).  Rounding does not appear in
+  // the graphs differentiated here.
+  struct VjpResult {
+    std::map<int, int> ext;                        // external id -> adjoint node
+    std::vector<std::pair<int, int>> field_views;  // (view of a Field node, adjoint node)
+  };
+  VjpResult vjp(const std::vector<std::pair<int, Term>>& seeds);
+  // Field adjoint of one view of a stage field into the runtime's gradients.
+  void field_vjp(int adj, int view);
+  // w mu_pol restricted to a (row class, col class) block, w from the
+  // runtime's weight table.
+  int seed(Cls rows, Cls cols);
+  void trace_vjp(int dij, int l1, int dai, int dab);
   void trace(int dij, int l1, int dai, int dab);
   void assign(int dst_ext, int src);
 

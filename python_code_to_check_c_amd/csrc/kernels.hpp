@@ -269,6 +269,52 @@ struct SkinnyArgs {
 };
 void launch_skinny(const SkinnyArgs& a, hipStream_t s);
 
+// ---------------------------------------------------------------------------
+// Adjoint kernels for d(sum_n w_n mu_pol(n))/d(A, w0, mu_pol).  The step
+// counter selects the row of the per-step tables as in field_kernel.
+//
+// field_vjp: Fbar, the adjoint of one (rows x cols) block at (p0, q0) of the
+// stage field F_s = F + (-A) mu_pol exp(i theta_s), theta_s = 2 pi w0 t_s:
+//   g[0..1] += sum Fbar (-mu e^{i theta})                 (d/dA)
+//   g[2..3] += sum Fbar (-A) mu e^{i theta} i 2 pi t_s     (d/dw0)
+//   gmu[p][q] += Fbar (-A) e^{i theta}                     (d/dmu_pol)
+struct FieldVjpArgs {
+  const double* ab_re; const double* ab_im;  // contiguous (rows, cols) block adjoint
+  int rows, cols, p0, q0, n;
+  const double* mu_pol;
+  double negA;
+  const double* table;   // [steps][3][2] cos, sin
+  const double* ttab;    // [steps][3] stage times
+  const int64_t* counter;
+  int stage;
+  double* g;             // [4] (+=)
+  double* gmu_re; double* gmu_im;  // (n, n) (+=)
+};
+void launch_field_vjp(const FieldVjpArgs& a, hipStream_t s);
+
+// seed: out = w mu_pol[p0.., q0..] (a real block stored complex), w = wtab[counter - 1].
+struct SeedArgs {
+  double* out_re; double* out_im;
+  int rows, cols, p0, q0, n;
+  const double* mu_pol;
+  const double* wtab;
+  const int64_t* counter;
+};
+void launch_seed(const SeedArgs& a, hipStream_t s);
+
+// trace_vjp: gmu[p][q] += w D[p][q] over the density blocks [[Dij, lam1], [Dai, Dab]].
+struct TraceVjpArgs {
+  const double* dij_re; const double* dij_im;
+  const double* l1_re;  const double* l1_im;
+  const double* dai_re; const double* dai_im;
+  const double* dab_re; const double* dab_im;
+  int o, v;
+  const double* wtab;
+  const int64_t* counter;
+  double* gmu_re; double* gmu_im;
+};
+void launch_trace_vjp(const TraceVjpArgs& a, hipStream_t s);
+
 // Interleaved complex <-> two planes.
 void launch_split_complex(const double* inter, double* re, double* im, int64_t n, hipStream_t s);
 void launch_join_complex(const double* re, const double* im, double* inter, int64_t n, hipStream_t s);
