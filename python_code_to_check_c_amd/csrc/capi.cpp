@@ -69,10 +69,14 @@ struct tdcc_ctx {
   unsigned long long* counts = nullptr;    // kCountBlocks + 1 partial counts
   Runtime rt;
   std::map<std::string, Program> progs;
+  // t2 and lam2 of the state (of the checkpoint) are antisymmetric in
+  // (i, j): the ladders then run on the i < j rows only
+  bool pairs = true, backup_pairs = true;
   // batched trajectories of this system (tdcc_set_batch): per-slot state and
   // rounded base, per-slot field tables and dipole rows ([slot][cap][3][2])
   int nb = 0;
   std::vector<std::array<Planes, 4>> bstate, bbase;
+  std::vector<char> slot_pairs;
   double* btable = nullptr;
   double* bmu = nullptr;
   int64_t bcap = 0;
@@ -111,6 +115,28 @@ void ensure_staging(tdcc_ctx* c, int64_t n) {
   c->staging_n = 0;
   TDCC_HIP(hipMalloc(&c->staging, 2 * n * sizeof(double)));
   c->staging_n = n;
+}
+
+// True when x (interleaved complex, (o,o,v,v)) is antisymmetric in its first
+// two indices to 1e-12 of its largest entry, as CC amplitudes are: the
+// ladders may then compute the i < j rows only and mirror them, which agrees
+// with the full rows to that tolerance.  TDCC_IJ_PAIRS=0 turns this off.
+bool antisym_ij(const double* x, int o, int v) {
+  const char* e = std::getenv("TDCC_IJ_PAIRS");
+  if (e && e[0] == '0') return false;
+  if (!x) return true;
+  const int64_t len = 2 * (int64_t)v * v;
+  double mx = 0.0, dev = 0.0;
+  for (int i = 0; i < o; ++i)
+    for (int j = 0; j <= i; ++j) {
+      const double* a = x + ((int64_t)i * o + j) * len;
+      const double* b = x + ((int64_t)j * o + i) * len;
+      for (int64_t k = 0; k < len; ++k) {
+        mx = std::max(mx, std::max(std::fabs(a[k]), std::fabs(b[k])));
+        dev = std::max(dev, std::fabs(a[k] + b[k]));
+      }
+    }
+  return dev <= 1e-12 * mx;
 }
 
 void upload_complex(tdcc_ctx* c, const double* host, const Planes& p) {
@@ -231,11 +257,12 @@ int build_rhs_node(CCBuilder& b, int kind, int F, int t1, int t2, int l1, int l2
   }
 }
 
-Program& rhs_program(tdcc_ctx* c, int kind) {
-  const std::string key = "rhs:" + std::to_string(kind);
+Program& rhs_program(tdcc_ctx* c, int kind, bool pairs) {
+  const std::string key = "rhs:" + std::to_string(kind) + (pairs ? ":p" : ":f");
   auto it = c->progs.find(key);
   if (it != c->progs.end()) return it->second;
   auto g = std::make_unique<Graph>(c->o, c->v, &c->rt);
+  g->set_ij_pairs(pairs);
   const int tei = ext_tei(c, *g);
   Shape sF;
   sF.rank = 2;
@@ -345,10 +372,11 @@ std::string hexd(double x) {
 }
 
 Program& step_program(tdcc_ctx* c, int method, double dt) {
-  const std::string key = "step:" + std::to_string(method) + ":" + hexd(dt);
+  const std::string key = "step:" + std::to_string(method) + ":" + hexd(dt) + (c->pairs ? ":p" : ":f");
   auto it = c->progs.find(key);
   if (it != c->progs.end()) return it->second;
   auto g = std::make_unique<Graph>(c->o, c->v, &c->rt);
+  g->set_ij_pairs(c->pairs);
   StepIO io;
   io.tei = ext_tei(c, *g);
   for (int k = 0; k < 4; ++k) {
@@ -363,10 +391,14 @@ Program& step_program(tdcc_ctx* c, int method, double dt) {
 // One program advancing every batch slot by one step: the slots' step
 // graphs side by side (integral-only nodes shared), one device counter.
 Program& batch_step_program(tdcc_ctx* c, int method, double dt) {
-  const std::string key = "bstep:" + std::to_string(method) + ":" + hexd(dt) + ":" + std::to_string(c->nb);
+  bool pairs = true;
+  for (char f : c->slot_pairs) pairs = pairs && f;
+  const std::string key =
+      "bstep:" + std::to_string(method) + ":" + hexd(dt) + ":" + std::to_string(c->nb) + (pairs ? ":p" : ":f");
   auto it = c->progs.find(key);
   if (it != c->progs.end()) return it->second;
   auto g = std::make_unique<Graph>(c->o, c->v, &c->rt);
+  g->set_ij_pairs(pairs);
   const int tei = ext_tei(c, *g);
   for (int b = 0; b < c->nb; ++b) {
     g->set_traj(b);
@@ -388,10 +420,11 @@ Program& batch_step_program(tdcc_ctx* c, int method, double dt) {
 // Adjoint programs of the windowed-dipole gradient (tdcc_dipole_gradient):
 // their externals are the context's adjoint buffers.
 Program& vjp_step_program(tdcc_ctx* c, int method, double dt) {
-  const std::string key = "vstep:" + std::to_string(method) + ":" + hexd(dt);
+  const std::string key = "vstep:" + std::to_string(method) + ":" + hexd(dt) + (c->pairs ? ":p" : ":f");
   auto it = c->progs.find(key);
   if (it != c->progs.end()) return it->second;
   auto g = std::make_unique<Graph>(c->o, c->v, &c->rt);
+  g->set_ij_pairs(c->pairs);  // forward ladders only; their adjoints run on full rows
   StepVjpIO io;
   io.tei = ext_tei(c, *g);
   for (int k = 0; k < 4; ++k) {
@@ -622,6 +655,7 @@ void free_batch(tdcc_ctx* c) {
   c->btable = c->bmu = nullptr;
   c->bcap = 0;
   c->nb = 0;
+  c->slot_pairs.clear();
   c->rt.batch.clear();
 }
 
@@ -685,6 +719,7 @@ int tdcc_create(int nocc, int nvir, int device, tdcc_ctx** out) {
     c->rt.F = c->F;
     c->rt.mu = c->mu;
     c->rt.counter = c->counter;
+    c->pairs = c->backup_pairs = antisym_ij(nullptr, c->o, c->v);  // zero amplitudes
     TDCC_HIP(hipStreamSynchronize(c->stream));
   });
   if (rc) {
@@ -782,6 +817,7 @@ int tdcc_set_state(tdcc_ctx* c, const double* t1, const double* t2, const double
       if (!h[k]) throw Error("all four amplitude arrays are required");
       upload_complex(c, h[k], c->state[k]);
     }
+    c->pairs = antisym_ij(t2, c->o, c->v) && antisym_ij(l2, c->o, c->v);
   });
 }
 
@@ -815,6 +851,7 @@ int tdcc_set_batch(tdcc_ctx* c, int nb) {
         alloc_planes(c->bbase[b][k], block_size(c, k));
       }
     c->nb = nb;
+    c->slot_pairs.assign(nb, antisym_ij(nullptr, c->o, c->v));  // zero amplitudes
     c->rt.batch.assign(nb, Runtime::Traj{});
   });
 }
@@ -828,6 +865,7 @@ int tdcc_set_state_slot(tdcc_ctx* c, int slot, const double* t1, const double* t
       if (!h[k]) throw Error("all four amplitude arrays are required");
       upload_complex(c, h[k], c->bstate[slot][k]);
     }
+    c->slot_pairs[slot] = antisym_ij(t2, c->o, c->v) && antisym_ij(l2, c->o, c->v);
   });
 }
 
@@ -1030,7 +1068,8 @@ int tdcc_rhs(tdcc_ctx* c, int kind, const double* F, const double* t1, const dou
     if (!F || !t1 || !t2 || !out) throw Error("F, t1, t2 and out are required");
     const bool lam = kind == TDCC_RHS_L1 || kind == TDCC_RHS_L2 || kind == TDCC_RHS_L1_CC2 || kind == TDCC_RHS_L2_CC2;
     if (lam && (!l1 || !l2)) throw Error("lam1 and lam2 are required for a Lambda right-hand side");
-    Program& p = rhs_program(c, kind);
+    const bool pairs = antisym_ij(t2, c->o, c->v) && (!lam || antisym_ij(l2, c->o, c->v));
+    Program& p = rhs_program(c, kind, pairs);
     upload_complex(c, F, c->io_F);
     const double* h[4] = {t1, t2, l1, l2};
     for (int k = 0; k < 4; ++k)
@@ -1127,6 +1166,8 @@ int tdcc_checkpoint(tdcc_ctx* c, int op) {
       const Planes& dst = op == 0 ? c->backup[k] : c->state[k];
       TDCC_HIP(hipMemcpyAsync(dst.d, src.d, 2 * src.n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     }
+    if (op == 0) c->backup_pairs = c->pairs;
+    else c->pairs = c->backup_pairs;
     TDCC_HIP(hipStreamSynchronize(c->stream));
   });
 }

@@ -40,6 +40,8 @@ struct LadderArgs {
   double* up = nullptr;   // (2 rows) x P packed input
   double* rp = nullptr;   // (2 rows) x P packed result
   const double* vp = nullptr;
+  int o = 0;
+  bool ij_pairs = false;  // rows are the pairs i < j
   int64_t rows = 0;
   int v = 0;
   int64_t P = 0;
@@ -415,7 +417,8 @@ int Graph::ladder(int u, int ket) {
   n.cplx = true;
   n.a = u;
   n.stage = ket;
-  n.key = "LAD|" + std::to_string(u) + "|" + std::to_string(ket);
+  n.ij_pairs = ij_pairs_;
+  n.key = "LAD|" + std::to_string(u) + "|" + std::to_string(ket) + "|" + (ij_pairs_ ? "p" : "f");
   return add(std::move(n));
 }
 
@@ -497,7 +500,13 @@ Graph::VjpResult Graph::vjp(const std::vector<std::pair<int, Term>>& seeds) {
       case Op::Ladder:
         // out = sum_ef <ab||ef> u_ijef (ket 0) has the transpose
         // ubar_ijef = sum_ab <ab||ef> outbar_ijab, the ket-1 ladder (and back)
-        acc[n.a].push_back(tm(ladder(bar, 1 - n.stage)));
+        {
+          // the adjoint need not be antisymmetric in (i, j): full rows
+          const bool keep = ij_pairs_;
+          ij_pairs_ = false;
+          acc[n.a].push_back(tm(ladder(bar, 1 - n.stage)));
+          ij_pairs_ = keep;
+        }
         break;
       case Op::Field:
         throw Error("vjp: a stage field used other than through its blocks");
@@ -584,21 +593,27 @@ static void inputs_of(const Node& n, std::vector<int>& in) {
   }
 }
 
-// Ladder scratch: packed input up and result rp (2 o^2 x P doubles each),
-// then, in split precision, the digit planes of up and its row scales.
+// Ladder scratch: packed input up and result rp (2 rows x P doubles each;
+// rows = o^2, or o (o - 1) / 2 with ij pairs), then, in split precision, the
+// digit planes of up, its row scales and the split workspace.
 static int64_t al256(int64_t x) { return (x + 255) / 256 * 256; }
-static int64_t ladder_rows2(int o) { return 2 * (int64_t)o * o; }
+static int64_t ladder_rows(int o, bool pr) { return pr ? (int64_t)o * (o - 1) / 2 : (int64_t)o * o; }
+static int64_t ladder_rows2(int o, bool pr) { return 2 * ladder_rows(o, pr); }
 static int64_t ladder_P(int v) { return (int64_t)v * (v - 1) / 2; }
 static int64_t ladder_Kp(int v) { return (ladder_P(v) + 63) / 64 * 64; }
-static int64_t ladder_plane(int o, int v) { return al256(ladder_rows2(o) * ladder_Kp(v)); }
-static int64_t ladder_digits_off(int o, int v) { return al256(2 * ladder_rows2(o) * ladder_P(v) * (int64_t)sizeof(double)); }
-static int64_t ladder_scale_off(int o, int v, int S) { return ladder_digits_off(o, v) + S * ladder_plane(o, v); }
-static int64_t ladder_pmax_off(int o, int v, int S) {
-  return ladder_scale_off(o, v, S) + al256(ladder_rows2(o) * (int64_t)sizeof(double));
+static int64_t ladder_plane(int o, bool pr, int v) { return al256(ladder_rows2(o, pr) * ladder_Kp(v)); }
+static int64_t ladder_digits_off(int o, bool pr, int v) {
+  return al256(2 * ladder_rows2(o, pr) * ladder_P(v) * (int64_t)sizeof(double));
 }
-static int64_t ladder_scratch_bytes(int o, int v, int S) {
-  if (S <= 0) return ladder_digits_off(o, v);
-  return ladder_pmax_off(o, v, S) + al256(ozaki_split_workspace(ladder_rows2(o), ladder_Kp(v), false));
+static int64_t ladder_scale_off(int o, bool pr, int v, int S) {
+  return ladder_digits_off(o, pr, v) + S * ladder_plane(o, pr, v);
+}
+static int64_t ladder_pmax_off(int o, bool pr, int v, int S) {
+  return ladder_scale_off(o, pr, v, S) + al256(ladder_rows2(o, pr) * (int64_t)sizeof(double));
+}
+static int64_t ladder_scratch_bytes(int o, bool pr, int v, int S) {
+  if (S <= 0) return ladder_digits_off(o, pr, v);
+  return ladder_pmax_off(o, pr, v, S) + al256(ozaki_split_workspace(ladder_rows2(o, pr), ladder_Kp(v), false));
 }
 
 bool Graph::is_static(int id) const {
@@ -681,7 +696,7 @@ void Graph::plan() {
     if (!live[i]) continue;
     Node& n = nodes_[i];
     if (owns_buffer(n.op)) n.arena_off = alloc(size(n.shape) * (n.cplx ? 2 : 1) * (int64_t)sizeof(double));
-    if (n.op == Op::Ladder) n.scratch_len = ladder_scratch_bytes(o_, v_, rt_->split_digits);
+    if (n.op == Op::Ladder) n.scratch_len = ladder_scratch_bytes(o_, n.ij_pairs, v_, rt_->split_digits);
     if (n.op == Op::Contract || n.op == Op::Ladder) {
       if (n.op == Op::Contract) lower_contract(i);
       if (n.scratch_len > 0) {
@@ -1283,7 +1298,9 @@ void Graph::bind(double* arena) {
         a.u_im = U.im;
         a.out_re = O.re;
         a.out_im = O.im;
-        a.rows = (int64_t)o_ * o_;
+        a.o = o_;
+        a.ij_pairs = n.ij_pairs;
+        a.rows = ladder_rows(o_, n.ij_pairs);
         a.v = v_;
         a.P = (int64_t)v_ * (v_ - 1) / 2;
         a.up = arena_ + n.scratch_off / (int64_t)sizeof(double);
@@ -1294,7 +1311,7 @@ void Graph::bind(double* arena) {
         if (a.S > 0) {
           if (!rt_->vpack8[n.stage]) throw Error("digit planes of the packed <ab||ef> not built");
           char* scr = reinterpret_cast<char*>(arena_) + n.scratch_off;
-          const int64_t Kp = ladder_Kp(v_), plane = ladder_plane(o_, v_);
+          const int64_t Kp = ladder_Kp(v_), plane = ladder_plane(o_, a.ij_pairs, v_);
           if (Kp != rt_->vpack_Kp) throw Error("internal: ladder digit row length");
           SplitArgs& su = a.su;  // rows of up, dense
           su.src_re = a.up;
@@ -1304,10 +1321,10 @@ void Graph::bind(double* arena) {
           su.K = a.P;
           su.Kp = Kp;
           su.S = a.S;
-          su.D = reinterpret_cast<int8_t*>(scr + ladder_digits_off(o_, v_));
+          su.D = reinterpret_cast<int8_t*>(scr + ladder_digits_off(o_, a.ij_pairs, v_));
           su.plane = plane;
-          su.scale = reinterpret_cast<double*>(scr + ladder_scale_off(o_, v_, a.S));
-          su.pmax = reinterpret_cast<double*>(scr + ladder_pmax_off(o_, v_, a.S));
+          su.scale = reinterpret_cast<double*>(scr + ladder_scale_off(o_, a.ij_pairs, v_, a.S));
+          su.pmax = reinterpret_cast<double*>(scr + ladder_pmax_off(o_, a.ij_pairs, v_, a.S));
           su.multipass = split_multipass();
           OzGemmArgs& oz = a.oz;
           oz.A = su.D;
@@ -1488,14 +1505,14 @@ static DgemmArgs ladder_gemm(const LadderArgs& a) {
 }
 
 static void run_ladder(const LadderArgs& a, hipStream_t s) {
-  launch_pack_pairs(a.u_re, a.u_im, a.up, a.rows, a.v, s);
+  launch_pack_pairs(a.u_re, a.u_im, a.up, a.o, a.v, a.ij_pairs, s);
   if (a.S > 0) {
     launch_ozaki_split(a.su, false, s);
     launch_ozgemm(a.oz, s);
   } else {
     launch_dgemm(ladder_gemm(a), true, s);
   }
-  launch_unpack_pairs(a.rp, a.out_re, a.out_im, a.rows, a.v, 1.0, s);
+  launch_unpack_pairs(a.rp, a.out_re, a.out_im, a.o, a.v, 1.0, a.ij_pairs, s);
 }
 
 void Graph::run(hipStream_t s) const {
@@ -1645,9 +1662,9 @@ void Graph::run_profiled(hipStream_t s, std::vector<LaunchRecord>& out) const {
         }
         case Launch::LADDER: {
           const LadderArgs& a = L->lad;
-          const double elems = 2.0 * a.rows * a.v * a.v;
-          timed("pack_pairs_kernel", 0.0, 8.0 * (elems + 2.0 * a.rows * a.P),
-                [&] { launch_pack_pairs(a.u_re, a.u_im, a.up, a.rows, a.v, s); });
+          const double elems = 2.0 * a.o * a.o * a.v * a.v, packed = 2.0 * a.rows * a.P;
+          timed("pack_pairs_kernel", 0.0, 8.0 * (2.0 * a.rows * a.v * a.v + packed),
+                [&] { launch_pack_pairs(a.u_re, a.u_im, a.up, a.o, a.v, a.ij_pairs, s); });
           const DgemmArgs g = ladder_gemm(a);
           char dt[96];
           if (a.S > 0) {
@@ -1661,8 +1678,8 @@ void Graph::run_profiled(hipStream_t s, std::vector<LaunchRecord>& out) const {
             detail = dt;
             timed("dgemm_kernel<BT>", 2.0 * g.M * (double)g.N * g.K, 0.0, [&] { launch_dgemm(g, true, s); });
           }
-          timed("unpack_pairs_kernel", 0.0, 8.0 * (elems + 2.0 * a.rows * a.P),
-                [&] { launch_unpack_pairs(a.rp, a.out_re, a.out_im, a.rows, a.v, 1.0, s); });
+          timed("unpack_pairs_kernel", 0.0, 8.0 * (elems + packed),
+                [&] { launch_unpack_pairs(a.rp, a.out_re, a.out_im, a.o, a.v, 1.0, a.ij_pairs, s); });
           break;
         }
         case Launch::FIELD:
@@ -1705,7 +1722,8 @@ int64_t Graph::contract_flops() const {
   }
   const int64_t P = (int64_t)v_ * (v_ - 1) / 2;
   for (size_t i = 0; i < nodes_.size(); ++i)
-    if (nodes_[i].op == Op::Ladder && i < live_.size() && live_[i]) f += 2 * (2 * (int64_t)o_ * o_) * P * P;
+    if (nodes_[i].op == Op::Ladder && i < live_.size() && live_[i])
+      f += 2 * ladder_rows2(o_, nodes_[i].ij_pairs) * P * P;
   return f;
 }
 

@@ -79,6 +79,8 @@ struct Node {
   int64_t scratch_len = 0;
   // stays FP64 when the runtime runs the large contractions in split precision
   bool fp64 = false;
+  // Ladder: only the i < j rows are computed (input antisymmetric in (i, j))
+  bool ij_pairs = false;
 };
 
 struct Handle {
@@ -160,6 +162,10 @@ class Graph {
   //   ket == 1: out_ijab = sum_ef <ef||ab> u_ijef   ('efab,ijef->ijab')
   // computed over pairs e < f and a < b (exact for antisymmetrized integrals).
   int ladder(int u, int ket);
+  // Ladders created from now on compute only the i < j rows and mirror them
+  // (valid when every ladder input is antisymmetric in its first two indices,
+  // as the amplitudes of the propagation are).
+  void set_ij_pairs(bool on) { ij_pairs_ = on; }
   int ones_view(const Shape& s);                     // all-ones tensor (stride 0)
   // While an Fp64Scope is open, every node created or shared is pinned to
   // FP64 arithmetic regardless of the runtime's split precision.
@@ -231,6 +237,7 @@ class Graph {
   int ones_id_ = -1;
   int fp64_depth_ = 0;
   int traj_ = -1;
+  bool ij_pairs_ = false;
   // digit planes of static operands (split precision), filled at bind
   bool is_static(int id) const;
   struct DigitCache {

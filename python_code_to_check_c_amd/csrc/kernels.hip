@@ -668,44 +668,81 @@ __device__ __forceinline__ int64_t pair_index(int a, int b, int v) {
   return (int64_t)a * v - (int64_t)a * (a + 1) / 2 + (b - a - 1);
 }
 
-__global__ void pack_pairs_kernel(const double* xr, const double* xi, double* X, int64_t rows, int v) {
-  const int64_t P = (int64_t)v * (v - 1) / 2, vv = (int64_t)v * v, total = 2 * rows * vv;
+// Rows of the packed ladder operands: all (i, j) (r = i o + j), or with
+// ij_pairs only i < j (r = pair index of (i, j) among o), the (i, j)
+// antisymmetry of the amplitudes making the other half redundant.
+__device__ __forceinline__ bool ladder_row(int i, int j, int o, bool ij_pairs, int64_t& r) {
+  if (!ij_pairs) {
+    r = (int64_t)i * o + j;
+    return true;
+  }
+  if (i >= j) return false;
+  r = pair_index(i, j, o);
+  return true;
+}
+
+__global__ void pack_pairs_kernel(const double* xr, const double* xi, double* X, int o, int v, int ij_pairs) {
+  const int64_t P = (int64_t)v * (v - 1) / 2, vv = (int64_t)v * v, oo = (int64_t)o * o;
+  const int64_t rows = ij_pairs ? (int64_t)o * (o - 1) / 2 : oo, total = 2 * oo * vv;
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
        idx += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t pr = idx / vv;  // plane * rows + r
-    const int64_t ef = idx - pr * vv;
+    const int64_t pij = idx / vv;  // plane * o^2 + (i o + j)
+    const int64_t ef = idx - pij * vv;
     const int e = (int)(ef / v), f = (int)(ef - (int64_t)e * v);
     if (e >= f) continue;
-    const int64_t r = pr % rows;
-    const double* x = (pr < rows) ? xr : xi;
-    const double val = __dsub_rn(x[r * vv + (int64_t)e * v + f], x[r * vv + (int64_t)f * v + e]);
-    X[pr * P + pair_index(e, f, v)] = val;
+    const int64_t pl = pij / oo, ij = pij - pl * oo;
+    const int i = (int)(ij / o), j = (int)(ij - (int64_t)i * o);
+    int64_t r;
+    if (!ladder_row(i, j, o, ij_pairs, r)) continue;
+    const double* x = pl ? xi : xr;
+    const double val = __dsub_rn(x[ij * vv + (int64_t)e * v + f], x[ij * vv + (int64_t)f * v + e]);
+    X[(pl * rows + r) * P + pair_index(e, f, v)] = val;
   }
 }
 
-__global__ void unpack_pairs_kernel(const double* Y, double* yr, double* yi, int64_t rows, int v, double alpha) {
-  const int64_t P = (int64_t)v * (v - 1) / 2, vv = (int64_t)v * v, total = 2 * rows * vv;
+__global__ void unpack_pairs_kernel(const double* Y, double* yr, double* yi, int o, int v, double alpha, int ij_pairs) {
+  const int64_t P = (int64_t)v * (v - 1) / 2, vv = (int64_t)v * v, oo = (int64_t)o * o;
+  const int64_t rows = ij_pairs ? (int64_t)o * (o - 1) / 2 : oo, total = 2 * oo * vv;
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
        idx += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t pr = idx / vv;
-    const int64_t ab = idx - pr * vv;
+    const int64_t pij = idx / vv;
+    const int64_t ab = idx - pij * vv;
     const int a = (int)(ab / v), b = (int)(ab - (int64_t)a * v);
+    const int64_t pl = pij / oo, ij = pij - pl * oo;
+    const int i = (int)(ij / o), j = (int)(ij - (int64_t)i * o);
+    double sign = alpha;
+    int64_t r = 0;
+    bool live = true;
+    if (ij_pairs) {
+      if (i == j) live = false;
+      else if (i < j) r = pair_index(i, j, o);
+      else {
+        r = pair_index(j, i, o);
+        sign = -sign;
+      }
+    } else {
+      r = ij;
+    }
     double val = 0.0;
-    if (a < b) val = __dmul_rn(alpha, Y[pr * P + pair_index(a, b, v)]);
-    else if (a > b) val = -__dmul_rn(alpha, Y[pr * P + pair_index(b, a, v)]);
-    const int64_t r = pr % rows;
-    double* y = (pr < rows) ? yr : yi;
-    y[r * vv + ab] = val;
+    if (live && a != b) {
+      const double* yrow = Y + (pl * rows + r) * P;
+      val = a < b ? __dmul_rn(sign, yrow[pair_index(a, b, v)]) : -__dmul_rn(sign, yrow[pair_index(b, a, v)]);
+    }
+    double* y = pl ? yi : yr;
+    y[ij * vv + ab] = val;
   }
 }
 
-void launch_pack_pairs(const double* xr, const double* xi, double* X, int64_t rows, int v, hipStream_t s) {
-  hipLaunchKernelGGL(pack_pairs_kernel, dim3(grid_for(2 * rows * v * v)), dim3(256), 0, s, xr, xi, X, rows, v);
+void launch_pack_pairs(const double* xr, const double* xi, double* X, int o, int v, bool ij_pairs, hipStream_t s) {
+  const int64_t total = 2 * (int64_t)o * o * v * v;
+  hipLaunchKernelGGL(pack_pairs_kernel, dim3(grid_for(total)), dim3(256), 0, s, xr, xi, X, o, v, (int)ij_pairs);
 }
 
-void launch_unpack_pairs(const double* Y, double* yr, double* yi, int64_t rows, int v, double alpha, hipStream_t s) {
-  hipLaunchKernelGGL(unpack_pairs_kernel, dim3(grid_for(2 * rows * v * v)), dim3(256), 0, s, Y, yr, yi, rows, v,
-                     alpha);
+void launch_unpack_pairs(const double* Y, double* yr, double* yi, int o, int v, double alpha, bool ij_pairs,
+                         hipStream_t s) {
+  const int64_t total = 2 * (int64_t)o * o * v * v;
+  hipLaunchKernelGGL(unpack_pairs_kernel, dim3(grid_for(total)), dim3(256), 0, s, Y, yr, yi, o, v, alpha,
+                     (int)ij_pairs);
 }
 
 __global__ void build_vpack_kernel(const double* tei, int o, int v, int ket, double* Vp) {

@@ -140,14 +140,17 @@ struct DgemmArgs {
 void launch_dgemm(const DgemmArgs& g, bool b_trans, hipStream_t s);
 
 // Antisymmetric pair packing of the virtual-virtual index pair (e < f,
-// lexicographic; P = v (v - 1) / 2):
-//   pack:   X[p * rows + r][q] = x[p][r][e_q][f_q] - x[p][r][f_q][e_q]
-//           for planes p of a complex (rows, v, v) tensor, written as a
-//           (2 rows) x P real matrix;
-//   unpack: y[p][r][a][b] = alpha * Y[p * rows + r][q(a,b)] for a < b,
+// lexicographic; P = v (v - 1) / 2) of a complex (o, o, v, v) tensor:
+//   pack:   X[p * rows + r][q] = x[p][i][j][e_q][f_q] - x[p][i][j][f_q][e_q]
+//           for planes p, written as a (2 rows) x P real matrix;
+//   unpack: y[p][i][j][a][b] = alpha * Y[p * rows + r][q(a,b)] for a < b,
 //           -alpha * Y[...][q(b,a)] for a > b, 0 for a == b.
-void launch_pack_pairs(const double* x_re, const double* x_im, double* X, int64_t rows, int v, hipStream_t s);
-void launch_unpack_pairs(const double* Y, double* y_re, double* y_im, int64_t rows, int v, double alpha,
+// Rows r = i o + j (rows = o^2), or with ij_pairs only i < j, r = q(i, j)
+// (rows = o (o - 1) / 2), and unpack writes y[j][i] = -y[i][j], y[i][i] = 0:
+// for tensors antisymmetric in (i, j) the other half is redundant.
+void launch_pack_pairs(const double* x_re, const double* x_im, double* X, int o, int v, bool ij_pairs,
+                       hipStream_t s);
+void launch_unpack_pairs(const double* Y, double* y_re, double* y_im, int o, int v, double alpha, bool ij_pairs,
                          hipStream_t s);
 // Packed <ab||ef> of the virtual block of a (n,n,n,n) TEI with o occupied:
 //   ladder_ket == 0: Vp[q(a,b)][q(e,f)] = <ab||ef>

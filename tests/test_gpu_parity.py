@@ -27,11 +27,13 @@ RHS = [("T1eq_rhs", _lib.RHS_T1), ("T2eq_rhs", _lib.RHS_T2), ("lam_1eq_rhs", _li
 # at most 32 rows) on the streaming VALU kernel; no skinny path (everything
 # small on the strided MFMA kernel); every batch-free contraction through the
 # dense-GEMM path; the dense-GEMM path with its K dimension split over 3
-# partial products.
+# partial products; the particle-particle ladders on all (i, j) rows instead
+# of the i < j rows of the antisymmetric amplitudes.
 MODES = {"default": {}, "skinny": {"TDCC_SKINNY_MIN": "0"}, "gett": {"TDCC_SKINNY_MIN": "-1"},
          "gemm": {"TDCC_GEMM_MIN_FLOPS": "0"},
-         "gemm_splitk": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_GEMM_SPLITS": "3"}}
-ENV_KEYS = ("TDCC_GEMM_MIN_FLOPS", "TDCC_GEMM_SPLITS", "TDCC_SKINNY_MIN")
+         "gemm_splitk": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_GEMM_SPLITS": "3"},
+         "full_rows": {"TDCC_IJ_PAIRS": "0"}}
+ENV_KEYS = ("TDCC_GEMM_MIN_FLOPS", "TDCC_GEMM_SPLITS", "TDCC_SKINNY_MIN", "TDCC_IJ_PAIRS")
 
 
 @pytest.fixture(scope="module", params=[(n, m) for m in MODES for n in ("o4v6", "c1_o10v4")],
@@ -89,3 +91,26 @@ def test_rk4_trajectory(case, method):
     state = ctx.get_state(0)
     for k, key in enumerate(("t1", "t2", "lam1", "lam2")):
         assert np.abs(state[k] - g[f"rk_{method}_{key}"][-1]).max() <= TOL, key
+
+
+@pytest.mark.parametrize("kind,name", [(_lib.RHS_T2, "T2eq_rhs"), (_lib.RHS_L2, "lam2eq_rhs")])
+def test_rhs_without_ij_antisymmetry(kind, name):
+    """Amplitudes that are not antisymmetric in (i, j) take the full-row
+    ladders; the result still matches the oracle's einsums on those inputs."""
+    from oracle.cc_equations import CCEquations
+    g = dict(np.load(os.path.join(HERE, "golden", "golden_o4v6.npz")))
+    o = int(g["o"])
+    rng = np.random.default_rng(7)
+    t2 = g["t2"] + 0.05 * (rng.standard_normal(g["t2"].shape) + 1j * rng.standard_normal(g["t2"].shape))
+    l2 = g["lam2"] + 0.05 * rng.standard_normal(g["lam2"].shape)
+    assert np.abs(t2 + t2.transpose(1, 0, 2, 3)).max() > 1e-3
+    eq = CCEquations(g["TEI"], o)
+    ref = eq.T2eq_rhs(g["t1"], t2, g["Fc"]) if name == "T2eq_rhs" else \
+        eq.lam2eq_rhs(g["t1"], t2, g["lam1"], l2, g["Fc"])
+    ctx = _lib.Context(o, int(g["v"]))
+    try:
+        ctx.set_system(g["TEI"], g["F"], g["mu"])
+        got = ctx.rhs(kind, g["Fc"], g["t1"], t2, g["lam1"], l2)
+    finally:
+        ctx.close()
+    assert np.abs(got - ref).max() <= TOL
