@@ -171,6 +171,22 @@ int CCBuilder::LWfiea(int t1) {  // CCSD_Helper.py:533-540
   return L(Terms().add(blk("vovv")).add(C("jiea,jf->fiea", blk("oovv"), t1), -1.0));
 }
 
+// The (v,o,v,v) LWfiea = <fi||ea> - sum_j <ji||ea> t_jf is only ever
+// contracted with an (o,v) or (v,v) tensor; distributing that contraction
+// over its two terms replaces the o v^3 intermediate by o^2 v^3 and o^3 v^2
+// contractions of the integral blocks themselves.
+//   sum_fe G_fe W_fiea = sum_fe G_fe <fi||ea> - sum_je (sum_f G_fe t_jf) <ji||ea>
+int CCBuilder::gef_weifa_term(int t1, int gef) {  // 'fe,fiea->ia' of CCSD_Helper.py:578
+  const int y = C("fe,jf->je", gef, t1);
+  return L(Terms().add(C("fe,fiea->ia", gef, blk("vovv"))).add(C("je,jiea->ia", y, blk("oovv")), -1.0));
+}
+
+//   sum_e W_ejab l_ie = sum_e <ej||ab> l_ie - sum_m (sum_e l_ie t_me) <mj||ab>
+int CCBuilder::weifa_l1_term(int t1, int l1) {  // 'ejab,ie->ijab' of CCSD_Helper.py:632, CC2_Helper.py:162
+  const int z = C("ie,me->im", l1, t1);
+  return L(Terms().add(C("ejab,ie->ijab", blk("vovv"), l1)).add(C("mjab,im->ijab", blk("oovv"), z), -1.0));
+}
+
 int CCBuilder::LWmina(int t1) {  // CCSD_Helper.py:543-550
   return L(Terms().add(blk("ooov")).add(C("mifa,nf->mina", blk("oovv"), t1)));
 }
@@ -179,7 +195,7 @@ int CCBuilder::lam_1eq_rhs(int t1, int t2, int l1, int l2, int F) {  // CCSD_Hel
   const int fia = Fme(t1, t2, F), fea = LRFea(t1, t2, F), fim = LRFim(t1, t2, F);
   const int wieam = LSWieam(t1, t2, F), wibjm = LRWibjm(t1, t2, F);
   const int gef = Gfe(t2, l2), gmn = Gmn(t2, l2);
-  const int weifa = LWfiea(t1), wmina = LWmina(t1);
+  const int wmina = LWmina(t1);
   return L(Terms()
                .add(fia)
                .add(C("ea,ie->ia", fea, l1))
@@ -187,7 +203,7 @@ int CCBuilder::lam_1eq_rhs(int t1, int t2, int l1, int l2, int F) {  // CCSD_Hel
                .add(C("ieam,me->ia", wieam, l1))
                .add(L1_wefam_term(t1, t2, l2, F))
                .add(C("ibjm,jmab->ia", wibjm, l2))
-               .add(C("fe,fiea->ia", gef, weifa), -1.0)
+               .add(gef_weifa_term(t1, gef), -1.0)
                .add(C("nm,mina->ia", gmn, wmina), -1.0));
 }
 
@@ -212,7 +228,7 @@ int CCBuilder::LSWabef(int t1, int t2, int F) {  // CCSD_Helper.py:598-607
 int CCBuilder::lam2eq_rhs(int t1, int t2, int l1, int l2, int F) {  // CCSD_Helper.py:610-648
   const int feb = LRFea(t1, t2, F), fjm = LRFim(t1, t2, F);
   const int wijmn = LSWmnij(t1, t2, F), wjebm = LSWieam(t1, t2, F);
-  const int wejab = LWfiea(t1), wijmb = LWmina(t1), fjb = Fme(t1, t2, F);
+  const int wijmb = LWmina(t1), fjb = Fme(t1, t2, F);
   const int gbe = Gfe(t2, l2), gmj = Gmn(t2, l2);
   const int oovv = blk("oovv");
   const int ring = L(Terms().add(C("jebm,imae->ijab", wjebm, l2)).add(C("jb,ia->ijab", fjb, l1)));
@@ -222,7 +238,7 @@ int CCBuilder::lam2eq_rhs(int t1, int t2, int l1, int l2, int F) {  // CCSD_Help
                         .add(C("ijfb,af->ijab", oovv, gbe)));
   const int p01 = L(Terms()
                         .add(C("jm,imab->ijab", fjm, l2), -1.0)
-                        .add(C("ejab,ie->ijab", wejab, l1))
+                        .add(weifa_l1_term(t1, l1))
                         .add(C("mjab,im->ijab", oovv, gmj), -1.0));
   const int direct = L(Terms()
                            .add(oovv)
@@ -294,13 +310,14 @@ int CCBuilder::L1eq_rhs_CC2(int t1, int t2, int l1, int l2, int F) {  // CC2_Hel
 
 int CCBuilder::L2eq_rhs_CC2(int t1, int t2, int l1, int l2, int F) {  // CC2_Helper.py:128-175
   const int fvv = fb(F, "vv"), foo = fb(F, "oo");
-  const int wmina = LWmina(t1), weifa = LWfiea(t1), fme = Fme(t1, t2, F);
+  const int wmina = LWmina(t1), fme = Fme(t1, t2, F);
   Terms t;
   t.add(blk("oovv"));
   t.add(C("ijab,eb->ijab", l2, fvv)).add(C("ijab,eb->ijba", l2, fvv), -1.0);
   t.add(C("imab,jm->ijab", l2, foo), -1.0).add(C("imab,jm->jiab", l2, foo));
   t.add(C("ijmb,ma->ijab", wmina, l1), -1.0).add(C("ijmb,ma->ijba", wmina, l1));
-  t.add(C("ejab,ie->ijab", weifa, l1)).add(C("ejab,ie->jiab", weifa, l1), -1.0);
+  const int wl = weifa_l1_term(t1, l1);
+  t.add(wl).addp(wl, {1, 0, 2, 3}, -1.0);
   const int pp = C("ia,jb->ijab", l1, fme);
   t.anti_both(pp);  // ijab - ijba - jiab + jiba of the same outer product
   return L(t);

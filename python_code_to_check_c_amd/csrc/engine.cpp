@@ -361,7 +361,7 @@ int Graph::lincomb(std::vector<Term> terms) {
   std::vector<Term> flat;
   for (const Term& t : terms) {
     const Node& s = nodes_[t.src];
-    if (s.op == Op::Lincomb && flat.size() + s.terms.size() + (terms.size() - 1) <= (size_t)kMaxTerms) {
+    if (s.op == Op::Lincomb && flat.size() + s.terms.size() + (terms.size() - 1) <= (size_t)kInlineTerms) {
       for (const Term& u : s.terms) {
         Term c;
         c.src = u.src;
@@ -707,6 +707,7 @@ void Graph::plan() {
     for (int x : ends[i]) release(nodes_[x].arena_off, size(nodes_[x].shape) * (nodes_[x].cplx ? 2 : 1) * (int64_t)sizeof(double));
   }
   arena_bytes_ = top;
+  if (const char* d = std::getenv("TDCC_DUMP_NODES"); d && d[0] == '1') dump_nodes();
   if (dev_tables_) {
     (void)hipFree(dev_tables_);
     dev_tables_ = nullptr;
@@ -720,6 +721,29 @@ void Graph::plan() {
     cache_buf_ = nullptr;
   }
   if (cache_bytes_ > 0) TDCC_HIP(hipMalloc(&cache_buf_, cache_bytes_));
+}
+
+// One line per live node on stderr: id, op, shape classes, planes, last use
+// and what the node computes (contraction string, lincomb terms with their
+// permutations and coefficients).
+void Graph::dump_nodes() const {
+  static const char* ops[] = {"ext", "view", "contract", "lincomb", "field", "finalize", "trace", "assign",
+                              "ladder", "field_vjp", "seed", "trace_vjp"};
+  for (int i = 0; i < (int)nodes_.size(); ++i) {
+    if (!live_[i]) continue;
+    const Node& n = nodes_[i];
+    std::string cls;
+    for (int k = 0; k < n.shape.rank; ++k) cls += "ovn"[(int)n.shape.cls[k]];
+    std::fprintf(stderr, "node %d %s [%s] %s last=%d", i, ops[(int)n.op], cls.c_str(), n.cplx ? "c" : "r", n.last_use);
+    if (n.op == Op::Contract) std::fprintf(stderr, " %s a=%d b=%d", n.spec.c_str(), n.a, n.b);
+    if (n.op == Op::View) std::fprintf(stderr, " base=%d", n.base);
+    if (n.op == Op::Ladder) std::fprintf(stderr, " u=%d", n.a);
+    if (n.op == Op::Lincomb)
+      for (const Term& t : n.terms)
+        std::fprintf(stderr, " %d(%d%d%d%d)*(%g,%g)", t.src, t.perm[0], t.perm[1], t.perm[2], t.perm[3], t.cr, t.ci);
+    if (n.op == Op::External) std::fprintf(stderr, " %s", n.key.c_str());
+    std::fprintf(stderr, "\n");
+  }
 }
 
 Dev Graph::resolve(int id) const {

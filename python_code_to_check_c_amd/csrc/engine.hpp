@@ -219,6 +219,7 @@ class Graph {
  private:
   int add(Node n);
   Dev resolve(int id) const;
+  void dump_nodes() const;  // TDCC_DUMP_NODES=1 at plan
   void lower_contract(int id);
   void lower_lincomb(int id);
 

@@ -339,7 +339,133 @@ static int grid_for(int64_t total) {
   return (int)b;
 }
 
+// Tiled variant for outputs whose innermost axis is contiguous while some
+// term reads it with a stride and has its own contiguous axis on the output's
+// second-innermost axis (a transposed pair of trailing indices, e.g. 'ijba'
+// summed into 'ijab').  A block owns one 32 x 32 tile of the two trailing axes
+// at one outer index; such a term is loaded along its contiguous axis into
+// LDS and read back transposed, so every global access is coalesced.  The
+// arithmetic per element is that of lincomb_kernel, term by term in order.
+constexpr int kLinTile = 32;
+
+__device__ __forceinline__ bool lincomb_transposed(const LincombTerm& T, int R) {
+  return T.st[R - 1] != 1 && T.st[R - 2] == 1;
+}
+
+__global__ __launch_bounds__(256) void lincomb_tiled_kernel(LincombArgs a) {
+  __shared__ double tr[kLinTile][kLinTile + 1];
+  __shared__ double ti[kLinTile][kLinTile + 1];
+  const int R = a.rank;
+  const int64_t D2 = a.dims[R - 2], D3 = a.dims[R - 1];
+  const int64_t n2 = (D2 + kLinTile - 1) / kLinTile, n3 = (D3 + kLinTile - 1) / kLinTile;
+  const int64_t b = blockIdx.x;
+  const int64_t outer = b / (n2 * n3), tile = b - outer * (n2 * n3);
+  const int64_t r0 = (tile / n3) * kLinTile, c0 = (tile % n3) * kLinTile;
+  // outer index -> leading axes (rank 4: (i0, i1); rank 3: i0; rank 2: none)
+  int64_t lead[2] = {0, 0};
+  if (R == 4) {
+    lead[0] = outer / a.dims[1];
+    lead[1] = outer - lead[0] * a.dims[1];
+  } else if (R == 3) {
+    lead[0] = outer;
+  }
+  const int lx = threadIdx.x & 31, ly = threadIdx.x >> 5;
+  int64_t obase = 0;
+  for (int d = 0; d < R - 2; ++d) obase += lead[d] * a.ost[d];
+  double sr[4], si[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    sr[k] = 0.0;
+    si[k] = 0.0;
+    const int64_t r = r0 + ly + 8 * k, c = c0 + lx;
+    if (a.accumulate && r < D2 && c < D3) {
+      const int64_t oo = obase + r * a.ost[R - 2] + c;
+      sr[k] = a.out_re[oo];
+      if (a.out_im) si[k] = a.out_im[oo];
+    }
+  }
+  for (int t = 0; t < a.nterms; ++t) {
+    const LincombTerm& T = a.t[t];
+    int64_t sbase = 0;
+    for (int d = 0; d < R - 2; ++d) sbase += lead[d] * T.st[d];
+    double xr[4], xi[4];
+    if (lincomb_transposed(T, R)) {
+      // stage: thread (lx, ly) loads trailing-axis element (r0 + lx, c0 + ly + 8k)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int64_t r = r0 + lx, c = c0 + ly + 8 * k;
+        const bool ok = r < D2 && c < D3;
+        const int64_t so = sbase + r + c * T.st[R - 1];
+        tr[ly + 8 * k][lx] = ok ? T.re[so] : 0.0;
+        ti[ly + 8 * k][lx] = (ok && T.im) ? T.im[so] : 0.0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        xr[k] = tr[lx][ly + 8 * k];
+        xi[k] = ti[lx][ly + 8 * k];
+      }
+      __syncthreads();
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int64_t r = r0 + ly + 8 * k, c = c0 + lx;
+        const bool ok = r < D2 && c < D3;
+        const int64_t so = sbase + r * T.st[R - 2] + c * T.st[R - 1];
+        xr[k] = ok ? T.re[so] : 0.0;
+        xi[k] = (ok && T.im) ? T.im[so] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      double pr, pi;
+      if (T.ci == 0.0) {
+        pr = __dmul_rn(T.cr, xr[k]);
+        pi = __dmul_rn(T.cr, xi[k]);
+      } else {
+        pr = __dsub_rn(__dmul_rn(T.cr, xr[k]), __dmul_rn(T.ci, xi[k]));
+        pi = __dadd_rn(__dmul_rn(T.cr, xi[k]), __dmul_rn(T.ci, xr[k]));
+      }
+      if (t == 0 && !a.accumulate) {
+        sr[k] = pr;
+        si[k] = pi;
+      } else {
+        sr[k] = __dadd_rn(sr[k], pr);
+        si[k] = __dadd_rn(si[k], pi);
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int64_t r = r0 + ly + 8 * k, c = c0 + lx;
+    if (r >= D2 || c >= D3) continue;
+    const int64_t oo = obase + r * a.ost[R - 2] + c;
+    a.out_re[oo] = sr[k];
+    if (a.out_im) a.out_im[oo] = si[k];
+  }
+}
+
+// Whether the tiled variant applies (and pays) for these arguments.
+static bool lincomb_use_tiled(const LincombArgs& a) {
+  if (a.rank < 2 || a.ost[a.rank - 1] != 1) return false;
+  for (int t = 0; t < a.nterms; ++t)
+    if (a.t[t].st[a.rank - 1] != 1 && a.t[t].st[a.rank - 2] == 1) return true;
+  return false;
+}
+
 void launch_lincomb(const LincombArgs& a, hipStream_t s) {
+  if (a.total <= 0) return;
+  if (lincomb_use_tiled(a)) {
+    const int R = a.rank;
+    const int64_t n2 = (a.dims[R - 2] + kLinTile - 1) / kLinTile, n3 = (a.dims[R - 1] + kLinTile - 1) / kLinTile;
+    int64_t outer = 1;
+    for (int d = 0; d < R - 2; ++d) outer *= a.dims[d];
+    const int64_t blocks = outer * n2 * n3;
+    if (blocks < (int64_t)1 << 31) {
+      hipLaunchKernelGGL(lincomb_tiled_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a);
+      return;
+    }
+  }
   hipLaunchKernelGGL(lincomb_kernel, dim3(grid_for(a.total)), dim3(256), 0, s, a);
 }
 
