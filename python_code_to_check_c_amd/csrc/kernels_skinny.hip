@@ -14,6 +14,7 @@
 namespace tdcc {
 
 constexpr int SK_KC = 64;  // k chunk staged in LDS
+constexpr int SK_KU = 8;   // big-operand loads in flight per thread (rows kernel)
 
 // Small-operand chunk in LDS, [i][k] with a padded row so that lanes reading
 // consecutive k hit distinct banks.
@@ -35,65 +36,89 @@ __device__ __forceinline__ void load_small(SmallChunk<SC, IT>& sm, const SkinnyA
   }
 }
 
-// Store (or write a partial of) the IT accumulators of big row j.
+// Store (or write a partial of) the IT accumulators of big row j, which hold
+// small rows i0 .. i0 + IT - 1.
 template <bool SC, bool GC, int IT>
-__device__ __forceinline__ void store_row(const SkinnyArgs& a, int64_t j, const double* acc_re, const double* acc_im) {
+__device__ __forceinline__ void store_row(const SkinnyArgs& a, int64_t j, const double* acc_re, const double* acc_im,
+                                          int i0 = 0) {
   constexpr bool CC = SC || GC;
   if (a.splits > 1) {
     double* p = a.partial + (int64_t)blockIdx.y * (CC ? 2 : 1) * a.I * a.J;
 #pragma unroll
-    for (int i = 0; i < IT; ++i)
+    for (int ii = 0; ii < IT; ++ii) {
+      const int i = i0 + ii;
       if (i < a.I) {
-        p[(int64_t)i * a.J + j] = acc_re[i];
-        if (CC) p[(int64_t)(a.I + i) * a.J + j] = acc_im[i];
+        p[(int64_t)i * a.J + j] = acc_re[ii];
+        if (CC) p[(int64_t)(a.I + i) * a.J + j] = acc_im[ii];
       }
+    }
     return;
   }
 #pragma unroll
-  for (int i = 0; i < IT; ++i)
+  for (int ii = 0; ii < IT; ++ii) {
+    const int i = i0 + ii;
     if (i < a.I) {
       const int64_t o = a.cs[i] + a.cb[j];
-      const double yr = a.alpha_re * acc_re[i] - (CC ? a.alpha_im * acc_im[i] : 0.0);
+      const double yr = a.alpha_re * acc_re[ii] - (CC ? a.alpha_im * acc_im[ii] : 0.0);
       a.c_re[o] = yr;
-      if (CC) a.c_im[o] = a.alpha_re * acc_im[i] + a.alpha_im * acc_re[i];
+      if (CC) a.c_im[o] = a.alpha_re * acc_im[ii] + a.alpha_im * acc_re[ii];
     }
+  }
 }
 
-// The big operand's contiguous index is a row index: one thread per big row,
-// 256 rows per block, loop over k (reads coalesced across threads).
-template <bool SC, bool GC, int IT>
+// The big operand's contiguous index is a row index: one lane per big row,
+// loop over k (reads coalesced across lanes).  With IG > 1 the IT small rows
+// are split over the block's IG waves, which then cover the same 64 big rows
+// (the repeated reads hit the CU's cache): fewer accumulators per thread,
+// more waves in flight.
+template <bool SC, bool GC, int IT, int IG>
 __global__ __launch_bounds__(256) void skinny_rows_kernel(SkinnyArgs a) {
   __shared__ SmallChunk<SC, IT> sm;
-  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  constexpr int RT = IT / IG;  // small rows per thread
+  const int ig = IG > 1 ? (int)(threadIdx.x >> 6) : 0;
+  const int i0 = ig * RT;
+  const int64_t j = (int64_t)blockIdx.x * (256 / IG) + (IG > 1 ? (threadIdx.x & 63) : threadIdx.x);
   const bool live = j < a.J;
   const int64_t kb = (int64_t)blockIdx.y * a.k_chunk, ke = min(a.K, kb + a.k_chunk);
   const double* gre = a.g_re + (live ? a.gr[j] : 0);
   const double* gim = GC ? a.g_im + (live ? a.gr[j] : 0) : nullptr;
-  double acc_re[IT], acc_im[IT];
+  double acc_re[RT], acc_im[RT];
 #pragma unroll
-  for (int i = 0; i < IT; ++i) acc_re[i] = acc_im[i] = 0.0;
+  for (int i = 0; i < RT; ++i) acc_re[i] = acc_im[i] = 0.0;
   for (int64_t k0 = kb; k0 < ke; k0 += SK_KC) {
     __syncthreads();
     load_small<SC, IT>(sm, a, k0, ke);
     __syncthreads();
     const int kn = (int)min((int64_t)SK_KC, ke - k0);
-    if (!live) continue;
-    for (int kk = 0; kk < kn; ++kk) {
-      const int64_t go = a.gk[k0 + kk];
-      const double br = gre[go];
-      const double bi = GC ? gim[go] : 0.0;
+    if (!live || i0 >= a.I) continue;
+    // the big-operand loads of SK_KU consecutive k are issued together, so a
+    // thread has that many in flight; the sums still run in k order
+    for (int kb8 = 0; kb8 < kn; kb8 += SK_KU) {
+      double br[SK_KU], bi[SK_KU];
 #pragma unroll
-      for (int i = 0; i < IT; ++i) {
-        const double sr = sm.re[i][kk];
-        const double si = SC ? sm.im[i][kk] : 0.0;
-        acc_re[i] = fma(sr, br, acc_re[i]);
-        if (SC && GC) acc_re[i] = fma(-si, bi, acc_re[i]);
-        if (GC) acc_im[i] = fma(sr, bi, acc_im[i]);
-        if (SC) acc_im[i] = fma(si, br, acc_im[i]);
+      for (int u = 0; u < SK_KU; ++u) {
+        const bool ok = kb8 + u < kn;
+        const int64_t go = ok ? a.gk[k0 + kb8 + u] : 0;
+        br[u] = ok ? gre[go] : 0.0;
+        bi[u] = (GC && ok) ? gim[go] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < SK_KU; ++u) {
+        if (kb8 + u >= kn) break;
+        const int kk = kb8 + u;
+#pragma unroll
+        for (int i = 0; i < RT; ++i) {
+          const double sr = sm.re[i0 + i][kk];
+          const double si = SC ? sm.im[i0 + i][kk] : 0.0;
+          acc_re[i] = fma(sr, br[u], acc_re[i]);
+          if (SC && GC) acc_re[i] = fma(-si, bi[u], acc_re[i]);
+          if (GC) acc_im[i] = fma(sr, bi[u], acc_im[i]);
+          if (SC) acc_im[i] = fma(si, br[u], acc_im[i]);
+        }
       }
     }
   }
-  if (live) store_row<SC, GC, IT>(a, j, acc_re, acc_im);
+  if (live) store_row<SC, GC, RT>(a, j, acc_re, acc_im, i0);
 }
 
 // The big operand's contiguous index lies in k: one wave per big row, lanes
@@ -158,10 +183,11 @@ __global__ void skinny_reduce_kernel(SkinnyArgs a, int cplx) {
 
 template <bool SC, bool GC, int IT>
 static void skinny_launch(const SkinnyArgs& a, hipStream_t s) {
-  const int64_t per = a.g_kmajor ? 4 : 256;
+  constexpr int IG = IT >= 8 ? 4 : 1;
+  const int64_t per = a.g_kmajor ? 4 : 256 / IG;
   const dim3 grid((unsigned)((a.J + per - 1) / per), (unsigned)a.splits), block(256);
   if (a.g_kmajor) hipLaunchKernelGGL((skinny_k_kernel<SC, GC, IT>), grid, block, 0, s, a);
-  else hipLaunchKernelGGL((skinny_rows_kernel<SC, GC, IT>), grid, block, 0, s, a);
+  else hipLaunchKernelGGL((skinny_rows_kernel<SC, GC, IT, IG>), grid, block, 0, s, a);
 }
 
 template <bool SC, bool GC>

@@ -366,6 +366,7 @@ __global__ __launch_bounds__(256) void ozgemm_kernel(OzGemmArgs g) {
   constexpr int CA = OZ_BM * 4 / 256, CB = BN * 4 / 256;  // 16-byte chunks per thread and digit
   __shared__ v4i As[2][S][OZ_BM * 4];
   __shared__ v4i Bs[2][S][BN * 4];
+  __shared__ double sA[OZ_BM], sB[BN];  // row and column scales of the tile
   const int tiles_m = (g.M + OZ_BM - 1) / OZ_BM, tiles_n = (g.N + BN - 1) / BN;
   const int L = remap_tile(blockIdx.x, tiles_m * tiles_n);
   const int tm = L % tiles_m, tn = L / tiles_m;
@@ -423,6 +424,8 @@ __global__ __launch_bounds__(256) void ozgemm_kernel(OzGemmArgs g) {
   const int lr = lane & 15, lk = lane >> 4;
 
   load_global(kb);
+  if (tid < OZ_BM) sA[tid] = m0 + tid < g.M ? g.sa[m0 + tid] : 0.0;
+  else if (tid - OZ_BM < BN) sB[tid - OZ_BM] = n0 + tid - OZ_BM < g.N ? g.sb[n0 + tid - OZ_BM] : 0.0;
   store_lds(0);
   __syncthreads();
   int buf = 0;
@@ -457,7 +460,7 @@ __global__ __launch_bounds__(256) void ozgemm_kernel(OzGemmArgs g) {
     for (int j = 0; j < JB; ++j) {
       const int col = n0 + wn + j * 16 + lr;
       if (col >= g.N) continue;
-      const double sbc = g.sb[col];
+      const double sbc = sB[wn + j * 16 + lr];
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -467,7 +470,7 @@ __global__ __launch_bounds__(256) void ozgemm_kernel(OzGemmArgs g) {
           double v = 0.0;
 #pragma unroll
           for (int q = S - 1; q >= 0; --q) v += ldexp((double)acc[q][i][j][r], -7 * (q + 2));
-          Cout[(int64_t)row * g.ldc + col] = v * (g.sa[row] * sbc);
+          Cout[(int64_t)row * g.ldc + col] = v * (sA[row - m0] * sbc);
         }
     }
     return;
@@ -480,7 +483,7 @@ __global__ __launch_bounds__(256) void ozgemm_kernel(OzGemmArgs g) {
   for (int j = 0; j < JB; ++j) {
     const int col = n0 + wn + j * 16 + lr;
     const bool colok = col < g.N;
-    const double sbc = colok ? g.sb[col] : 0.0;
+    const double sbc = sB[wn + j * 16 + lr];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int row0 = m0 + wm + i * 16 + 4 * lk;
@@ -490,7 +493,7 @@ __global__ __launch_bounds__(256) void ozgemm_kernel(OzGemmArgs g) {
         double v = 0.0;
 #pragma unroll
         for (int q = S - 1; q >= 0; --q) v += ldexp((double)acc[q][i][j][r], -7 * (q + 2));
-        z[r] = (colok && row0 + r < g.M) ? v * (g.sa[row0 + r] * sbc) : 0.0;
+        z[r] = (colok && row0 + r < g.M) ? v * (sA[row0 + r - m0] * sbc) : 0.0;
       }
       if (g.epi == 0) {
 #pragma unroll
