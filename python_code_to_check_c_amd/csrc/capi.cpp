@@ -72,6 +72,8 @@ struct tdcc_ctx {
   // t2 and lam2 of the state (of the checkpoint) are antisymmetric in
   // (i, j): the ladders then run on the i < j rows only
   bool pairs = true, backup_pairs = true;
+  // the whole integral array is antisymmetric in each index pair
+  bool tei_antisym = false;
   // batched trajectories of this system (tdcc_set_batch): per-slot state and
   // rounded base, per-slot field tables and dipole rows ([slot][cap][3][2])
   int nb = 0;
@@ -117,10 +119,11 @@ void ensure_staging(tdcc_ctx* c, int64_t n) {
   c->staging_n = n;
 }
 
-// True when x (interleaved complex, (o,o,v,v)) is antisymmetric in its first
-// two indices to 1e-12 of its largest entry, as CC amplitudes are: the
-// ladders may then compute the i < j rows only and mirror them, which agrees
-// with the full rows to that tolerance.  TDCC_IJ_PAIRS=0 turns this off.
+// True when x (interleaved complex, (o,o,v,v)) is antisymmetric in its two
+// index pairs to 1e-12 of its largest entry, as CC amplitudes are: the
+// ladders may then compute the i < j rows only and mirror them, and sums
+// over an antisymmetric index pair may run over p < q, which agrees with the
+// full sums to that tolerance.  TDCC_IJ_PAIRS=0 turns this off.
 bool antisym_ij(const double* x, int o, int v) {
   const char* e = std::getenv("TDCC_IJ_PAIRS");
   if (e && e[0] == '0') return false;
@@ -128,13 +131,17 @@ bool antisym_ij(const double* x, int o, int v) {
   const int64_t len = 2 * (int64_t)v * v;
   double mx = 0.0, dev = 0.0;
   for (int i = 0; i < o; ++i)
-    for (int j = 0; j <= i; ++j) {
+    for (int j = 0; j < o; ++j) {
       const double* a = x + ((int64_t)i * o + j) * len;
       const double* b = x + ((int64_t)j * o + i) * len;
-      for (int64_t k = 0; k < len; ++k) {
-        mx = std::max(mx, std::max(std::fabs(a[k]), std::fabs(b[k])));
-        dev = std::max(dev, std::fabs(a[k] + b[k]));
-      }
+      for (int p = 0; p < v; ++p)
+        for (int q = 0; q < v; ++q)
+          for (int c = 0; c < 2; ++c) {
+            const double xa = a[2 * ((int64_t)p * v + q) + c];
+            mx = std::max(mx, std::fabs(xa));
+            dev = std::max(dev, std::fabs(xa + b[2 * ((int64_t)p * v + q) + c]));
+            dev = std::max(dev, std::fabs(xa + a[2 * ((int64_t)q * v + p) + c]));
+          }
     }
   return dev <= 1e-12 * mx;
 }
@@ -192,7 +199,16 @@ int ext_tei(tdcc_ctx* c, Graph& g) {
   d.st[1] = n * n;
   d.st[2] = n;
   d.st[3] = 1;
-  return g.external(s, false, d, "tei");
+  const int id = g.external(s, false, d, "tei");
+  if (c->tei_antisym) g.set_anti(id, anti_bit(0, 1) | anti_bit(2, 3));
+  return id;
+}
+
+// t2 and lam2 (blocks 1 and 3) of an antisymmetric state are antisymmetric
+// in both index pairs.
+int mark_amplitude(Graph& g, int id, int k, bool pairs) {
+  if (pairs && (k == 1 || k == 3)) g.set_anti(id, anti_bit(0, 1) | anti_bit(2, 3));
+  return id;
 }
 
 void invalidate_exec(Program& p) {
@@ -244,6 +260,11 @@ Program& finish_program(tdcc_ctx* c, const std::string& key, std::unique_ptr<Gra
   return p;
 }
 
+// Program key suffix of the pair shortcuts a program was built with.
+std::string pairs_suffix(const tdcc_ctx* c, bool pairs) {
+  return pairs ? (c->tei_antisym ? ":pk" : ":p") : ":f";
+}
+
 int build_rhs_node(CCBuilder& b, int kind, int F, int t1, int t2, int l1, int l2) {
   switch (kind) {
     case TDCC_RHS_T1: return b.T1eq_rhs(t1, t2, F);
@@ -258,18 +279,19 @@ int build_rhs_node(CCBuilder& b, int kind, int F, int t1, int t2, int l1, int l2
 }
 
 Program& rhs_program(tdcc_ctx* c, int kind, bool pairs) {
-  const std::string key = "rhs:" + std::to_string(kind) + (pairs ? ":p" : ":f");
+  const std::string key = "rhs:" + std::to_string(kind) + pairs_suffix(c, pairs);
   auto it = c->progs.find(key);
   if (it != c->progs.end()) return it->second;
   auto g = std::make_unique<Graph>(c->o, c->v, &c->rt);
   g->set_ij_pairs(pairs);
+  g->set_k_pairs(pairs && c->tei_antisym);
   const int tei = ext_tei(c, *g);
   Shape sF;
   sF.rank = 2;
   sF.cls[0] = sF.cls[1] = Cls::N;
   const int F = ext_planes(*g, c->io_F, sF, "F");
   int y[4];
-  for (int k = 0; k < 4; ++k) y[k] = ext_planes(*g, c->io_in[k], shape_of_block(k), "in");
+  for (int k = 0; k < 4; ++k) y[k] = mark_amplitude(*g, ext_planes(*g, c->io_in[k], shape_of_block(k), "in"), k, pairs);
   CCBuilder b(*g, tei);
   const int r = build_rhs_node(b, kind, F, y[0], y[1], y[2], y[3]);
   const int out_k = (kind == TDCC_RHS_T1 || kind == TDCC_RHS_L1 || kind == TDCC_RHS_L1_CC2) ? 0 : 1;
@@ -372,16 +394,17 @@ std::string hexd(double x) {
 }
 
 Program& step_program(tdcc_ctx* c, int method, double dt) {
-  const std::string key = "step:" + std::to_string(method) + ":" + hexd(dt) + (c->pairs ? ":p" : ":f");
+  const std::string key = "step:" + std::to_string(method) + ":" + hexd(dt) + pairs_suffix(c, c->pairs);
   auto it = c->progs.find(key);
   if (it != c->progs.end()) return it->second;
   auto g = std::make_unique<Graph>(c->o, c->v, &c->rt);
   g->set_ij_pairs(c->pairs);
+  g->set_k_pairs(c->pairs && c->tei_antisym);
   StepIO io;
   io.tei = ext_tei(c, *g);
   for (int k = 0; k < 4; ++k) {
-    io.y[k] = ext_planes(*g, c->state[k], shape_of_block(k), "y");
-    io.base[k] = ext_planes(*g, c->base[k], shape_of_block(k), "base");
+    io.y[k] = mark_amplitude(*g, ext_planes(*g, c->state[k], shape_of_block(k), "y"), k, c->pairs);
+    io.base[k] = mark_amplitude(*g, ext_planes(*g, c->base[k], shape_of_block(k), "base"), k, c->pairs);
   }
   for (int k = 0; k < 4; ++k) io.ynew[k] = ext_planes(*g, c->state[k], shape_of_block(k), "ynew");
   build_rk4_step(*g, io, method, dt);
@@ -394,19 +417,20 @@ Program& batch_step_program(tdcc_ctx* c, int method, double dt) {
   bool pairs = true;
   for (char f : c->slot_pairs) pairs = pairs && f;
   const std::string key =
-      "bstep:" + std::to_string(method) + ":" + hexd(dt) + ":" + std::to_string(c->nb) + (pairs ? ":p" : ":f");
+      "bstep:" + std::to_string(method) + ":" + hexd(dt) + ":" + std::to_string(c->nb) + pairs_suffix(c, pairs);
   auto it = c->progs.find(key);
   if (it != c->progs.end()) return it->second;
   auto g = std::make_unique<Graph>(c->o, c->v, &c->rt);
   g->set_ij_pairs(pairs);
+  g->set_k_pairs(pairs && c->tei_antisym);
   const int tei = ext_tei(c, *g);
   for (int b = 0; b < c->nb; ++b) {
     g->set_traj(b);
     StepIO io;
     io.tei = tei;
     for (int k = 0; k < 4; ++k) {
-      io.y[k] = ext_planes(*g, c->bstate[b][k], shape_of_block(k), "y");
-      io.base[k] = ext_planes(*g, c->bbase[b][k], shape_of_block(k), "base");
+      io.y[k] = mark_amplitude(*g, ext_planes(*g, c->bstate[b][k], shape_of_block(k), "y"), k, pairs);
+      io.base[k] = mark_amplitude(*g, ext_planes(*g, c->bbase[b][k], shape_of_block(k), "base"), k, pairs);
     }
     for (int k = 0; k < 4; ++k) io.ynew[k] = ext_planes(*g, c->bstate[b][k], shape_of_block(k), "ynew");
     build_rk4_step(*g, io, method, dt);
@@ -595,6 +619,9 @@ void build_vpack(tdcc_ctx* c) {
   const unsigned long long bad = device_count(c);
   if (bad) throw Error("the <ab||ef> block is not antisymmetric (" + std::to_string(bad) + " entries); "
                        "the integrals must be antisymmetrized spin-orbital <pq||rs>");
+  launch_tei_antisym_check(c->tei, c->n, c->counts, c->stream);
+  TDCC_HIP(hipGetLastError());
+  c->tei_antisym = device_count(c) == 0;
   const int64_t P = (int64_t)c->v * (c->v - 1) / 2;
   if (P == 0) return;
   TDCC_HIP(hipMalloc(&c->vpack[0], P * P * sizeof(double)));

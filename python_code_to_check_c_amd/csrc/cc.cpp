@@ -187,6 +187,14 @@ int CCBuilder::weifa_l1_term(int t1, int l1) {  // 'ejab,ie->ijab' of CCSD_Helpe
   return L(Terms().add(C("ejab,ie->ijab", blk("vovv"), l1)).add(C("mjab,im->ijab", blk("oovv"), z), -1.0));
 }
 
+// With antisymmetric amplitudes, the part of x antisymmetric in its last two
+// indices, which is all that a sum against an amplitude over them sees (and
+// makes that sum run over pairs); otherwise x itself.
+int CCBuilder::mn_antisym(int x) {
+  if (!g_.k_pairs()) return x;
+  return L(Terms().add(x, 0.5).addp(x, {0, 1, 3, 2}, -0.5));
+}
+
 int CCBuilder::LWmina(int t1) {  // CCSD_Helper.py:543-550
   return L(Terms().add(blk("ooov")).add(C("mifa,nf->mina", blk("oovv"), t1)));
 }
@@ -242,7 +250,7 @@ int CCBuilder::lam2eq_rhs(int t1, int t2, int l1, int l2, int F) {  // CCSD_Help
                         .add(C("mjab,im->ijab", oovv, gmj), -1.0));
   const int direct = L(Terms()
                            .add(oovv)
-                           .add(C("ijmn,mnab->ijab", wijmn, l2))
+                           .add(C("ijmn,mnab->ijab", mn_antisym(wijmn), l2))
                            .add(L2_vvvv_term(t1, t2, l2)));
   return L(Terms().add(direct).anti23(p23).anti01(p01).anti_both(ring));
 }
@@ -365,7 +373,9 @@ int CCBuilder::T2_vvvv_terms(int t1, int t2, int tau) {
   // 1/4 sum_mn <mn||ef> t_mnab and P(ab) 1/2 sum_mn <mn||ef> t_ma t_nb, on t2
   const int g = C("mnef,ijef->ijmn", oovv, t2);
   const int y = C("ijmn,mnab->ijab", g, t2);
-  const int z = C("ijmn,mnab->ijab", g, tt);
+  // g is antisymmetric in (m, n) (from <mn||ef>), so only the (m, n)
+  // antisymmetric part of t_ma t_nb, half of tau, meets it
+  const int z = g_.k_pairs() ? C("ijmn,mnab->ijab", g, tau, 0.5) : C("ijmn,mnab->ijab", g, tt);
   // 1/4 sum_mn <mn||ef> tau2_mnab of Wabef_2, on tau
   const int tau2 = L(Terms().add(tt).addp(tt, {0, 1, 3, 2}, -1.0));
   const int g2 = C("mnef,ijef->ijmn", oovv, tau);
@@ -391,7 +401,12 @@ int CCBuilder::T2_vvvv_terms_CC2(int t1, int tau) {
 int CCBuilder::L2_vvvv_term(int t1, int t2, int l2) {
   // sum_ef LSWabef_efab lam_ijef, LSWabef = 1/2 <ef||ab> - <em||ab> t_mf + <nm||ab> tau'_nmef
   const int vovv = blk("vovv"), oovv = blk("oovv");
-  const int tau = L(Terms().add(t2, 0.25).add(t1t1(t1), 0.5));
+  // Only the part of tau antisymmetric in (e, f) meets lam_ijef; with
+  // antisymmetric amplitudes that part is used, which makes h antisymmetric
+  // in (n, m) too and lets both sums over those pairs run over n < m, e < f.
+  const int tt = t1t1(t1);
+  const int tau = g_.k_pairs() ? L(Terms().add(t2, 0.25).add(tt, 0.25).addp(tt, {0, 1, 3, 2}, -0.25))
+                               : L(Terms().add(t2, 0.25).add(tt, 0.5));
   // 'efab,ijef->ijab' over pairs e<f, a<b
   const int ladder = g_.ladder(l2, 1);
   const int p = C("mf,ijef->ijem", t1, l2);

@@ -61,6 +61,7 @@ class CCBuilder {
   int LWfiea(int t1);
   int LWmina(int t1);
   int gef_weifa_term(int t1, int gef);  // Gfe . LWfiea, distributed
+  int mn_antisym(int x);                 // antisymmetric part in the last index pair
   int weifa_l1_term(int t1, int l1);    // LWfiea . lam1, distributed
   int lam_1eq_rhs(int t1, int t2, int l1, int l2, int F);
   int LSWmnij(int t1, int t2, int F);

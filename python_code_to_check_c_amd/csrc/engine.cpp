@@ -183,7 +183,15 @@ int Graph::block(int base, const char* classes) {
     s.cls[d] = cls_of(classes[d]);
     if (s.cls[d] == Cls::V) off += o_ * st[d];
   }
-  return view(base, s, off, st);
+  const int id = view(base, s, off, st);
+  // the base's antisymmetric axis pairs that stay inside the block
+  uint8_t m = 0;
+  if (nodes_[base].shape.rank == 4)
+    for (int d1 = 0; d1 < 4; ++d1)
+      for (int d2 = d1 + 1; d2 < 4; ++d2)
+        if ((nodes_[base].anti & anti_bit(d1, d2)) && s.cls[d1] == s.cls[d2]) m |= anti_bit(d1, d2);
+  nodes_[id].anti = m;
+  return id;
 }
 
 int Graph::fblock(int base, const char* classes) {
@@ -297,6 +305,18 @@ int Graph::contract(const std::string& spec, int a, int b, double ar, double ai)
   n.b = b;
   n.ar = ar;
   n.ai = ai;
+  n.xsym = a == b && la.size() == 2 && lb.size() == 2 && lc.size() == 4 && lc[0] == la[0] && lc[1] == lb[0] &&
+           lc[2] == la[1] && lc[3] == lb[1] && ai == 0.0;
+  // output axis pairs taken from one antisymmetric pair of an operand
+  for (int d1 = 0; d1 < n.shape.rank; ++d1)
+    for (int d2 = d1 + 1; d2 < n.shape.rank; ++d2)
+      for (int side = 0; side < 2; ++side) {
+        const std::string& lx = side ? lb : la;
+        const size_t x1 = lx.find(lc[d1]), x2 = lx.find(lc[d2]);
+        if (x1 != std::string::npos && x2 != std::string::npos &&
+            (nodes_[side ? b : a].anti & anti_bit((int)x1, (int)x2)))
+          n.anti |= anti_bit(d1, d2);
+      }
   n.key = "C|" + nspec + "|" + std::to_string(a) + "|" + std::to_string(b) + "|" + hexf(ar) + "|" + hexf(ai);
   return add(std::move(n));
 }
@@ -389,6 +409,39 @@ int Graph::lincomb(std::vector<Term> terms) {
     n.cplx = n.cplx || s.cplx || t.ci != 0.0;
     k << "|" << t.src << ":" << t.perm[0] << t.perm[1] << t.perm[2] << t.perm[3] << ":" << hexf(t.cr) << ":" << hexf(t.ci);
   }
+  // an axis pair is antisymmetric when every term either reads an
+  // antisymmetric pair of its source there or is cancelled in sign by a
+  // partner term reading the same source with the two axes exchanged
+  for (int d1 = 0; d1 < n.shape.rank; ++d1)
+    for (int d2 = d1 + 1; d2 < n.shape.rank; ++d2) {
+      bool ok = true;
+      for (size_t t = 0; t < flat.size() && ok; ++t) {
+        const Term& T = flat[t];
+        if (nodes_[T.src].anti & anti_bit(T.perm[d1], T.perm[d2])) continue;
+        // the reading of T with output axes d1, d2 exchanged (and, for a
+        // source unchanged under exchanging its axes 0, 1 and 2, 3 together,
+        // the same reading with that exchange applied)
+        int want[4], alt[4];
+        for (int d = 0; d < n.shape.rank; ++d) {
+          want[d] = T.perm[d == d1 ? d2 : (d == d2 ? d1 : d)];
+          alt[d] = want[d] ^ 1;
+        }
+        const bool xs = nodes_[T.src].xsym;
+        bool partner = false;
+        for (size_t u = 0; u < flat.size() && !partner; ++u) {
+          const Term& U = flat[u];
+          if (u == t || U.src != T.src || U.cr != -T.cr || U.ci != -T.ci) continue;
+          bool same = true, same_alt = xs;
+          for (int d = 0; d < n.shape.rank; ++d) {
+            same = same && U.perm[d] == want[d];
+            same_alt = same_alt && U.perm[d] == alt[d];
+          }
+          partner = same || same_alt;
+        }
+        ok = partner;
+      }
+      if (ok) n.anti |= anti_bit(d1, d2);
+    }
   n.terms = std::move(flat);
   n.key = k.str();
   return add(std::move(n));
@@ -418,6 +471,7 @@ int Graph::ladder(int u, int ket) {
   n.a = u;
   n.stage = ket;
   n.ij_pairs = ij_pairs_;
+  n.anti = (uint8_t)(anti_bit(2, 3) | (U.anti & anti_bit(0, 1)));
   n.key = "LAD|" + std::to_string(u) + "|" + std::to_string(ket) + "|" + (ij_pairs_ ? "p" : "f");
   return add(std::move(n));
 }
@@ -491,6 +545,7 @@ Graph::VjpResult Graph::vjp(const std::vector<std::pair<int, Term>>& seeds) {
         }
         break;
       case Op::Contract: {
+        if (!n.kpair.empty()) throw Error("vjp: pair-restricted contraction");
         std::string la, lb, lc;
         letters_of(n.spec, la, lb, lc);
         if (var[n.a]) acc[n.a].push_back(tm(contract(lc + "," + lb + "->" + la, bar, n.b, n.ar, n.ai)));
@@ -734,8 +789,10 @@ void Graph::dump_nodes() const {
     const Node& n = nodes_[i];
     std::string cls;
     for (int k = 0; k < n.shape.rank; ++k) cls += "ovn"[(int)n.shape.cls[k]];
-    std::fprintf(stderr, "node %d %s [%s] %s last=%d", i, ops[(int)n.op], cls.c_str(), n.cplx ? "c" : "r", n.last_use);
-    if (n.op == Op::Contract) std::fprintf(stderr, " %s a=%d b=%d", n.spec.c_str(), n.a, n.b);
+    std::fprintf(stderr, "node %d %s [%s] %s last=%d anti=%d%s", i, ops[(int)n.op], cls.c_str(), n.cplx ? "c" : "r",
+                 n.last_use, (int)n.anti, n.xsym ? "x" : "");
+    if (n.op == Op::Contract)
+      std::fprintf(stderr, " %s a=%d b=%d%s%s", n.spec.c_str(), n.a, n.b, n.kpair.empty() ? "" : " pairs=", n.kpair.c_str());
     if (n.op == Op::View) std::fprintf(stderr, " base=%d", n.base);
     if (n.op == Op::Ladder) std::fprintf(stderr, " u=%d", n.a);
     if (n.op == Op::Lincomb)
@@ -770,20 +827,29 @@ Dev Graph::resolve(int id) const {
 
 // Offset tables of one index group: row-major enumeration of `letters` with
 // the given strides.
+// Offsets of every combination of the letters (first letter slowest); with
+// a pair "pq" of them, only the combinations with index(p) < index(q).
 static void group_table(const std::string& letters, const std::map<char, int64_t>& ext,
-                        const std::map<char, int64_t>& st, std::vector<int64_t>& out) {
+                        const std::map<char, int64_t>& st, std::vector<int64_t>& out,
+                        const std::string& pair = "") {
   int64_t total = 1;
   for (char c : letters) total *= ext.at(c);
-  out.assign(total, 0);
+  const int dp = pair.empty() ? -1 : (int)letters.find(pair[0]);
+  const int dq = pair.empty() ? -1 : (int)letters.find(pair[1]);
+  out.clear();
+  out.reserve(total);
   for (int64_t idx = 0; idx < total; ++idx) {
-    int64_t rem = idx, off = 0;
+    int64_t rem = idx, off = 0, ip = 0, iq = 0;
     for (int d = (int)letters.size() - 1; d >= 0; --d) {
       const int64_t e = ext.at(letters[d]);
       const int64_t i = rem % e;
       rem /= e;
       off += i * st.at(letters[d]);
+      if (d == dp) ip = i;
+      if (d == dq) iq = i;
     }
-    out[idx] = off;
+    if (dp >= 0 && dq >= 0 && ip >= iq) continue;
+    out.push_back(off);
   }
 }
 
@@ -852,7 +918,23 @@ void Graph::lower_contract(int id) {
   // Large contractions run as dense GEMMs on the DGEMM kernel; the complex
   // operand (if only one is complex) goes on the row side, planes stacked.
   const double mul0 = (nodes_[a].cplx && nodes_[b].cplx) ? 8 : ((nodes_[a].cplx || nodes_[b].cplx) ? 4 : 2);
-  const double gflops = mul0 * gsize(gm) * (double)gsize(gn) * gsize(gk);
+  // two summed letters over which both operands are antisymmetric: the
+  // summand is symmetric in them, so the sum is twice the sum over p < q
+  n.kpair.clear();
+  n.kfac = 1.0;
+  if (k_pairs_ && gb.empty())
+    for (size_t x = 0; x < gk.size() && n.kpair.empty(); ++x)
+      for (size_t y = x + 1; y < gk.size() && n.kpair.empty(); ++y) {
+        const char p = gk[x], q = gk[y];
+        if (ext[p] != ext[q]) continue;
+        if ((nodes_[a].anti & anti_bit((int)la.find(p), (int)la.find(q))) &&
+            (nodes_[b].anti & anti_bit((int)lb.find(p), (int)lb.find(q)))) {
+          n.kpair = std::string{p, q};
+          n.kfac = 2.0;
+        }
+      }
+  const double kpair_frac = n.kpair.empty() ? 1.0 : (ext[n.kpair[0]] - 1.0) / (2.0 * ext[n.kpair[0]]);
+  const double gflops = mul0 * gsize(gm) * (double)gsize(gn) * gsize(gk) * kpair_frac;
   const double min_flops = gemm_min_flops();
   // dot-product shapes (both free groups small, K long) also stream well
   // through the GEMM path's split-K
@@ -900,13 +982,13 @@ void Graph::lower_contract(int id) {
   group_table(gm, ext, sa, t);
   L->tab[0] = push(t);
   const int64_t M = (int64_t)t.size();
-  group_table(gk, ext, sa, t);
+  group_table(gk, ext, sa, t, n.kpair);
   L->tab[1] = push(t);
   const int64_t K = (int64_t)t.size();
   group_table(gn, ext, sb, t);
   L->tab[2] = push(t);
   const int64_t Nn = (int64_t)t.size();
-  group_table(gk, ext, sb, t);
+  group_table(gk, ext, sb, t, n.kpair);
   L->tab[3] = push(t);
   group_table(gm, ext, sc, t);
   L->tab[4] = push(t);
@@ -930,8 +1012,8 @@ void Graph::lower_contract(int id) {
   g.batch = (int)Bt;
   g.tiles_m = (int)((M + 63) / 64);
   g.tiles_n = (int)((Nn + 63) / 64);
-  g.alpha_re = n.ar;
-  g.alpha_im = n.ai;
+  g.alpha_re = n.ar * n.kfac;
+  g.alpha_im = n.ai * n.kfac;
   g.accumulate = 0;
   const int64_t tiles = (int64_t)g.tiles_m * g.tiles_n * Bt;
   int splits = 1;
@@ -976,8 +1058,8 @@ void Graph::lower_contract(int id) {
     sp = (K + ch - 1) / ch;
     k.splits = (int)sp;
     k.k_chunk = ch;
-    k.alpha_re = n.ar;
-    k.alpha_im = n.ai;
+    k.alpha_re = n.ar * n.kfac;
+    k.alpha_im = n.ai * n.kfac;
     n.scratch_len = sp > 1 ? sp * (L->ac || L->bc ? 2 : 1) * (int64_t)k.I * k.J * (int64_t)sizeof(double) : 0;
   }
   if (use_gemm) {
@@ -1021,8 +1103,8 @@ void Graph::lower_contract(int id) {
       n.scratch_len = gp.zb_off + (gp.fused ? 0 : al256(8 * sp * R * NB));
       auto cache_entry = [&](int x, const std::string& rows, bool cols, int64_t nrows, int64_t plane) {
         if (!is_static(x)) return -1;
-        const std::string key = nodes_[x].key + "|" + rows + "|" + gk + "|" + (cols ? "c" : "r") + "|" +
-                                std::to_string(nrows) + "|" + std::to_string(S);
+        const std::string key = nodes_[x].key + "|" + rows + "|" + gk + "|" + n.kpair + "|" + (cols ? "c" : "r") +
+                                "|" + std::to_string(nrows) + "|" + std::to_string(S);
         auto it = cache_index_.find(key);
         if (it != cache_index_.end()) return it->second;
         DigitCache e;
@@ -1188,8 +1270,8 @@ void Graph::bind(double* arena) {
             oz.cn = g.cn;
             oz.out_re = C.re;
             oz.out_im = n.cplx ? C.im : nullptr;
-            oz.alpha_re = n.ar;
-            oz.alpha_im = n.ai;
+            oz.alpha_re = n.ar * n.kfac;
+            oz.alpha_im = n.ai * n.kfac;
             if (!L->ac && L->bc) throw Error("internal: complex operand not on the row side");
           }
           GemmScatterArgs& sc = L->sc;
@@ -1205,8 +1287,8 @@ void Graph::bind(double* arena) {
           sc.cn = g.cn;
           sc.out_re = C.re;
           sc.out_im = n.cplx ? C.im : nullptr;
-          sc.alpha_re = n.ar;
-          sc.alpha_im = n.ai;
+          sc.alpha_re = n.ar * n.kfac;
+          sc.alpha_im = n.ai * n.kfac;
           sc.accumulate = 0;
         } else if (L->gp.on) {
           const GemmPath& gp = L->gp;
@@ -1271,8 +1353,8 @@ void Graph::bind(double* arena) {
           sc.cn = g.cn;
           sc.out_re = C.re;
           sc.out_im = n.cplx ? C.im : nullptr;
-          sc.alpha_re = n.ar;
-          sc.alpha_im = n.ai;
+          sc.alpha_re = n.ar * n.kfac;
+          sc.alpha_im = n.ai * n.kfac;
           sc.accumulate = 0;
         }
         launches_.push_back(std::move(L));

@@ -81,6 +81,16 @@ struct Node {
   bool fp64 = false;
   // Ladder: only the i < j rows are computed (input antisymmetric in (i, j))
   bool ij_pairs = false;
+  // Contract: two summed letters p, q of which only the terms p < q are
+  // summed (empty: all), and the factor the launch applies then
+  std::string kpair;
+  double kfac = 1.0;
+  // antisymmetric axis pairs: bit anti_bit(d1, d2) set when the tensor
+  // changes sign under exchanging axes d1 and d2
+  uint8_t anti = 0;
+  // unchanged under exchanging axes 0, 1 and axes 2, 3 together (the outer
+  // product 'ia,jb->ijab' of a tensor with itself)
+  bool xsym = false;
 };
 
 struct Handle {
@@ -151,6 +161,15 @@ class Graph {
   int block(int base_n4, const char* classes);       // TEI block, e.g. "ovvv"
   int fblock(int base_nn, const char* classes);      // (n,n) block, e.g. "ov"
   int contract(const std::string& spec, int a, int b, double ar = 1.0, double ai = 0.0);
+  // Declares an external antisymmetric under the given axis pairs (bits of
+  // anti_bit); views of integral blocks, linear combinations, contractions
+  // and ladders derive theirs from their inputs.
+  void set_anti(int id, uint8_t mask) { nodes_[id].anti = mask; }
+  // A contraction summing over two letters in which both operands are
+  // antisymmetric has a summand symmetric under exchanging them: it is then
+  // lowered as twice the sum over the pairs p < q.  Only while enabled.
+  void set_k_pairs(bool on) { k_pairs_ = on; }
+  bool k_pairs() const { return k_pairs_; }
   int contract3(const std::string& spec, int a, int b, int c, double ar = 1.0);
   int lincomb(std::vector<Term> terms);
   int field(int stage);
@@ -239,6 +258,7 @@ class Graph {
   int fp64_depth_ = 0;
   int traj_ = -1;
   bool ij_pairs_ = false;
+  bool k_pairs_ = false;
   // digit planes of static operands (split precision), filled at bind
   bool is_static(int id) const;
   struct DigitCache {
@@ -251,6 +271,10 @@ class Graph {
   char* cache_buf_ = nullptr;
   double* ones_buf_ = nullptr;
 };
+
+constexpr uint8_t anti_bit(int d1, int d2) {
+  return (uint8_t)(1u << (d1 < d2 ? (d1 == 0 ? d2 - 1 : (d1 == 1 ? d2 + 1 : 5)) : (d2 == 0 ? d1 - 1 : (d2 == 1 ? d1 + 1 : 5))));
+}
 
 // Elementwise helpers for the builders.
 Term tm(int src, double c = 1.0);                           // identity permutation

@@ -918,6 +918,23 @@ __global__ void antisym_check_kernel(const double* tei, int o, int v, unsigned l
   block_count_store(bad, partial);
 }
 
+// Entries of the whole (n,n,n,n) block that break <pq||rs> = -<qp||rs> =
+// -<pq||sr> (exact comparison).
+__global__ void tei_antisym_check_kernel(const double* tei, int n, unsigned long long* partial) {
+  const int64_t nn = (int64_t)n * n, total = nn * nn;
+  unsigned long long bad = 0;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t pq = idx / nn, rs = idx - pq * nn;
+    const int64_t p = pq / n, q = pq - p * n, r = rs / n, t = rs - r * n;
+    const double x = tei[idx];
+    const double y = tei[(q * n + p) * nn + rs];
+    const double z = tei[pq * nn + t * n + r];
+    bad += (x != -y) || (x != -z);
+  }
+  block_count_store(bad, partial);
+}
+
 __global__ void mismatch_kernel(const double* x, const double* y, int64_t n, unsigned long long* partial) {
   unsigned long long bad = 0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -936,6 +953,11 @@ __global__ void sum_counts_kernel(const unsigned long long* partial, int nb, uns
 // `count` must hold kCountBlocks + 1 entries; the total lands in count[kCountBlocks].
 void launch_vvvv_antisym_check(const double* tei, int o, int v, unsigned long long* count, hipStream_t s) {
   hipLaunchKernelGGL(antisym_check_kernel, dim3(kCountBlocks), dim3(256), 0, s, tei, o, v, count);
+  hipLaunchKernelGGL(sum_counts_kernel, dim3(1), dim3(64), 0, s, count, kCountBlocks, count + kCountBlocks);
+}
+
+void launch_tei_antisym_check(const double* tei, int n, unsigned long long* count, hipStream_t s) {
+  hipLaunchKernelGGL(tei_antisym_check_kernel, dim3(kCountBlocks), dim3(256), 0, s, tei, n, count);
   hipLaunchKernelGGL(sum_counts_kernel, dim3(1), dim3(64), 0, s, count, kCountBlocks, count + kCountBlocks);
 }
 

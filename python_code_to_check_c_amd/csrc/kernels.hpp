@@ -161,6 +161,8 @@ void launch_build_vpack(const double* tei, int o, int v, int ladder_ket, double*
 // `count` holds kCountBlocks + 1 entries; the total lands in count[kCountBlocks].
 constexpr int kCountBlocks = 1024;
 void launch_vvvv_antisym_check(const double* tei, int o, int v, unsigned long long* count, hipStream_t s);
+// The same count over the whole (n,n,n,n) array.
+void launch_tei_antisym_check(const double* tei, int n, unsigned long long* count, hipStream_t s);
 // Counts mismatches between two equally shaped arrays.
 void launch_count_mismatch(const double* x, const double* y, int64_t n, unsigned long long* count, hipStream_t s);
 
