@@ -9,6 +9,7 @@
 namespace tdcc {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
 
 // ---------------------------------------------------------------------------
 // Row split in three launches:
@@ -528,6 +529,149 @@ __global__ __launch_bounds__(256) void ozgemm_kernel(OzGemmArgs g) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Digit-pair GEMM for up to 4 digits on v_mfma_i32_32x32x32_i8: tile 128 x 128
+// x 64, 512 threads = 8 waves (2 x 4), two per SIMD so that one wave's LDS
+// and memory waits overlap the other's MFMAs; a wave owns 64 x 32 = 2 MFMA
+// blocks of 32 x 32 per digit group, accumulators in VGPRs.  Lane l supplies
+// A[row l%32][k 16(l/32) .. +15] of a 32-wide K slice (two slices per K step)
+// and holds D[8(r/4) + 4(l/32) + r%4][l%32] in accumulator element r
+// (checked against the 16x16x64 kernel's output).  Operands are read with
+// buffer loads from rows clamped to the matrix (the extra rows only reach
+// outputs that are not stored); LDS layout and swizzle as above.
+// ---------------------------------------------------------------------------
+template <int S>
+__global__ __launch_bounds__(512, 1) void ozgemm8_kernel(OzGemmArgs g) {
+  constexpr int BN = 128;
+  __shared__ v4i As[2][S][OZ_BM * 4];
+  __shared__ v4i Bs[2][S][BN * 4];
+  __shared__ double sA[OZ_BM], sB[BN];
+  const int tiles_m = (g.M + OZ_BM - 1) / OZ_BM, tiles_n = (g.N + BN - 1) / BN;
+  const int L = remap_tile(blockIdx.x, tiles_m * tiles_n);
+  const int tm = L % tiles_m, tn = L / tiles_m;
+  const int m0 = tm * OZ_BM, n0 = tn * BN;
+  const int tid = threadIdx.x;
+  const int kb = g.k_chunk > 0 ? (int)blockIdx.y * g.k_chunk : 0;
+  const int ke = g.k_chunk > 0 ? min(g.Kp, kb + g.k_chunk) : g.Kp;
+  double* const Cout = g.C + (int64_t)blockIdx.y * g.c_split_stride;
+  const __amdgpu_buffer_rsrc_t rA =
+      __builtin_amdgcn_make_buffer_rsrc((void*)g.A, (short)0, (int)(S * g.a_plane), 0x00020000);
+  const __amdgpu_buffer_rsrc_t rB =
+      __builtin_amdgcn_make_buffer_rsrc((void*)g.B, (short)0, (int)(S * g.b_plane), 0x00020000);
+  // one 16-byte chunk of A and one of B per thread and digit
+  const int va = min(m0 + (tid >> 2), g.M - 1) * g.Kp + (tid & 3) * 16;
+  const int vb = min(n0 + (tid >> 2), g.N - 1) * g.Kp + (tid & 3) * 16;
+  v4i ra[S], rb[S];
+  auto load_global = [&](int k0) {
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      ra[s] = __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(rA, va, (int)(s * g.a_plane) + k0, 0));
+      rb[s] = __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(rB, vb, (int)(s * g.b_plane) + k0, 0));
+    }
+  };
+  auto store_lds = [&](int buf) {
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      As[buf][s][oz_swz(tid >> 2, tid & 3)] = ra[s];
+      Bs[buf][s][oz_swz(tid >> 2, tid & 3)] = rb[s];
+    }
+  };
+  v16i acc[S][2];
+#pragma unroll
+  for (int q = 0; q < S; ++q)
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[q][i][e] = 0;
+  const int lane = tid & 63, w = tid >> 6;
+  const int wm = (w & 1) * 64, wn = (w >> 1) * 32;
+  const int lr = lane & 31, lh = lane >> 5;
+
+  load_global(kb);
+  if (tid < OZ_BM) sA[tid] = m0 + tid < g.M ? g.sa[m0 + tid] : 0.0;
+  else if (tid - OZ_BM < BN) sB[tid - OZ_BM] = n0 + tid - OZ_BM < g.N ? g.sb[n0 + tid - OZ_BM] : 0.0;
+  store_lds(0);
+  __syncthreads();
+  int buf = 0;
+  for (int k0 = kb; k0 < ke; k0 += OZ_BK) {
+    const bool more = k0 + OZ_BK < ke;
+    if (more) load_global(k0 + OZ_BK);
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk) {
+      v4i a[S][2], b[S];
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) a[s][i] = As[buf][s][oz_swz(wm + i * 32 + lr, 2 * kk + lh)];
+        b[s] = Bs[buf][s][oz_swz(wn + lr, 2 * kk + lh)];
+      }
+#pragma unroll
+      for (int t = 0; t < S; ++t)
+#pragma unroll
+        for (int s = 0; s + t < S; ++s)
+#pragma unroll
+          for (int i = 0; i < 2; ++i)
+            acc[s + t][i] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[s][i], b[t], acc[s + t][i], 0, 0, 0);
+    }
+    if (more) store_lds(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+  // digit group q (pairs with s + t = q + 2, 1-based digits) weighs 128^-(q+2)
+  const int col = n0 + wn + lr;
+  const bool colok = col < g.N;
+  const double sbc = sB[wn + lr];
+  const double ar = g.alpha_re, ai = g.alpha_im;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int gq = 0; gq < 4; ++gq) {
+      // rows row0 .. row0 + 3 of this lane; row0 is even
+      const int row0 = m0 + wm + i * 32 + 8 * gq + 4 * lh;
+      double z[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        double v = 0.0;
+#pragma unroll
+        for (int q = S - 1; q >= 0; --q) v += ldexp((double)acc[q][i][4 * gq + r], -7 * (q + 2));
+        z[r] = (colok && row0 + r < g.M) ? v * (sA[row0 + r - m0] * sbc) : 0.0;
+      }
+      if (g.epi < 0) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (colok && row0 + r < g.M) Cout[(int64_t)(row0 + r) * g.ldc + col] = z[r];
+      } else if (g.epi == 0) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (colok && row0 + r < g.M) g.out_re[g.cm[row0 + r] + g.cn[col]] = ar * z[r];
+      } else if (g.epi == 1) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          if (!colok || row0 + 2 * h >= g.M) continue;
+          const double re = z[2 * h], im = z[2 * h + 1];
+          const int64_t o = g.cm[(row0 >> 1) + h] + g.cn[col];
+          g.out_re[o] = ar * re - ai * im;
+          g.out_im[o] = ar * im + ai * re;
+        }
+      } else {
+        double x[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) x[r] = __shfl_xor(z[r], 1);
+        if ((lr & 1) == 0 && colok) {
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            if (row0 + 2 * h >= g.M) continue;
+            const double re = z[2 * h] - x[2 * h + 1];  // rr - ii
+            const double im = x[2 * h] + z[2 * h + 1];  // ri + ir
+            const int64_t o = g.cm[(row0 >> 1) + h] + g.cn[col >> 1];
+            g.out_re[o] = ar * re - ai * im;
+            g.out_im[o] = ar * im + ai * re;
+          }
+        }
+      }
+    }
+}
+
 int64_t ozgemm_max_k(int S) {
   // S pairs share the largest group; |d| <= 64 for every digit
   const int64_t lim = ((int64_t)1 << 31) - 1;
@@ -538,11 +682,11 @@ void launch_ozgemm(const OzGemmArgs& g, hipStream_t s) {
   if (g.M <= 0 || g.N <= 0) return;
   const int bn = g.S <= 4 ? 128 : 64;
   const int tiles = ((g.M + OZ_BM - 1) / OZ_BM) * ((g.N + bn - 1) / bn);
-  const dim3 grid(tiles, g.k_chunk > 0 ? g.splits : 1), block(256);
+  const dim3 grid(tiles, g.k_chunk > 0 ? g.splits : 1), block(256), block8(512);
   switch (g.S) {
-    case 2: hipLaunchKernelGGL((ozgemm_kernel<2, 4>), grid, block, 0, s, g); break;
-    case 3: hipLaunchKernelGGL((ozgemm_kernel<3, 4>), grid, block, 0, s, g); break;
-    case 4: hipLaunchKernelGGL((ozgemm_kernel<4, 4>), grid, block, 0, s, g); break;
+    case 2: hipLaunchKernelGGL((ozgemm8_kernel<2>), grid, block8, 0, s, g); break;
+    case 3: hipLaunchKernelGGL((ozgemm8_kernel<3>), grid, block8, 0, s, g); break;
+    case 4: hipLaunchKernelGGL((ozgemm8_kernel<4>), grid, block8, 0, s, g); break;
     case 5: hipLaunchKernelGGL((ozgemm_kernel<5, 2>), grid, block, 0, s, g); break;
     case 6: hipLaunchKernelGGL((ozgemm_kernel<6, 2>), grid, block, 0, s, g); break;
     default: break;
