@@ -42,6 +42,7 @@ struct LadderArgs {
   const double* vp = nullptr;
   int o = 0;
   bool ij_pairs = false;  // rows are the pairs i < j
+  int splits = 1;         // K chunks of the digit GEMM (partial results summed before the unpack)
   int64_t rows = 0;
   int v = 0;
   int64_t P = 0;
@@ -648,26 +649,44 @@ static void inputs_of(const Node& n, std::vector<int>& in) {
   }
 }
 
-// Ladder scratch: packed input up and result rp (2 rows x P doubles each;
-// rows = o^2, or o (o - 1) / 2 with ij pairs), then, in split precision, the
-// digit planes of up, its row scales and the split workspace.
+// Ladder scratch: packed input up (2 rows x P doubles; rows = o^2, or
+// o (o - 1) / 2 with ij pairs) and the result rp (one 2 rows x P partial per
+// K chunk of the digit GEMM), then, in split precision, the digit planes of
+// up, its row scales and the split workspace.
 static int64_t al256(int64_t x) { return (x + 255) / 256 * 256; }
 static int64_t ladder_rows(int o, bool pr) { return pr ? (int64_t)o * (o - 1) / 2 : (int64_t)o * o; }
 static int64_t ladder_rows2(int o, bool pr) { return 2 * ladder_rows(o, pr); }
 static int64_t ladder_P(int v) { return (int64_t)v * (v - 1) / 2; }
 static int64_t ladder_Kp(int v) { return (ladder_P(v) + 63) / 64 * 64; }
 static int64_t ladder_plane(int o, bool pr, int v) { return al256(ladder_rows2(o, pr) * ladder_Kp(v)); }
-static int64_t ladder_digits_off(int o, bool pr, int v) {
-  return al256(2 * ladder_rows2(o, pr) * ladder_P(v) * (int64_t)sizeof(double));
+// K chunk of the digit GEMM (0: one chunk): a grid of fewer than two rounds
+// of 128 x 128 tiles over the 256 CUs is cut along K so the last round is
+// not mostly idle; chunks stay within the exact int32 range.
+static int64_t ladder_k_chunk(int o, bool pr, int v, int S) {
+  if (S <= 0) return 0;
+  const int64_t Kp = ladder_Kp(v), steps = Kp / 64;
+  const int64_t tiles = ((ladder_rows2(o, pr) + 127) / 128) * ((ladder_P(v) + 127) / 128);
+  int64_t sp = tiles >= 512 ? 1 : std::min<int64_t>(8, (1280 + tiles - 1) / tiles);
+  sp = std::max<int64_t>(1, std::min<int64_t>(sp, steps / 2));
+  int64_t chunk = (steps + sp - 1) / sp * 64;
+  chunk = std::min<int64_t>(chunk, ozgemm_max_k(S));
+  return chunk >= Kp ? 0 : chunk;
+}
+static int64_t ladder_splits(int o, bool pr, int v, int S) {
+  const int64_t c = ladder_k_chunk(o, pr, v, S);
+  return c > 0 ? (ladder_Kp(v) + c - 1) / c : 1;
+}
+static int64_t ladder_digits_off(int o, bool pr, int v, int S) {
+  return al256((1 + ladder_splits(o, pr, v, S)) * ladder_rows2(o, pr) * ladder_P(v) * (int64_t)sizeof(double));
 }
 static int64_t ladder_scale_off(int o, bool pr, int v, int S) {
-  return ladder_digits_off(o, pr, v) + S * ladder_plane(o, pr, v);
+  return ladder_digits_off(o, pr, v, S) + S * ladder_plane(o, pr, v);
 }
 static int64_t ladder_pmax_off(int o, bool pr, int v, int S) {
   return ladder_scale_off(o, pr, v, S) + al256(ladder_rows2(o, pr) * (int64_t)sizeof(double));
 }
 static int64_t ladder_scratch_bytes(int o, bool pr, int v, int S) {
-  if (S <= 0) return ladder_digits_off(o, pr, v);
+  if (S <= 0) return ladder_digits_off(o, pr, v, S);
   return ladder_pmax_off(o, pr, v, S) + al256(ozaki_split_workspace(ladder_rows2(o, pr), ladder_Kp(v), false));
 }
 
@@ -1427,7 +1446,7 @@ void Graph::bind(double* arena) {
           su.K = a.P;
           su.Kp = Kp;
           su.S = a.S;
-          su.D = reinterpret_cast<int8_t*>(scr + ladder_digits_off(o_, a.ij_pairs, v_));
+          su.D = reinterpret_cast<int8_t*>(scr + ladder_digits_off(o_, a.ij_pairs, v_, a.S));
           su.plane = plane;
           su.scale = reinterpret_cast<double*>(scr + ladder_scale_off(o_, a.ij_pairs, v_, a.S));
           su.pmax = reinterpret_cast<double*>(scr + ladder_pmax_off(o_, a.ij_pairs, v_, a.S));
@@ -1445,10 +1464,12 @@ void Graph::bind(double* arena) {
           oz.N = (int)a.P;
           oz.Kp = (int)Kp;
           oz.S = a.S;
-          if (Kp > ozgemm_max_k(a.S)) throw Error("ladder K beyond the exact int32 range of the digit GEMM");
-          oz.k_chunk = 0;
-          oz.splits = 1;
-          oz.c_split_stride = 0;
+          const int64_t chunk = ladder_k_chunk(o_, a.ij_pairs, v_, a.S);
+          if (chunk == 0 && Kp > ozgemm_max_k(a.S)) throw Error("ladder K beyond the exact int32 range of the digit GEMM");
+          oz.k_chunk = (int)chunk;
+          oz.splits = (int)ladder_splits(o_, a.ij_pairs, v_, a.S);
+          oz.c_split_stride = 2 * a.rows * a.P;
+          a.splits = oz.splits;
         }
         launches_.push_back(std::move(L));
         break;
@@ -1618,6 +1639,7 @@ static void run_ladder(const LadderArgs& a, hipStream_t s) {
   } else {
     launch_dgemm(ladder_gemm(a), true, s);
   }
+  launch_sum_splits(a.rp, a.splits, 2 * a.rows * a.P, 2 * a.rows * a.P, s);
   launch_unpack_pairs(a.rp, a.out_re, a.out_im, a.o, a.v, 1.0, a.ij_pairs, s);
 }
 
@@ -1784,6 +1806,9 @@ void Graph::run_profiled(hipStream_t s, std::vector<LaunchRecord>& out) const {
             detail = dt;
             timed("dgemm_kernel<BT>", 2.0 * g.M * (double)g.N * g.K, 0.0, [&] { launch_dgemm(g, true, s); });
           }
+          if (a.splits > 1)
+            timed("sum_splits_kernel", 0.0, 8.0 * packed * (a.splits + 1),
+                  [&] { launch_sum_splits(a.rp, a.splits, 2 * a.rows * a.P, 2 * a.rows * a.P, s); });
           timed("unpack_pairs_kernel", 0.0, 8.0 * (elems + packed),
                 [&] { launch_unpack_pairs(a.rp, a.out_re, a.out_im, a.o, a.v, 1.0, a.ij_pairs, s); });
           break;

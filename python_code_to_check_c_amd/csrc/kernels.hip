@@ -871,6 +871,19 @@ void launch_unpack_pairs(const double* Y, double* yr, double* yi, int o, int v, 
                      (int)ij_pairs);
 }
 
+__global__ void sum_splits_kernel(double* y, int splits, int64_t stride, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double acc = y[i];
+    for (int c = 1; c < splits; ++c) acc = __dadd_rn(acc, y[c * stride + i]);
+    y[i] = acc;
+  }
+}
+
+void launch_sum_splits(double* y, int splits, int64_t stride, int64_t n, hipStream_t s) {
+  if (splits <= 1 || n <= 0) return;
+  hipLaunchKernelGGL(sum_splits_kernel, dim3(grid_for(n)), dim3(256), 0, s, y, splits, stride, n);
+}
+
 __global__ void build_vpack_kernel(const double* tei, int o, int v, int ket, double* Vp) {
   const int64_t n = o + v, P = (int64_t)v * (v - 1) / 2, vv = (int64_t)v * v, total = vv * vv;
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;

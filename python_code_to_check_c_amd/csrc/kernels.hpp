@@ -153,6 +153,8 @@ void launch_pack_pairs(const double* x_re, const double* x_im, double* X, int o,
                        hipStream_t s);
 void launch_unpack_pairs(const double* Y, double* y_re, double* y_im, int o, int v, double alpha, bool ij_pairs,
                          hipStream_t s);
+// y[i] = y[i] + y[stride + i] + ... over `splits` partial results, in order.
+void launch_sum_splits(double* y, int splits, int64_t stride, int64_t n, hipStream_t s);
 // Packed <ab||ef> of the virtual block of a (n,n,n,n) TEI with o occupied:
 //   ladder_ket == 0: Vp[q(a,b)][q(e,f)] = <ab||ef>
 //   ladder_ket == 1: Vp[q(a,b)][q(e,f)] = <ef||ab>
