@@ -417,7 +417,7 @@ def pmc_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, copy_mib):
 
 def traffic_of(kernel_name):
     """HBM bytes per launch of a kernel from the committed counter table (the
-    base name, e.g. ozgemm_kernel, matches tdcc::ozgemm_kernel<4, 4>), or None."""
+    base name, e.g. ozgemm8_kernel, matches tdcc::ozgemm8_kernel<4>), or None."""
     if not os.path.exists(TRAFFIC_TABLE):
         return None
     base = kernel_name.split("<")[0]

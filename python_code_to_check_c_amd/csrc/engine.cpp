@@ -1732,7 +1732,7 @@ void Graph::run_profiled(hipStream_t s, std::vector<LaunchRecord>& out) const {
             std::snprintf(dt, sizeof dt, "%s rows=%d N=%d K=%d splits=%d digits=%d", nodes_[L->node].spec.c_str(),
                           L->oz.M, L->oz.N, L->g.K, gp.splits, gp.S);
             detail = dt;
-            timed("ozgemm_kernel", fl, 0.0, [&] { launch_ozgemm(L->oz, s); });
+            timed(ozgemm_kernel_name(L->oz.S), fl, 0.0, [&] { launch_ozgemm(L->oz, s); });
             const double sbytes = 8.0 * L->sc.M * (double)L->sc.N * (L->sc.kind == 2 ? 4 : 2) * L->sc.splits;
             if (!gp.fused) timed("gemm_scatter_kernel", 0.0, sbytes, [&] { launch_gemm_scatter(L->sc, s); });
             break;
@@ -1800,7 +1800,7 @@ void Graph::run_profiled(hipStream_t s, std::vector<LaunchRecord>& out) const {
                   [&] { launch_ozaki_split(a.su, false, s); });
             std::snprintf(dt, sizeof dt, "packed ladder M=%d N=%d K=%d digits=%d", g.M, g.N, g.K, a.S);
             detail = dt;
-            timed("ozgemm_kernel", 2.0 * g.M * (double)g.N * g.K, 0.0, [&] { launch_ozgemm(a.oz, s); });
+            timed(ozgemm_kernel_name(a.oz.S), 2.0 * g.M * (double)g.N * g.K, 0.0, [&] { launch_ozgemm(a.oz, s); });
           } else {
             std::snprintf(dt, sizeof dt, "packed ladder M=%d N=%d K=%d", g.M, g.N, g.K);
             detail = dt;

@@ -251,6 +251,8 @@ struct OzGemmArgs {
   double alpha_re = 1.0, alpha_im = 0.0;
 };
 void launch_ozgemm(const OzGemmArgs& a, hipStream_t s);
+// Name of the kernel launch_ozgemm runs for S digits (profiles).
+inline const char* ozgemm_kernel_name(int S) { return S <= 4 ? "ozgemm8_kernel" : "ozgemm_kernel"; }
 // Largest K range one int32 accumulator may sum exactly for S digits.
 int64_t ozgemm_max_k(int S);
 
