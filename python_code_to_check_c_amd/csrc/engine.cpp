@@ -92,7 +92,7 @@ struct Graph::Launch {
   SkinnyArgs sk{};
   GettArgs g{};
   bool ac = false, bc = false, akc = false, bkc = false, c_cplx = false;
-  int64_t tab[9] = {0, 0, 0, 0, 0, 0, -1, -1, -1};  // am ak bn bk cm cn ab bb cb offsets into tables
+  int64_t tab[11] = {0, 0, 0, 0, 0, 0, -1, -1, -1, -1, -1};  // am ak bn bk cm cn ab bb cb cm2 cn2 offsets into tables
   LincombArgs l{};
   FieldArgs f{};
   FinalizeArgs fin{};
@@ -245,6 +245,13 @@ static bool split_multipass() {
 // Skinny contractions with at least this much big-operand work (rows x K) take
 // the VALU streaming kernel; TDCC_SKINNY_MIN overrides it (0: all skinny
 // shapes, a negative value: none).
+// Elements of an integral operand above which a skinny contraction runs on
+// the split GEMM path (TDCC_STATIC_STREAM_MIN; negative: never).
+static int64_t static_stream_min() {
+  const char* e = std::getenv("TDCC_STATIC_STREAM_MIN");
+  return e ? std::atoll(e) : ((int64_t)1 << 23);
+}
+
 static double skinny_min_work() {
   const char* e = std::getenv("TDCC_SKINNY_MIN");
   const double v = e ? std::atof(e) : 65536.0;
@@ -847,10 +854,12 @@ Dev Graph::resolve(int id) const {
 // Offset tables of one index group: row-major enumeration of `letters` with
 // the given strides.
 // Offsets of every combination of the letters (first letter slowest); with
-// a pair "pq" of them, only the combinations with index(p) < index(q).
+// a pair "pq" of them, only the combinations with index(p) < index(q), or
+// index(p) <= index(q) with `diag`.  With `mirror` the offsets written are
+// those of the same combinations with p and q exchanged (-1 where equal).
 static void group_table(const std::string& letters, const std::map<char, int64_t>& ext,
                         const std::map<char, int64_t>& st, std::vector<int64_t>& out,
-                        const std::string& pair = "") {
+                        const std::string& pair = "", bool diag = false, bool mirror = false) {
   int64_t total = 1;
   for (char c : letters) total *= ext.at(c);
   const int dp = pair.empty() ? -1 : (int)letters.find(pair[0]);
@@ -867,7 +876,11 @@ static void group_table(const std::string& letters, const std::map<char, int64_t
       if (d == dp) ip = i;
       if (d == dq) iq = i;
     }
-    if (dp >= 0 && dq >= 0 && ip >= iq) continue;
+    if (dp >= 0 && dq >= 0 && (diag ? ip > iq : ip >= iq)) continue;
+    if (mirror) {
+      const int64_t sp = st.at(letters[dp]), sq = st.at(letters[dq]);
+      off = ip == iq ? -1 : off - ip * sp - iq * sq + iq * sp + ip * sq;
+    }
     out.push_back(off);
   }
 }
@@ -958,9 +971,18 @@ void Graph::lower_contract(int id) {
   // dot-product shapes (both free groups small, K long) also stream well
   // through the GEMM path's split-K
   const bool dot_shape = gsize(gm) >= 16 && gsize(gn) >= 16 && gsize(gk) >= 8192 && gflops >= 2.5e8;
+  // In split precision an integral block of at least 64 MB against a few
+  // amplitude rows also goes through the GEMM path: its digit planes are cut
+  // once per bind, so each step streams S bytes per element from the digit
+  // cache instead of 8 through the skinny or strided kernels.
+  const int64_t stream_min = static_stream_min();
+  const bool static_stream =
+      rt_->split_digits > 0 && !n.fp64 && min_flops > 0.0 && stream_min >= 0 && gsize(gk) >= 64 &&
+      ((is_static(a) && gsize(gm) * gsize(gk) >= stream_min && gsize(gn) >= 4) ||
+       (is_static(b) && gsize(gn) * gsize(gk) >= stream_min && gsize(gm) >= 4));
   const bool use_gemm = gb.empty() && (min_flops == 0.0 ||
                                        (gflops >= min_flops && gsize(gm) >= 64 && gsize(gn) >= 64 && gsize(gk) >= 32) ||
-                                       (dot_shape && min_flops <= 2e9));
+                                       (dot_shape && min_flops <= 2e9) || static_stream);
   if (use_gemm && !nodes_[a].cplx && nodes_[b].cplx) {
     std::swap(la, lb);
     std::swap(a, b);
@@ -988,6 +1010,23 @@ void Graph::lower_contract(int id) {
   };
   gm = akc ? gm : by_stride(gm, sa);
   gn = bkc ? gn : by_stride(gn, sb);
+  // On the GEMM path an output antisymmetric in two letters of the M (or N)
+  // group is computed for their ordered combinations (diagonal included) and
+  // the epilogue stores each element's mirror images as well.
+  std::string& mpair = n.mpair;
+  std::string& npair = n.npair;
+  mpair.clear();
+  npair.clear();
+  if (k_pairs_ && use_gemm) {
+    auto find_pair = [&](const std::string& grp, std::string& pr) {
+      for (size_t x = 0; x < grp.size() && pr.empty(); ++x)
+        for (size_t y = x + 1; y < grp.size() && pr.empty(); ++y)
+          if (ext[grp[x]] == ext[grp[y]] && (n.anti & anti_bit((int)lc.find(grp[x]), (int)lc.find(grp[y]))))
+            pr = std::string{grp[x], grp[y]};
+    };
+    find_pair(gm, mpair);
+    find_pair(gn, npair);
+  }
   gk = akc ? by_stride(gk, sa) : (bkc ? by_stride(gk, sb) : by_stride(gk, sa));
 
   auto push = [&](const std::vector<int64_t>& t) {
@@ -998,21 +1037,29 @@ void Graph::lower_contract(int id) {
   std::vector<int64_t> t;
   auto L = std::make_unique<Launch>();
   L->kind = Launch::GETT;
-  group_table(gm, ext, sa, t);
+  group_table(gm, ext, sa, t, mpair, true);
   L->tab[0] = push(t);
   const int64_t M = (int64_t)t.size();
   group_table(gk, ext, sa, t, n.kpair);
   L->tab[1] = push(t);
   const int64_t K = (int64_t)t.size();
-  group_table(gn, ext, sb, t);
+  group_table(gn, ext, sb, t, npair, true);
   L->tab[2] = push(t);
   const int64_t Nn = (int64_t)t.size();
   group_table(gk, ext, sb, t, n.kpair);
   L->tab[3] = push(t);
-  group_table(gm, ext, sc, t);
+  group_table(gm, ext, sc, t, mpair, true);
   L->tab[4] = push(t);
-  group_table(gn, ext, sc, t);
+  group_table(gn, ext, sc, t, npair, true);
   L->tab[5] = push(t);
+  if (!mpair.empty()) {
+    group_table(gm, ext, sc, t, mpair, true, true);
+    L->tab[9] = push(t);
+  }
+  if (!npair.empty()) {
+    group_table(gn, ext, sc, t, npair, true, true);
+    L->tab[10] = push(t);
+  }
   int64_t Bt = 1;
   if (!gb.empty()) {
     group_table(gb, ext, sa, t);
@@ -1120,10 +1167,12 @@ void Graph::lower_contract(int id) {
       gp.zb_off = gp.pb_off + al256(ozaki_split_workspace(NB, gp.Kp, gp.b_cols));
       gp.fused = sp == 1;
       n.scratch_len = gp.zb_off + (gp.fused ? 0 : al256(8 * sp * R * NB));
-      auto cache_entry = [&](int x, const std::string& rows, bool cols, int64_t nrows, int64_t plane) {
+      auto cache_entry = [&](int x, const std::string& rows, const std::string& rpair, bool cols, int64_t nrows,
+                             int64_t plane) {
         if (!is_static(x)) return -1;
-        const std::string key = nodes_[x].key + "|" + rows + "|" + gk + "|" + n.kpair + "|" + (cols ? "c" : "r") +
-                                "|" + std::to_string(nrows) + "|" + std::to_string(S);
+        const std::string key = nodes_[x].key + "|" + rows + "|" + rpair + "|" + gk + "|" +
+                                n.kpair + "|" + (cols ? "c" : "r") + "|" + std::to_string(nrows) + "|" +
+                                std::to_string(S);
         auto it = cache_index_.find(key);
         if (it != cache_index_.end()) return it->second;
         DigitCache e;
@@ -1135,8 +1184,8 @@ void Graph::lower_contract(int id) {
         cache_index_[key] = idx;
         return idx;
       };
-      gp.a_cache = cache_entry(a, gm, gp.a_cols, R, gp.a_plane);
-      gp.b_cache = cache_entry(b, gn, gp.b_cols, NB, gp.b_plane);
+      gp.a_cache = cache_entry(a, gm, mpair, gp.a_cols, R, gp.a_plane);
+      gp.b_cache = cache_entry(b, gn, npair, gp.b_cols, NB, gp.b_plane);
       L->node = id;
       launches_.push_back(std::move(L));
       return;
@@ -1287,6 +1336,8 @@ void Graph::bind(double* arena) {
             oz.epi = (L->ac && L->bc) ? 2 : (L->ac ? 1 : 0);
             oz.cm = g.cm;
             oz.cn = g.cn;
+            oz.cm2 = L->tab[9] >= 0 ? dev_tables_ + L->tab[9] : nullptr;
+            oz.cn2 = L->tab[10] >= 0 ? dev_tables_ + L->tab[10] : nullptr;
             oz.out_re = C.re;
             oz.out_im = n.cplx ? C.im : nullptr;
             oz.alpha_re = n.ar * n.kfac;
@@ -1304,6 +1355,8 @@ void Graph::bind(double* arena) {
           if (!L->ac && L->bc) throw Error("internal: complex operand not on the row side");
           sc.cm = g.cm;
           sc.cn = g.cn;
+          sc.cm2 = L->tab[9] >= 0 ? dev_tables_ + L->tab[9] : nullptr;
+          sc.cn2 = L->tab[10] >= 0 ? dev_tables_ + L->tab[10] : nullptr;
           sc.out_re = C.re;
           sc.out_im = n.cplx ? C.im : nullptr;
           sc.alpha_re = n.ar * n.kfac;
@@ -1370,6 +1423,8 @@ void Graph::bind(double* arena) {
           if (!L->ac && L->bc) throw Error("internal: complex operand not on the row side");
           sc.cm = g.cm;
           sc.cn = g.cn;
+          sc.cm2 = L->tab[9] >= 0 ? dev_tables_ + L->tab[9] : nullptr;
+          sc.cn2 = L->tab[10] >= 0 ? dev_tables_ + L->tab[10] : nullptr;
           sc.out_re = C.re;
           sc.out_im = n.cplx ? C.im : nullptr;
           sc.alpha_re = n.ar * n.kfac;

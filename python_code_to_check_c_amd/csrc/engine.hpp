@@ -85,6 +85,10 @@ struct Node {
   // summed (empty: all), and the factor the launch applies then
   std::string kpair;
   double kfac = 1.0;
+  // Contract: letters of an antisymmetric output pair among the rows (M) and
+  // among the columns (N) of a GEMM-path launch, computed for index(p) <=
+  // index(q) only and mirrored by the epilogue (empty: none)
+  std::string mpair, npair;
   // antisymmetric axis pairs: bit anti_bit(d1, d2) set when the tensor
   // changes sign under exchanging axes d1 and d2
   uint8_t anti = 0;

@@ -191,8 +191,32 @@ struct GemmScatterArgs {
   double* out_re; double* out_im;
   double alpha_re, alpha_im;
   int accumulate;
+  // output antisymmetric in a pair of row (column) letters and computed on the
+  // ordered rows (columns) only: offsets of the mirrored row (column), -1 on
+  // the diagonal; nullptr without such a pair (never with accumulate)
+  const int64_t* cm2 = nullptr; const int64_t* cn2 = nullptr;
 };
 void launch_gemm_scatter(const GemmScatterArgs& g, hipStream_t s);
+
+// Store of one output element and its mirror images (see GemmScatterArgs):
+// y at (om, on), -y at the mirrored row or column, y at both.
+__device__ __forceinline__ void store_mirrored(double* re, double* im, int64_t om, int64_t on, int64_t om2,
+                                               int64_t on2, double yr, double yi) {
+  re[om + on] = yr;
+  if (im) im[om + on] = yi;
+  if (om2 >= 0) {
+    re[om2 + on] = -yr;
+    if (im) im[om2 + on] = -yi;
+  }
+  if (on2 >= 0) {
+    re[om + on2] = -yr;
+    if (im) im[om + on2] = -yi;
+    if (om2 >= 0) {
+      re[om2 + on2] = yr;
+      if (im) im[om2 + on2] = yi;
+    }
+  }
+}
 
 // ---------------------------------------------------------------------------
 // Split-precision GEMM on int8 MFMA (Ozaki scheme with exact accumulation).
@@ -247,10 +271,12 @@ struct OzGemmArgs {
   //   epi 2 (complex x complex): value = (Z[2m][2n] - Z[2m+1][2n+1]) + i (Z[2m][2n+1] + Z[2m+1][2n])
   int epi = -1;
   const int64_t* cm = nullptr; const int64_t* cn = nullptr;
+  const int64_t* cm2 = nullptr; const int64_t* cn2 = nullptr;  // mirrored rows / columns, as in GemmScatterArgs
   double* out_re = nullptr; double* out_im = nullptr;
   double alpha_re = 1.0, alpha_im = 0.0;
 };
 void launch_ozgemm(const OzGemmArgs& a, hipStream_t s);
+
 // Name of the kernel launch_ozgemm runs for S digits (profiles).
 inline const char* ozgemm_kernel_name(int S) { return S <= 4 ? "ozgemm8_kernel" : "ozgemm_kernel"; }
 // Largest K range one int32 accumulator may sum exactly for S digits.

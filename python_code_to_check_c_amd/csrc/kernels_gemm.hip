@@ -69,8 +69,7 @@ __global__ void gemm_scatter_kernel(GemmScatterArgs g) {
       yr += g.out_re[o];
       if (g.out_im) yi += g.out_im[o];
     }
-    g.out_re[o] = yr;
-    if (g.out_im) g.out_im[o] = yi;
+    store_mirrored(g.out_re, g.out_im, g.cm[m], g.cn[n], g.cm2 ? g.cm2[m] : -1, g.cn2 ? g.cn2[n] : -1, yr, yi);
   }
 }
 

@@ -499,15 +499,17 @@ __global__ __launch_bounds__(256) void ozgemm_kernel(OzGemmArgs g) {
       if (g.epi == 0) {
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          if (colok && row0 + r < g.M) g.out_re[g.cm[row0 + r] + g.cn[col]] = ar * z[r];
+          if (colok && row0 + r < g.M)
+            store_mirrored(g.out_re, nullptr, g.cm[row0 + r], g.cn[col], g.cm2 ? g.cm2[row0 + r] : -1,
+                           g.cn2 ? g.cn2[col] : -1, ar * z[r], 0.0);
       } else if (g.epi == 1) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           if (!colok || row0 + 2 * h >= g.M) continue;
           const double re = z[2 * h], im = z[2 * h + 1];
-          const int64_t o = g.cm[(row0 >> 1) + h] + g.cn[col];
-          g.out_re[o] = ar * re - ai * im;
-          g.out_im[o] = ar * im + ai * re;
+          const int mr = (row0 >> 1) + h;
+          store_mirrored(g.out_re, g.out_im, g.cm[mr], g.cn[col], g.cm2 ? g.cm2[mr] : -1, g.cn2 ? g.cn2[col] : -1,
+                         ar * re - ai * im, ar * im + ai * re);
         }
       } else {
         double w[4];
@@ -519,9 +521,9 @@ __global__ __launch_bounds__(256) void ozgemm_kernel(OzGemmArgs g) {
             if (row0 + 2 * h >= g.M) continue;
             const double re = z[2 * h] - w[2 * h + 1];   // rr - ii
             const double im = w[2 * h] + z[2 * h + 1];   // ri + ir
-            const int64_t o = g.cm[(row0 >> 1) + h] + g.cn[col >> 1];
-            g.out_re[o] = ar * re - ai * im;
-            g.out_im[o] = ar * im + ai * re;
+            const int mr = (row0 >> 1) + h, nc = col >> 1;
+            store_mirrored(g.out_re, g.out_im, g.cm[mr], g.cn[nc], g.cm2 ? g.cm2[mr] : -1, g.cn2 ? g.cn2[nc] : -1,
+                           ar * re - ai * im, ar * im + ai * re);
           }
         }
       }
@@ -643,15 +645,17 @@ __global__ __launch_bounds__(512, 1) void ozgemm8_kernel(OzGemmArgs g) {
       } else if (g.epi == 0) {
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          if (colok && row0 + r < g.M) g.out_re[g.cm[row0 + r] + g.cn[col]] = ar * z[r];
+          if (colok && row0 + r < g.M)
+            store_mirrored(g.out_re, nullptr, g.cm[row0 + r], g.cn[col], g.cm2 ? g.cm2[row0 + r] : -1,
+                           g.cn2 ? g.cn2[col] : -1, ar * z[r], 0.0);
       } else if (g.epi == 1) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           if (!colok || row0 + 2 * h >= g.M) continue;
           const double re = z[2 * h], im = z[2 * h + 1];
-          const int64_t o = g.cm[(row0 >> 1) + h] + g.cn[col];
-          g.out_re[o] = ar * re - ai * im;
-          g.out_im[o] = ar * im + ai * re;
+          const int mr = (row0 >> 1) + h;
+          store_mirrored(g.out_re, g.out_im, g.cm[mr], g.cn[col], g.cm2 ? g.cm2[mr] : -1, g.cn2 ? g.cn2[col] : -1,
+                         ar * re - ai * im, ar * im + ai * re);
         }
       } else {
         double x[4];
@@ -663,9 +667,9 @@ __global__ __launch_bounds__(512, 1) void ozgemm8_kernel(OzGemmArgs g) {
             if (row0 + 2 * h >= g.M) continue;
             const double re = z[2 * h] - x[2 * h + 1];  // rr - ii
             const double im = x[2 * h] + z[2 * h + 1];  // ri + ir
-            const int64_t o = g.cm[(row0 >> 1) + h] + g.cn[col >> 1];
-            g.out_re[o] = ar * re - ai * im;
-            g.out_im[o] = ar * im + ai * re;
+            const int mr = (row0 >> 1) + h, nc = col >> 1;
+            store_mirrored(g.out_re, g.out_im, g.cm[mr], g.cn[nc], g.cm2 ? g.cm2[mr] : -1, g.cn2 ? g.cn2[nc] : -1,
+                           ar * re - ai * im, ar * im + ai * re);
           }
         }
       }

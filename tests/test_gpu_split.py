@@ -32,9 +32,12 @@ def bound(S):
 
 # The digit split has single-pass kernels (rows held in registers) and
 # three-pass kernels (partial maxima, scales, digits); "multipass" forces the
-# latter everywhere.
+# latter everywhere.  "stream" keeps the default routing but sends every
+# skinny contraction on an integral block through the split GEMM path (at
+# large sizes only the blocks above 8M elements go there).
 MODES = {"default": {"TDCC_GEMM_MIN_FLOPS": "0"},
-         "multipass": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_SPLIT_MULTIPASS": "1"}}
+         "multipass": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_SPLIT_MULTIPASS": "1"},
+         "stream": {"TDCC_STATIC_STREAM_MIN": "0"}}
 
 
 @pytest.fixture(scope="module", params=[(n, m) for m in MODES for n in ("o4v6", "c1_o10v4")],
@@ -43,7 +46,7 @@ def case(request):
     name, mode = request.param
     g = dict(np.load(os.path.join(HERE, "golden", f"golden_{name}.npz")))
     print(f"\n[split case {name} {mode}]", flush=True)
-    keys = ("TDCC_GEMM_MIN_FLOPS", "TDCC_SPLIT_MULTIPASS")
+    keys = ("TDCC_GEMM_MIN_FLOPS", "TDCC_SPLIT_MULTIPASS", "TDCC_STATIC_STREAM_MIN")
     saved = {k: os.environ.get(k) for k in keys}
     for k in keys:
         os.environ.pop(k, None)
