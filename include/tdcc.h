@@ -150,6 +150,16 @@ int tdcc_checkpoint(tdcc_ctx* ctx, int op);
  * arena size in bytes. */
 int tdcc_step_info(tdcc_ctx* ctx, int method, double dt, int64_t* flops, int64_t* arena_bytes);
 
+/* Restart block writer, replacing write_2data / write_4data
+ * (CCSD_Helper.py:1107-1125): x (rank 2 or 4, C order, float64) as one text
+ * row per element, "i\ta\tvalue\n" or "i\tj\ta\tb\tvalue\n", value =
+ * numpy.around(x, precs) printed as Python's csv module prints a numpy
+ * float64.  Byte-identical to the reference's files; no context, no device.
+ * Status 0 = ok; tdcc_io_error() gives the message of the last failure on
+ * the calling thread. */
+int tdcc_write_restart_block(const double* x, int rank, const int64_t* dims, int precs, const char* path);
+const char* tdcc_io_error(void);
+
 #ifdef __cplusplus
 }
 #endif

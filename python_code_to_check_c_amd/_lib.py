@@ -18,7 +18,7 @@ EXPORTS = (
     "tdcc_set_precision", "tdcc_set_state", "tdcc_get_state", "tdcc_rhs", "tdcc_intermediate", "tdcc_dipole",
     "tdcc_rk4_run", "tdcc_profile_step", "tdcc_checkpoint", "tdcc_step_info",
     "tdcc_set_batch", "tdcc_set_state_slot", "tdcc_get_state_slot", "tdcc_rk4_run_batch",
-    "tdcc_dipole_gradient",
+    "tdcc_dipole_gradient", "tdcc_write_restart_block", "tdcc_io_error",
 )
 
 RHS_T1, RHS_T2, RHS_L1, RHS_L2, RHS_T2_CC2, RHS_L1_CC2, RHS_L2_CC2 = range(7)
@@ -71,6 +71,8 @@ def load():
         "tdcc_get_state_slot": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]),
         "tdcc_rk4_run_batch": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                               ctypes.POINTER(ctypes.c_int), _dp, _dp, ctypes.c_int, _dp, _dp]),
+        "tdcc_write_restart_block": (ctypes.c_int, [_dp, ctypes.c_int, i64p, ctypes.c_int, ctypes.c_char_p]),
+        "tdcc_io_error": (ctypes.c_char_p, []),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -292,3 +294,15 @@ class Context:
         self._check(self.lib.tdcc_step_info(self._h, m, float(dt), ctypes.byref(f), ctypes.byref(a)),
                     "tdcc_step_info")
         return f.value, a.value
+
+
+def write_restart_block(x, path, precs):
+    """One restart block file (rank 2 or 4, real) through the C++ writer."""
+    lib = load()
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+    if a.ndim not in (2, 4):
+        raise ValueError("restart blocks are rank 2 or 4")
+    dims = (ctypes.c_int64 * a.ndim)(*a.shape)
+    rc = lib.tdcc_write_restart_block(a.ctypes.data_as(_dp), a.ndim, dims, int(precs), os.fsencode(path))
+    if rc:
+        raise TdccError(f"tdcc_write_restart_block: {lib.tdcc_io_error().decode()}")

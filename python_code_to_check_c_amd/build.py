@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(CSRC, "_build")
 LIB = os.path.join(HERE, "libtdcc_hip.so")
-SOURCES = ["kernels.hip", "kernels_gemm.hip", "kernels_split.hip", "kernels_skinny.hip", "kernels_adjoint.hip", "engine.cpp", "cc.cpp", "capi.cpp"]
+SOURCES = ["kernels.hip", "kernels_gemm.hip", "kernels_split.hip", "kernels_skinny.hip", "kernels_adjoint.hip", "engine.cpp", "cc.cpp", "capi.cpp", "restart_io.cpp"]
 ARCH = os.environ.get("TDCC_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

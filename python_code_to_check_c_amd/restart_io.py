@@ -11,41 +11,24 @@ Writers follow CCSD_Helper.py:1102-1145, readers CC_Calculator.py:70-135:
 """
 from __future__ import annotations
 
-import csv
 import os
 
 import numpy as np
 import pandas as pd
 
-
-def _writer(f):
-    return csv.writer(f, delimiter="\t", quotechar="|", quoting=csv.QUOTE_MINIMAL, lineterminator="\n")
+from . import _lib
 
 
 def write_2data(F, FileName, precs):
-    """CCSD_Helper.py:1107-1114"""
-    F = np.asarray(F)
-    r = np.around(F, decimals=precs)
-    with open(FileName, "w") as f:
-        w = _writer(f)
-        for i in range(F.shape[0]):
-            for a in range(F.shape[1]):
-                w.writerow([i, a, r[i, a]])
+    """CCSD_Helper.py:1107-1114 (rows i, a, numpy.around(value, precs)),
+    written by the C++ writer of the library."""
+    _lib.write_restart_block(np.asarray(F, dtype=np.float64), FileName, precs)
 
 
 def write_4data(F, FileName, precs):
-    """CCSD_Helper.py:1116-1125"""
-    F = np.asarray(F)
-    r = np.around(F, decimals=precs)
-    o, _, v, _ = F.shape
-    with open(FileName, "w") as f:
-        w = _writer(f)
-        for i in range(o):
-            for j in range(F.shape[1]):
-                for a in range(v):
-                    row = r[i, j, a]
-                    for b in range(F.shape[3]):
-                        w.writerow([i, j, a, b, row[b]])
+    """CCSD_Helper.py:1116-1125 (rows i, j, a, b, numpy.around(value, precs)),
+    written by the C++ writer of the library."""
+    _lib.write_restart_block(np.asarray(F, dtype=np.float64), FileName, precs)
 
 
 def save_parameters(w0, A, t, dt, precs, i, a, method, path="."):
