@@ -42,7 +42,8 @@ struct LadderArgs {
   const double* vp = nullptr;
   int o = 0;
   bool ij_pairs = false;  // rows are the pairs i < j
-  int splits = 1;         // K chunks of the digit GEMM (partial results summed before the unpack)
+  int splits = 1;         // K chunks of the GEMM (partial results summed before the unpack)
+  int64_t k_chunk = 0;
   int64_t rows = 0;
   int v = 0;
   int64_t P = 0;
@@ -666,22 +667,23 @@ static int64_t ladder_rows2(int o, bool pr) { return 2 * ladder_rows(o, pr); }
 static int64_t ladder_P(int v) { return (int64_t)v * (v - 1) / 2; }
 static int64_t ladder_Kp(int v) { return (ladder_P(v) + 63) / 64 * 64; }
 static int64_t ladder_plane(int o, bool pr, int v) { return al256(ladder_rows2(o, pr) * ladder_Kp(v)); }
-// K chunk of the digit GEMM (0: one chunk): a grid of fewer than two rounds
+// K chunk of the ladder GEMM (0: one chunk): a grid of fewer than two rounds
 // of 128 x 128 tiles over the 256 CUs is cut along K so the last round is
-// not mostly idle; chunks stay within the exact int32 range.
+// not mostly idle.  Digit GEMM: multiples of 64 within the exact int32
+// range; FP64 GEMM: multiples of 16 of K = P.
 static int64_t ladder_k_chunk(int o, bool pr, int v, int S) {
-  if (S <= 0) return 0;
-  const int64_t Kp = ladder_Kp(v), steps = Kp / 64;
+  const int64_t gran = S > 0 ? 64 : 16;
+  const int64_t K = S > 0 ? ladder_Kp(v) : ladder_P(v), steps = K / gran;
   const int64_t tiles = ((ladder_rows2(o, pr) + 127) / 128) * ((ladder_P(v) + 127) / 128);
   int64_t sp = tiles >= 512 ? 1 : std::min<int64_t>(8, (1280 + tiles - 1) / tiles);
   sp = std::max<int64_t>(1, std::min<int64_t>(sp, steps / 2));
-  int64_t chunk = (steps + sp - 1) / sp * 64;
-  chunk = std::min<int64_t>(chunk, ozgemm_max_k(S));
-  return chunk >= Kp ? 0 : chunk;
+  int64_t chunk = (steps + sp - 1) / sp * gran;
+  if (S > 0) chunk = std::min<int64_t>(chunk, ozgemm_max_k(S));
+  return chunk >= K ? 0 : chunk;
 }
 static int64_t ladder_splits(int o, bool pr, int v, int S) {
   const int64_t c = ladder_k_chunk(o, pr, v, S);
-  return c > 0 ? (ladder_Kp(v) + c - 1) / c : 1;
+  return c > 0 ? ((S > 0 ? ladder_Kp(v) : ladder_P(v)) + c - 1) / c : 1;
 }
 static int64_t ladder_digits_off(int o, bool pr, int v, int S) {
   return al256((1 + ladder_splits(o, pr, v, S)) * ladder_rows2(o, pr) * ladder_P(v) * (int64_t)sizeof(double));
@@ -1486,6 +1488,8 @@ void Graph::bind(double* arena) {
         a.up = arena_ + n.scratch_off / (int64_t)sizeof(double);
         a.rp = a.up + 2 * a.rows * a.P;
         a.vp = rt_->vpack[n.stage];
+        a.k_chunk = ladder_k_chunk(o_, a.ij_pairs, v_, rt_->split_digits);
+        a.splits = (int)ladder_splits(o_, a.ij_pairs, v_, rt_->split_digits);
         if (!a.vp) throw Error("packed <ab||ef> not built");
         a.S = rt_->split_digits;
         if (a.S > 0) {
@@ -1683,6 +1687,9 @@ static DgemmArgs ladder_gemm(const LadderArgs& a) {
   g.lda = g.ldb = g.ldc = a.P;
   g.alpha = 1.0;
   g.beta = 0.0;
+  g.k_chunk = (int)a.k_chunk;
+  g.splits = a.S > 0 ? 1 : a.splits;
+  g.c_split_stride = 2 * a.rows * a.P;
   return g;
 }
 
