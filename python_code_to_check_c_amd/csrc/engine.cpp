@@ -243,9 +243,6 @@ static bool split_multipass() {
   return e && std::atoi(e) != 0;
 }
 
-// Skinny contractions with at least this much big-operand work (rows x K) take
-// the VALU streaming kernel; TDCC_SKINNY_MIN overrides it (0: all skinny
-// shapes, a negative value: none).
 // Elements of an integral operand above which a skinny contraction runs on
 // the split GEMM path (TDCC_STATIC_STREAM_MIN; negative: never).
 static int64_t static_stream_min() {
@@ -253,6 +250,25 @@ static int64_t static_stream_min() {
   return e ? std::atoll(e) : ((int64_t)1 << 23);
 }
 
+// Summed length below which a GEMM-path launch computes every column of an
+// output antisymmetric in two column letters instead of their ordered pairs:
+// with few K steps the launch is bound by its stores, and the mirror image of
+// a column pair lands one column stride apart per lane (TDCC_NPAIR_MIN_K).
+static int64_t npair_min_k() {
+  const char* e = std::getenv("TDCC_NPAIR_MIN_K");
+  return e ? std::atoll(e) : 1024;
+}
+
+// Antisymmetric linear combinations summed at their canonical elements and
+// mirrored (TDCC_LINCOMB_ANTI=0: every element summed).
+static bool lincomb_anti_on() {
+  const char* e = std::getenv("TDCC_LINCOMB_ANTI");
+  return !e || std::atoi(e) != 0;
+}
+
+// Skinny contractions with at least this much big-operand work (rows x K) take
+// the VALU streaming kernel; TDCC_SKINNY_MIN overrides it (0: all skinny
+// shapes, a negative value: none).
 static double skinny_min_work() {
   const char* e = std::getenv("TDCC_SKINNY_MIN");
   const double v = e ? std::atof(e) : 65536.0;
@@ -1027,7 +1043,7 @@ void Graph::lower_contract(int id) {
             pr = std::string{grp[x], grp[y]};
     };
     find_pair(gm, mpair);
-    find_pair(gn, npair);
+    if ((double)gsize(gk) * kpair_frac >= (double)npair_min_k()) find_pair(gn, npair);
   }
   gk = akc ? by_stride(gk, sa) : (bkc ? by_stride(gk, sb) : by_stride(gk, sa));
 
@@ -1456,6 +1472,11 @@ void Graph::bind(double* arena) {
           }
           l.accumulate = t0 > 0;
           l.nterms = (int)std::min<size_t>(kMaxTerms, terms.size() - t0);
+          // an antisymmetric result is summed at its canonical elements and
+          // mirrored (one launch only)
+          l.anti = 0;
+          if (n.op == Op::Lincomb && n.shape.rank == 4 && terms.size() <= (size_t)kMaxTerms && lincomb_anti_on())
+            l.anti = ((n.anti & anti_bit(0, 1)) ? 1 : 0) | ((n.anti & anti_bit(2, 3)) ? 2 : 0);
           for (int k = 0; k < l.nterms; ++k) {
             const Term& T = terms[t0 + k];
             const Dev S = resolve(T.src);
@@ -1835,7 +1856,20 @@ void Graph::run_profiled(hipStream_t s, std::vector<LaunchRecord>& out) const {
           break;
         }
         case Launch::LIN: {
-          const double bytes = 8.0 * L->l.total * ((L->l.out_im ? 2 : 1) * (1 + L->l.accumulate) + 2 * L->l.nterms);
+          // algorithmic bytes: the output (read too when accumulating) and
+          // every term at the elements the kernel sums (the canonical ones
+          // for an antisymmetric result)
+          const char* kname = lincomb_kernel_name(L->l);
+          double summed = (double)L->l.total;
+          if (std::strcmp(kname, "lincomb_anti_kernel") == 0) {
+            const double d0 = (double)L->l.dims[0], d1 = (double)L->l.dims[1], d3 = (double)L->l.dims[3];
+            summed = ((L->l.anti & 1) ? d0 * (d0 - 1) / 2 : d0 * d1) *
+                     ((L->l.anti & 2) ? d3 * (d3 + 1) / 2 : (double)L->l.dims[2] * d3);
+          }
+          double term_planes = 0.0;
+          for (int k = 0; k < L->l.nterms; ++k) term_planes += L->l.t[k].im ? 2.0 : 1.0;
+          const double bytes =
+              8.0 * ((double)L->l.total * (L->l.out_im ? 2 : 1) * (1 + L->l.accumulate) + summed * term_planes);
           char dt[160];
           int perm = 0;  // terms read through a non-identity permutation
           for (int k = 0; k < L->l.nterms; ++k)
@@ -1847,7 +1881,7 @@ void Graph::run_profiled(hipStream_t s, std::vector<LaunchRecord>& out) const {
           std::snprintf(dt, sizeof dt, "node %d terms=%d permuted=%d n=%lld acc=%d", L->node, L->l.nterms, perm,
                         (long long)L->l.total, L->l.accumulate);
           detail = dt;
-          timed("lincomb_kernel", 0.0, bytes, [&] { launch_lincomb(L->l, s); });
+          timed(kname, 0.0, bytes, [&] { launch_lincomb(L->l, s); });
           break;
         }
         case Launch::LADDER: {

@@ -352,42 +352,21 @@ __device__ __forceinline__ bool lincomb_transposed(const LincombTerm& T, int R) 
   return T.st[R - 1] != 1 && T.st[R - 2] == 1;
 }
 
-__global__ __launch_bounds__(256) void lincomb_tiled_kernel(LincombArgs a) {
-  __shared__ double tr[kLinTile][kLinTile + 1];
-  __shared__ double ti[kLinTile][kLinTile + 1];
+// Adds the terms of `a` over the 32 x 32 tile (r0.., c0..) of the two
+// trailing axes at the leading index whose source offsets are sbase[t] into
+// sr/si (element k of a thread: row r0 + ly + 8k, column c0 + lx), term by
+// term in order with lincomb_kernel's arithmetic; sr/si hold the output's
+// old value when a.accumulate.  tr/ti are the block's staging tiles.
+__device__ __forceinline__ void lincomb_tile_terms(const LincombArgs& a, const int64_t* lead, int nlead, int64_t r0,
+                                                   int64_t c0, int64_t D2, int64_t D3,
+                                                   double (*tr)[kLinTile + 1], double (*ti)[kLinTile + 1],
+                                                   double* sr, double* si) {
   const int R = a.rank;
-  const int64_t D2 = a.dims[R - 2], D3 = a.dims[R - 1];
-  const int64_t n2 = (D2 + kLinTile - 1) / kLinTile, n3 = (D3 + kLinTile - 1) / kLinTile;
-  const int64_t b = blockIdx.x;
-  const int64_t outer = b / (n2 * n3), tile = b - outer * (n2 * n3);
-  const int64_t r0 = (tile / n3) * kLinTile, c0 = (tile % n3) * kLinTile;
-  // outer index -> leading axes (rank 4: (i0, i1); rank 3: i0; rank 2: none)
-  int64_t lead[2] = {0, 0};
-  if (R == 4) {
-    lead[0] = outer / a.dims[1];
-    lead[1] = outer - lead[0] * a.dims[1];
-  } else if (R == 3) {
-    lead[0] = outer;
-  }
   const int lx = threadIdx.x & 31, ly = threadIdx.x >> 5;
-  int64_t obase = 0;
-  for (int d = 0; d < R - 2; ++d) obase += lead[d] * a.ost[d];
-  double sr[4], si[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    sr[k] = 0.0;
-    si[k] = 0.0;
-    const int64_t r = r0 + ly + 8 * k, c = c0 + lx;
-    if (a.accumulate && r < D2 && c < D3) {
-      const int64_t oo = obase + r * a.ost[R - 2] + c;
-      sr[k] = a.out_re[oo];
-      if (a.out_im) si[k] = a.out_im[oo];
-    }
-  }
   for (int t = 0; t < a.nterms; ++t) {
     const LincombTerm& T = a.t[t];
     int64_t sbase = 0;
-    for (int d = 0; d < R - 2; ++d) sbase += lead[d] * T.st[d];
+    for (int d = 0; d < nlead; ++d) sbase += lead[d] * T.st[d];
     double xr[4], xi[4];
     if (lincomb_transposed(T, R)) {
       // stage: thread (lx, ly) loads trailing-axis element (r0 + lx, c0 + ly + 8k)
@@ -435,6 +414,41 @@ __global__ __launch_bounds__(256) void lincomb_tiled_kernel(LincombArgs a) {
       }
     }
   }
+}
+
+__global__ __launch_bounds__(256) void lincomb_tiled_kernel(LincombArgs a) {
+  __shared__ double tr[kLinTile][kLinTile + 1];
+  __shared__ double ti[kLinTile][kLinTile + 1];
+  const int R = a.rank;
+  const int64_t D2 = a.dims[R - 2], D3 = a.dims[R - 1];
+  const int64_t n2 = (D2 + kLinTile - 1) / kLinTile, n3 = (D3 + kLinTile - 1) / kLinTile;
+  const int64_t b = blockIdx.x;
+  const int64_t outer = b / (n2 * n3), tile = b - outer * (n2 * n3);
+  const int64_t r0 = (tile / n3) * kLinTile, c0 = (tile % n3) * kLinTile;
+  // outer index -> leading axes (rank 4: (i0, i1); rank 3: i0; rank 2: none)
+  int64_t lead[2] = {0, 0};
+  if (R == 4) {
+    lead[0] = outer / a.dims[1];
+    lead[1] = outer - lead[0] * a.dims[1];
+  } else if (R == 3) {
+    lead[0] = outer;
+  }
+  const int lx = threadIdx.x & 31, ly = threadIdx.x >> 5;
+  int64_t obase = 0;
+  for (int d = 0; d < R - 2; ++d) obase += lead[d] * a.ost[d];
+  double sr[4], si[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    sr[k] = 0.0;
+    si[k] = 0.0;
+    const int64_t r = r0 + ly + 8 * k, c = c0 + lx;
+    if (a.accumulate && r < D2 && c < D3) {
+      const int64_t oo = obase + r * a.ost[R - 2] + c;
+      sr[k] = a.out_re[oo];
+      if (a.out_im) si[k] = a.out_im[oo];
+    }
+  }
+  lincomb_tile_terms(a, lead, R - 2, r0, c0, D2, D3, tr, ti, sr, si);
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int64_t r = r0 + ly + 8 * k, c = c0 + lx;
@@ -443,6 +457,139 @@ __global__ __launch_bounds__(256) void lincomb_tiled_kernel(LincombArgs a) {
     a.out_re[oo] = sr[k];
     if (a.out_im) a.out_im[oo] = si[k];
   }
+}
+
+// p-th pair (r, c) of {0..n-1} with r < c (strict) or r <= c, row by row.
+__device__ __forceinline__ void lincomb_tri_index(int64_t p, int64_t n, bool strict, int64_t& r, int64_t& c) {
+  int64_t row = 0, len = strict ? n - 1 : n;
+  while (p >= len) {
+    p -= len;
+    ++row;
+    --len;
+  }
+  r = row;
+  c = row + (strict ? 1 : 0) + p;
+}
+
+// Rank-4 results antisymmetric in axes (0, 1) and/or (2, 3), innermost axis
+// contiguous.  A block owns one leading pair (i, j), i < j when axes 0, 1 are
+// antisymmetric, and one 32 x 32 tile (A, B) of the trailing axes, A <= B when
+// axes 2, 3 are; it sums the terms over the tile as lincomb_tiled_kernel does
+// and stores the tile with its mirror images x[j][i] = -x[i][j] and
+// x[..][b][a] = -x[..][a][b].  A diagonal tile takes its lower triangle from
+// the upper one and stores zeros on the diagonal; the blocks past the last
+// pair store the zero slices x[i][i].  Each source is read at the stored
+// canonical elements only, so a term and its permuted partners (x, x with
+// i, j exchanged, ...) read the source once between them.
+__global__ __launch_bounds__(256) void lincomb_anti_kernel(LincombArgs a, int64_t npairs, int64_t ntiles) {
+  __shared__ double tr[kLinTile][kLinTile + 1];
+  __shared__ double ti[kLinTile][kLinTile + 1];
+  const bool aij = (a.anti & 1) != 0, aab = (a.anti & 2) != 0;
+  const int64_t D0 = a.dims[0], D1 = a.dims[1], D2 = a.dims[2], D3 = a.dims[3];
+  const int64_t n3 = (D3 + kLinTile - 1) / kLinTile;
+  const int64_t b = blockIdx.x;
+  const int64_t pidx = b / ntiles, tidx = b - pidx * ntiles;
+  int64_t tA, tB;
+  if (aab) {
+    lincomb_tri_index(tidx, n3, false, tA, tB);
+  } else {
+    tA = tidx / n3;
+    tB = tidx - tA * n3;
+  }
+  const int64_t r0 = tA * kLinTile, c0 = tB * kLinTile;
+  const int lx = threadIdx.x & 31, ly = threadIdx.x >> 5;
+  const bool diag_tile = aab && tA == tB;
+  const int64_t os2 = a.ost[2];
+  if (pidx >= npairs) {  // zero slice x[i][i]
+    const int64_t i = pidx - npairs;
+    const int64_t ob = i * (a.ost[0] + a.ost[1]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int64_t r = r0 + ly + 8 * k, c = c0 + lx;
+      if (r < D2 && c < D3) {
+        a.out_re[ob + r * os2 + c] = 0.0;
+        if (a.out_im) a.out_im[ob + r * os2 + c] = 0.0;
+      }
+      const int64_t r2 = c0 + ly + 8 * k, c2 = r0 + lx;
+      if (aab && !diag_tile && r2 < D2 && c2 < D3) {
+        a.out_re[ob + r2 * os2 + c2] = 0.0;
+        if (a.out_im) a.out_im[ob + r2 * os2 + c2] = 0.0;
+      }
+    }
+    return;
+  }
+  int64_t lead[2];
+  if (aij) {
+    lincomb_tri_index(pidx, D0, true, lead[0], lead[1]);
+  } else {
+    lead[0] = pidx / D1;
+    lead[1] = pidx - lead[0] * D1;
+  }
+  const int64_t ob = lead[0] * a.ost[0] + lead[1] * a.ost[1];
+  const int64_t obm = lead[1] * a.ost[0] + lead[0] * a.ost[1];  // x[j][i] (aij)
+  double sr[4], si[4];
+  lincomb_tile_terms(a, lead, 2, r0, c0, D2, D3, tr, ti, sr, si);
+  if (diag_tile) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      tr[ly + 8 * k][lx] = sr[k];
+      ti[ly + 8 * k][lx] = si[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int rl = ly + 8 * k;
+      if (rl > lx) {
+        sr[k] = -tr[lx][rl];
+        si[k] = -ti[lx][rl];
+      } else if (rl == lx) {
+        sr[k] = 0.0;
+        si[k] = 0.0;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int64_t r = r0 + ly + 8 * k, c = c0 + lx;
+    if (r >= D2 || c >= D3) continue;
+    a.out_re[ob + r * os2 + c] = sr[k];
+    if (a.out_im) a.out_im[ob + r * os2 + c] = si[k];
+    if (aij) {
+      a.out_re[obm + r * os2 + c] = -sr[k];
+      if (a.out_im) a.out_im[obm + r * os2 + c] = -si[k];
+    }
+  }
+  if (aab && !diag_tile) {
+    // x[..][c][r] = -x[..][r][c]: thread (lx, ly) stores row c0 + ly + 8k,
+    // column r0 + lx, i.e. minus tile element (lx, ly + 8k)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      tr[ly + 8 * k][lx] = sr[k];
+      ti[ly + 8 * k][lx] = si[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int64_t r = c0 + ly + 8 * k, c = r0 + lx;
+      if (r >= D2 || c >= D3) continue;
+      const double vr = tr[lx][ly + 8 * k], vi = ti[lx][ly + 8 * k];
+      a.out_re[ob + r * os2 + c] = -vr;
+      if (a.out_im) a.out_im[ob + r * os2 + c] = -vi;
+      if (aij) {
+        a.out_re[obm + r * os2 + c] = vr;
+        if (a.out_im) a.out_im[obm + r * os2 + c] = vi;
+      }
+    }
+  }
+}
+
+// Whether the antisymmetric variant applies: rank 4, innermost output axis
+// contiguous, the antisymmetric axes of equal extent, one launch.
+static bool lincomb_use_anti(const LincombArgs& a) {
+  if (a.anti == 0 || a.rank != 4 || a.accumulate || a.ost[3] != 1) return false;
+  if ((a.anti & 1) && a.dims[0] != a.dims[1]) return false;
+  if ((a.anti & 2) && a.dims[2] != a.dims[3]) return false;
+  return true;
 }
 
 // Whether the tiled variant applies (and pays) for these arguments.
@@ -455,6 +602,16 @@ static bool lincomb_use_tiled(const LincombArgs& a) {
 
 void launch_lincomb(const LincombArgs& a, hipStream_t s) {
   if (a.total <= 0) return;
+  if (lincomb_use_anti(a)) {
+    const int64_t n2 = (a.dims[2] + kLinTile - 1) / kLinTile, n3 = (a.dims[3] + kLinTile - 1) / kLinTile;
+    const int64_t npairs = (a.anti & 1) ? a.dims[0] * (a.dims[0] - 1) / 2 : a.dims[0] * a.dims[1];
+    const int64_t ntiles = (a.anti & 2) ? n3 * (n3 + 1) / 2 : n2 * n3;
+    const int64_t blocks = (npairs + ((a.anti & 1) ? a.dims[0] : 0)) * ntiles;
+    if (blocks > 0 && blocks < (int64_t)1 << 31) {
+      hipLaunchKernelGGL(lincomb_anti_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a, npairs, ntiles);
+      return;
+    }
+  }
   if (lincomb_use_tiled(a)) {
     const int R = a.rank;
     const int64_t n2 = (a.dims[R - 2] + kLinTile - 1) / kLinTile, n3 = (a.dims[R - 1] + kLinTile - 1) / kLinTile;
@@ -467,6 +624,12 @@ void launch_lincomb(const LincombArgs& a, hipStream_t s) {
     }
   }
   hipLaunchKernelGGL(lincomb_kernel, dim3(grid_for(a.total)), dim3(256), 0, s, a);
+}
+
+const char* lincomb_kernel_name(const LincombArgs& a) {
+  if (a.total > 0 && lincomb_use_anti(a)) return "lincomb_anti_kernel";
+  if (a.total > 0 && lincomb_use_tiled(a)) return "lincomb_tiled_kernel";
+  return "lincomb_kernel";
 }
 
 // ===========================================================================

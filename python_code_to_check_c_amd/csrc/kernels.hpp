@@ -19,6 +19,17 @@ __device__ __forceinline__ int remap_tile(int bid, int tiles) {
   return x * q + (x < r ? x : r) + loc;
 }
 
+// Tile (tm, tn) of the L-th tile when the tile grid is walked in bands of G
+// tile rows, each band column by column: consecutive tiles (those one XCD runs
+// at the same time after remap_tile) then form G x (32 / G) patches that read
+// G A panels and 32 / G B panels instead of 32 A panels and one or two B panels.
+__device__ __forceinline__ void grouped_tile(int L, int tiles_m, int tiles_n, int G, int& tm, int& tn) {
+  const int width = G * tiles_n, band = L / width, first = band * G;
+  const int gs = tiles_m - first < G ? tiles_m - first : G, r = L - band * width;
+  tm = first + r % gs;
+  tn = r / gs;
+}
+
 // ---------------------------------------------------------------------------
 // Strided tensor contraction on FP64 MFMA ("GEMM-like tensor-tensor", GETT).
 // The M, N, K and batch index groups of an einsum are flattened on the host
@@ -64,8 +75,14 @@ struct LincombArgs {
   int nterms;
   int accumulate;
   LincombTerm t[kMaxTerms];
+  // rank 4: bit 0 set when the result changes sign under exchanging axes 0
+  // and 1, bit 1 for axes 2 and 3 (every term is then read at the elements
+  // with i < j / a <= b tiles only and the rest is stored as mirror images)
+  int anti = 0;
 };
 void launch_lincomb(const LincombArgs& a, hipStream_t s);
+// Name of the kernel launch_lincomb runs for these arguments (profiles).
+const char* lincomb_kernel_name(const LincombArgs& a);
 
 // ---------------------------------------------------------------------------
 // Field + Fock build for one stage time of the current step:
@@ -274,6 +291,7 @@ struct OzGemmArgs {
   const int64_t* cm2 = nullptr; const int64_t* cn2 = nullptr;  // mirrored rows / columns, as in GemmScatterArgs
   double* out_re = nullptr; double* out_im = nullptr;
   double alpha_re = 1.0, alpha_im = 0.0;
+  int tile_group = 1;                 // bands of tile rows (grouped_tile); 1: row index fastest
 };
 void launch_ozgemm(const OzGemmArgs& a, hipStream_t s);
 

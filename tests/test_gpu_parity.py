@@ -26,14 +26,18 @@ RHS = [("T1eq_rhs", _lib.RHS_T1), ("T2eq_rhs", _lib.RHS_T2), ("lam_1eq_rhs", _li
 # Engine modes: the default lowering; every skinny contraction (one side with
 # at most 32 rows) on the streaming VALU kernel; no skinny path (everything
 # small on the strided MFMA kernel); every batch-free contraction through the
-# dense-GEMM path; the dense-GEMM path with its K dimension split over 3
-# partial products; the particle-particle ladders on all (i, j) rows instead
-# of the i < j rows of the antisymmetric amplitudes.
+# dense-GEMM path, outputs antisymmetric in two column letters computed for
+# their ordered pairs at any summed length (by default only from K = 1024 on);
+# the dense-GEMM path with its K dimension split over 3 partial products; the
+# particle-particle ladders on all (i, j) rows instead of the i < j rows of
+# the antisymmetric amplitudes; antisymmetric linear combinations summed at
+# every element instead of the canonical ones.
 MODES = {"default": {}, "skinny": {"TDCC_SKINNY_MIN": "0"}, "gett": {"TDCC_SKINNY_MIN": "-1"},
-         "gemm": {"TDCC_GEMM_MIN_FLOPS": "0"},
+         "gemm": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_NPAIR_MIN_K": "0"},
          "gemm_splitk": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_GEMM_SPLITS": "3"},
-         "full_rows": {"TDCC_IJ_PAIRS": "0"}}
-ENV_KEYS = ("TDCC_GEMM_MIN_FLOPS", "TDCC_GEMM_SPLITS", "TDCC_SKINNY_MIN", "TDCC_IJ_PAIRS")
+         "full_rows": {"TDCC_IJ_PAIRS": "0"}, "plain_lincomb": {"TDCC_LINCOMB_ANTI": "0"}}
+ENV_KEYS = ("TDCC_GEMM_MIN_FLOPS", "TDCC_GEMM_SPLITS", "TDCC_SKINNY_MIN", "TDCC_IJ_PAIRS", "TDCC_NPAIR_MIN_K",
+            "TDCC_LINCOMB_ANTI")
 
 
 @pytest.fixture(scope="module", params=[(n, m) for m in MODES for n in ("o4v6", "c1_o10v4")],

@@ -34,8 +34,10 @@ def bound(S):
 # three-pass kernels (partial maxima, scales, digits); "multipass" forces the
 # latter everywhere.  "stream" keeps the default routing but sends every
 # skinny contraction on an integral block through the split GEMM path (at
-# large sizes only the blocks above 8M elements go there).
-MODES = {"default": {"TDCC_GEMM_MIN_FLOPS": "0"},
+# large sizes only the blocks above 8M elements go there).  "default" also
+# restricts outputs antisymmetric in two column letters to their ordered pairs
+# at every summed length (at large sizes only from K = 1024 on).
+MODES = {"default": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_NPAIR_MIN_K": "0"},
          "multipass": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_SPLIT_MULTIPASS": "1"},
          "stream": {"TDCC_STATIC_STREAM_MIN": "0"}}
 
@@ -46,7 +48,7 @@ def case(request):
     name, mode = request.param
     g = dict(np.load(os.path.join(HERE, "golden", f"golden_{name}.npz")))
     print(f"\n[split case {name} {mode}]", flush=True)
-    keys = ("TDCC_GEMM_MIN_FLOPS", "TDCC_SPLIT_MULTIPASS", "TDCC_STATIC_STREAM_MIN")
+    keys = ("TDCC_GEMM_MIN_FLOPS", "TDCC_SPLIT_MULTIPASS", "TDCC_STATIC_STREAM_MIN", "TDCC_NPAIR_MIN_K")
     saved = {k: os.environ.get(k) for k in keys}
     for k in keys:
         os.environ.pop(k, None)
