@@ -74,8 +74,14 @@ struct GemmPath {
   // operands that never change while the program is bound (integral blocks)
   // are split once per bind into the graph's digit cache: entry index or -1
   int a_cache = -1, b_cache = -1;
+  // operands computed by the program whose digits several launches share:
+  // entry of the graph's share table (or -1), and whether this launch cuts them
+  int a_share = -1, b_share = -1;
+  bool a_split = true, b_split = true;
   // one split: the digit GEMM scatters into C itself (planes interleaved by row)
   bool fused = false;
+  // dot-product kernel (FP64) instead of the GEMM: partials in the node scratch
+  bool dot = false;
 };
 
 struct Graph::Launch {
@@ -88,6 +94,7 @@ struct Graph::Launch {
   SplitArgs sa{}, sb{};
   OzGemmArgs oz{};
   GemmScatterArgs sc{};
+  DotArgs dt{};
   // skinny path: one operand with at most kSkinnyMax rows (small_is_a: the A side)
   bool skinny = false, small_is_a = true;
   SkinnyArgs sk{};
@@ -264,6 +271,13 @@ static int64_t npair_min_k() {
 static bool lincomb_anti_on() {
   const char* e = std::getenv("TDCC_LINCOMB_ANTI");
   return !e || std::atoi(e) != 0;
+}
+
+// Summed length from which dot-product shapes take the dot kernel
+// (TDCC_DOT_MIN_K; negative: never).
+static int64_t dot_min_k() {
+  const char* e = std::getenv("TDCC_DOT_MIN_K");
+  return e ? std::atoll(e) : 8192;
 }
 
 // Skinny contractions with at least this much big-operand work (rows x K) take
@@ -791,18 +805,30 @@ void Graph::plan() {
     if (live[i] && owns_buffer(nodes_[i].op)) ends[std::max(nodes_[i].last_use, i)].push_back(i);
   host_tables_.clear();
   launches_.clear();
+  dyn_uses_.clear();
+  shares_.clear();
+  for (int i = 0; i < N; ++i) {
+    if (!live[i]) continue;
+    Node& n = nodes_[i];
+    if (n.op == Op::Ladder) n.scratch_len = ladder_scratch_bytes(o_, n.ij_pairs, v_, rt_->split_digits);
+    if (n.op == Op::Contract) lower_contract(i);
+  }
+  plan_shares();
+  std::vector<std::vector<int>> share_at(N), share_end(N);
+  for (int k = 0; k < (int)shares_.size(); ++k) {
+    share_at[shares_[k].first].push_back(k);
+    share_end[shares_[k].last].push_back(k);
+  }
   for (int i = 0; i < N; ++i) {
     if (!live[i]) continue;
     Node& n = nodes_[i];
     if (owns_buffer(n.op)) n.arena_off = alloc(size(n.shape) * (n.cplx ? 2 : 1) * (int64_t)sizeof(double));
-    if (n.op == Op::Ladder) n.scratch_len = ladder_scratch_bytes(o_, n.ij_pairs, v_, rt_->split_digits);
-    if (n.op == Op::Contract || n.op == Op::Ladder) {
-      if (n.op == Op::Contract) lower_contract(i);
-      if (n.scratch_len > 0) {
-        n.scratch_off = alloc(n.scratch_len);
-        release(n.scratch_off, n.scratch_len);
-      }
+    for (int k : share_at[i]) shares_[k].off = alloc(shares_[k].bytes);
+    if ((n.op == Op::Contract || n.op == Op::Ladder) && n.scratch_len > 0) {
+      n.scratch_off = alloc(n.scratch_len);
+      release(n.scratch_off, n.scratch_len);
     }
+    for (int k : share_end[i]) release(shares_[k].off, shares_[k].bytes);
     for (int x : ends[i]) release(nodes_[x].arena_off, size(nodes_[x].shape) * (nodes_[x].cplx ? 2 : 1) * (int64_t)sizeof(double));
   }
   arena_bytes_ = top;
@@ -820,6 +846,59 @@ void Graph::plan() {
     cache_buf_ = nullptr;
   }
   if (cache_bytes_ > 0) TDCC_HIP(hipMalloc(&cache_buf_, cache_bytes_));
+}
+
+// Groups the launches that split one computed operand in one layout into
+// shares: consecutive uses with no write of the operand's external in
+// between (a finalize or assign) share one set of digit planes, cut by the
+// first launch of the group.  TDCC_SHARE_DIGITS=0 disables it.
+void Graph::plan_shares() {
+  if (const char* e = std::getenv("TDCC_SHARE_DIGITS"); e && std::atoi(e) == 0) return;
+  std::map<int, std::vector<int>> writes;  // external -> nodes writing it, ascending
+  for (int i = 0; i < (int)nodes_.size(); ++i) {
+    if (!live_[i]) continue;
+    const Node& n = nodes_[i];
+    if (n.op == Op::Finalize) {
+      writes[n.a].push_back(i);
+      writes[n.b].push_back(i);
+    } else if (n.op == Op::Assign) {
+      writes[n.b].push_back(i);
+    }
+  }
+  std::map<int, Launch*> by_node;
+  for (auto& L : launches_) by_node[L->node] = L.get();
+  auto root = [&](int x) {
+    while (nodes_[x].op == Op::View) x = nodes_[x].base;
+    return x;
+  };
+  auto written_between = [&](int src, int lo, int hi) {
+    auto it = writes.find(root(src));
+    if (it == writes.end()) return false;
+    for (int w : it->second)
+      if (w > lo && w < hi) return true;
+    return false;
+  };
+  for (auto& [key, uses] : dyn_uses_) {
+    size_t g0 = 0;
+    while (g0 < uses.size()) {
+      size_t g1 = g0 + 1;
+      while (g1 < uses.size() && !written_between(uses[g0].src, uses[g1 - 1].node, uses[g1].node)) ++g1;
+      if (g1 - g0 >= 2) {
+        DigitShare sh;
+        sh.bytes = uses[g0].bytes;
+        sh.first = uses[g0].node;
+        sh.last = uses[g1 - 1].node;
+        const int idx = (int)shares_.size();
+        shares_.push_back(sh);
+        for (size_t u = g0; u < g1; ++u) {
+          GemmPath& gp = by_node.at(uses[u].node)->gp;
+          (uses[u].side == 0 ? gp.a_share : gp.b_share) = idx;
+          (uses[u].side == 0 ? gp.a_split : gp.b_split) = u == g0;
+        }
+      }
+      g0 = g1;
+    }
+  }
 }
 
 // One line per live node on stderr: id, op, shape classes, planes, last use
@@ -1022,6 +1101,15 @@ void Graph::lower_contract(int id) {
   const char ia = innermost(la, sa), ib = innermost(lb, sb);
   const bool akc = ia && has(gk, ia);
   const bool bkc = ib && has(gk, ib);
+  // Dot products (A rows with planes at most 64, B rows at most 512, long
+  // summed group, both operands contiguous in the same summed letter) run in
+  // FP64 on the dot kernel, which streams both once through the offset
+  // tables: HBM-bound shapes on which neither the 128 x 128 GEMM tiles nor
+  // digit splits pay.
+  const int64_t dot_k = dot_min_k();
+  const bool use_dot = use_gemm && dot_k >= 0 && akc && ia == ib &&
+                       (nodes_[a].cplx ? 2 : 1) * gsize(gm) <= kDotMaxRows && (nodes_[b].cplx ? 2 : 1) * gsize(gn) <= 512 &&
+                       (double)gsize(gk) * kpair_frac >= (double)dot_k;
   auto by_stride = [&](std::string g, const std::map<char, int64_t>& st) {
     std::stable_sort(g.begin(), g.end(), [&](char x, char y) { return st.at(x) > st.at(y); });
     return g;
@@ -1035,7 +1123,7 @@ void Graph::lower_contract(int id) {
   std::string& npair = n.npair;
   mpair.clear();
   npair.clear();
-  if (k_pairs_ && use_gemm) {
+  if (k_pairs_ && use_gemm && !use_dot) {
     auto find_pair = [&](const std::string& grp, std::string& pr) {
       for (size_t x = 0; x < grp.size() && pr.empty(); ++x)
         for (size_t y = x + 1; y < grp.size() && pr.empty(); ++y)
@@ -1045,7 +1133,12 @@ void Graph::lower_contract(int id) {
     find_pair(gm, mpair);
     if ((double)gsize(gk) * kpair_frac >= (double)npair_min_k()) find_pair(gn, npair);
   }
-  gk = akc ? by_stride(gk, sa) : (bkc ? by_stride(gk, sb) : by_stride(gk, sa));
+  // summed letters in the memory order of an operand whose contiguous index
+  // is summed; when both are, the per-step digit split of the larger one
+  // follows its own memory order (integral blocks are split once per bind)
+  const double a_cut = is_static(a) ? 0.0 : (double)gsize(gm), b_cut = is_static(b) ? 0.0 : (double)gsize(gn);
+  const bool k_by_b = bkc && (!akc || (rt_->split_digits > 0 && !n.fp64 && use_gemm && !use_dot && b_cut > a_cut));
+  gk = k_by_b ? by_stride(gk, sb) : by_stride(gk, sa);
 
   auto push = [&](const std::vector<int64_t>& t) {
     const int64_t off = (int64_t)host_tables_.size();
@@ -1146,6 +1239,25 @@ void Graph::lower_contract(int id) {
     k.alpha_im = n.ai * n.kfac;
     n.scratch_len = sp > 1 ? sp * (L->ac || L->bc ? 2 : 1) * (int64_t)k.I * k.J * (int64_t)sizeof(double) : 0;
   }
+  if (use_dot) {
+    GemmPath& gp = L->gp;
+    gp.on = true;
+    gp.dot = true;
+    gp.pa = L->ac ? 2 : 1;
+    gp.pb = L->bc ? 2 : 1;
+    // about 1024 blocks over the column tiles and K ranges of 32-multiples
+    const int64_t ntn = (gp.pb * Nn + 63) / 64;
+    int64_t sp = std::max<int64_t>(1, std::min<int64_t>((K + 255) / 256, (1024 + ntn - 1) / ntn));
+    int64_t chunk = (K + sp - 1) / sp;
+    chunk = (chunk + 31) / 32 * 32;
+    sp = (K + chunk - 1) / chunk;
+    gp.splits = (int)sp;
+    gp.k_chunk = chunk;
+    n.scratch_len = sp * gp.pa * M * gp.pb * Nn * (int64_t)sizeof(double);
+    L->node = id;
+    launches_.push_back(std::move(L));
+    return;
+  }
   if (use_gemm) {
     GemmPath& gp = L->gp;
     gp.on = true;
@@ -1185,12 +1297,17 @@ void Graph::lower_contract(int id) {
       gp.zb_off = gp.pb_off + al256(ozaki_split_workspace(NB, gp.Kp, gp.b_cols));
       gp.fused = sp == 1;
       n.scratch_len = gp.zb_off + (gp.fused ? 0 : al256(8 * sp * R * NB));
-      auto cache_entry = [&](int x, const std::string& rows, const std::string& rpair, bool cols, int64_t nrows,
-                             int64_t plane) {
+      // layout of an operand's digit planes: its value, its index labels, the
+      // row and summed letters in order, pair restrictions, orientation, rows
+      // (planes included), digits, plane interleaving
+      auto layout_key = [&](int x, const std::string& labels, const std::string& rows, const std::string& rpair,
+                            bool cols, int64_t nrows) {
+        return nodes_[x].key + "|" + labels + "|" + rows + "|" + rpair + "|" + gk + "|" + n.kpair + "|" +
+               (cols ? "c" : "r") + "|" + std::to_string(nrows) + "|" + std::to_string(S) + "|" +
+               (gp.fused ? "i" : "s");
+      };
+      auto cache_entry = [&](int x, const std::string& key, int64_t nrows, int64_t plane) {
         if (!is_static(x)) return -1;
-        const std::string key = nodes_[x].key + "|" + rows + "|" + rpair + "|" + gk + "|" +
-                                n.kpair + "|" + (cols ? "c" : "r") + "|" + std::to_string(nrows) + "|" +
-                                std::to_string(S);
         auto it = cache_index_.find(key);
         if (it != cache_index_.end()) return it->second;
         DigitCache e;
@@ -1202,8 +1319,12 @@ void Graph::lower_contract(int id) {
         cache_index_[key] = idx;
         return idx;
       };
-      gp.a_cache = cache_entry(a, gm, mpair, gp.a_cols, R, gp.a_plane);
-      gp.b_cache = cache_entry(b, gn, npair, gp.b_cols, NB, gp.b_plane);
+      const std::string akey = layout_key(a, la, gm, mpair, gp.a_cols, R);
+      const std::string bkey = layout_key(b, lb, gn, npair, gp.b_cols, NB);
+      gp.a_cache = cache_entry(a, akey, R, gp.a_plane);
+      gp.b_cache = cache_entry(b, bkey, NB, gp.b_plane);
+      if (gp.a_cache < 0) dyn_uses_[akey].push_back({id, 0, a, S * gp.a_plane + al256(8 * R)});
+      if (gp.b_cache < 0 && bkey != akey) dyn_uses_[bkey].push_back({id, 1, b, S * gp.b_plane + al256(8 * NB)});
       L->node = id;
       launches_.push_back(std::move(L));
       return;
@@ -1284,7 +1405,42 @@ void Graph::bind(double* arena) {
           k.partial = g.partial;
           if ((k.s_im || k.g_im) && !k.c_im) throw Error("internal: complex product into a real output");
         }
-        if (L->gp.on && L->gp.S > 0) {
+        if (L->gp.dot) {
+          const GemmPath& gp = L->gp;
+          DotArgs& d = L->dt;
+          d.a_re = g.a_re;
+          d.a_im = g.a_im;
+          d.b_re = g.b_re;
+          d.b_im = g.b_im;
+          d.am = g.am;
+          d.ak = g.ak;
+          d.bn = g.bn;
+          d.bk = g.bk;
+          d.M = g.M;
+          d.N = g.N;
+          d.K = g.K;
+          d.k_chunk = gp.k_chunk;
+          d.splits = gp.splits;
+          d.Z = arena_ + n.scratch_off / (int64_t)sizeof(double);
+          GemmScatterArgs& sc = L->sc;
+          sc.Z = d.Z;
+          sc.ldz = gp.pb * (int64_t)g.N;
+          sc.splits = 1;
+          sc.z_stride = 0;
+          sc.M = g.M;
+          sc.N = g.N;
+          sc.kind = (L->ac && L->bc) ? 2 : (L->ac ? 1 : 0);
+          if (!L->ac && L->bc) throw Error("internal: complex operand not on the row side");
+          sc.cm = g.cm;
+          sc.cn = g.cn;
+          sc.cm2 = nullptr;
+          sc.cn2 = nullptr;
+          sc.out_re = C.re;
+          sc.out_im = n.cplx ? C.im : nullptr;
+          sc.alpha_re = n.ar * n.kfac;
+          sc.alpha_im = n.ai * n.kfac;
+          sc.accumulate = 0;
+        } else if (L->gp.on && L->gp.S > 0) {
           const GemmPath& gp = L->gp;
           char* scr = reinterpret_cast<char*>(arena_) + n.scratch_off;
           SplitArgs& sa = L->sa;  // A' = [A_re; A_im] rows x K
@@ -1321,6 +1477,14 @@ void Graph::bind(double* arena) {
           sb.interleave = gp.fused;
           // static operands: digits live in the cache, split here once per bind
           // (the node's scratch serves as the split workspace)
+          // shared digits of computed operands live in their arena block
+          for (auto [idx, sp, plane] : {std::make_tuple(gp.a_share, &sa, gp.a_plane),
+                                        std::make_tuple(gp.b_share, &sb, gp.b_plane)}) {
+            if (idx < 0) continue;
+            char* blk = reinterpret_cast<char*>(arena_) + shares_[idx].off;
+            sp->D = reinterpret_cast<int8_t*>(blk);
+            sp->scale = reinterpret_cast<double*>(blk + gp.S * plane);
+          }
           for (auto [idx, sp, cols] : {std::make_tuple(gp.a_cache, &sa, gp.a_cols),
                                        std::make_tuple(gp.b_cache, &sb, gp.b_cols)}) {
             if (idx < 0) continue;
@@ -1731,9 +1895,14 @@ void Graph::run(hipStream_t s) const {
     switch (L->kind) {
       case Launch::GETT:
         if (L->g.M == 0 || L->g.N == 0) break;
+        if (L->gp.dot) {
+          launch_dot(L->dt, s);
+          launch_gemm_scatter(L->sc, s);
+          break;
+        }
         if (L->gp.on && L->gp.S > 0) {
-          if (L->gp.a_cache < 0) launch_ozaki_split(L->sa, L->gp.a_cols, s);
-          if (L->gp.b_cache < 0) launch_ozaki_split(L->sb, L->gp.b_cols, s);
+          if (L->gp.a_cache < 0 && L->gp.a_split) launch_ozaki_split(L->sa, L->gp.a_cols, s);
+          if (L->gp.b_cache < 0 && L->gp.b_split) launch_ozaki_split(L->sb, L->gp.b_cols, s);
           launch_ozgemm(L->oz, s);
           if (!L->gp.fused) launch_gemm_scatter(L->sc, s);
           break;
@@ -1793,6 +1962,19 @@ void Graph::run_profiled(hipStream_t s, std::vector<LaunchRecord>& out) const {
           std::snprintf(nm, sizeof nm, "gett_kernel<%d,%d,%d,%d>%s", (int)L->ac, (int)L->bc, (int)L->akc, (int)L->bkc,
                         L->g.splits > 1 ? "+splitk" : "");
           char dt[160];
+          if (L->gp.dot) {
+            // algorithmic bytes: each operand read once, the partials written once
+            const DotArgs& d = L->dt;
+            const double pa = d.a_im ? 2 : 1, pb = d.b_im ? 2 : 1;
+            const double dbytes = 8.0 * ((pa * d.M + pb * d.N) * (double)d.K + pa * d.M * pb * d.N * d.splits);
+            std::snprintf(dt, sizeof dt, "%s rows=%d N=%d K=%lld splits=%d", nodes_[L->node].spec.c_str(),
+                          (int)(pa * d.M), (int)(pb * d.N), (long long)d.K, d.splits);
+            detail = dt;
+            timed("dot_kernel", fl, dbytes, [&] { launch_dot(L->dt, s); });
+            const double sbytes = 8.0 * L->sc.M * (double)L->sc.N * (L->sc.kind == 2 ? 6 : (L->sc.kind == 1 ? 4 : 2));
+            timed("gemm_scatter_kernel", 0.0, sbytes, [&] { launch_gemm_scatter(L->sc, s); });
+            break;
+          }
           if (L->gp.on && L->gp.S > 0) {
             const GemmPath& gp = L->gp;
             const double abytes = (8.0 + gp.S) * L->sa.planes * (double)L->sa.Rb * L->sa.K;
@@ -1801,13 +1983,13 @@ void Graph::run_profiled(hipStream_t s, std::vector<LaunchRecord>& out) const {
                             (long long)(x.planes * x.Rb), (long long)x.K);
               detail = dt;
             };
-            if (gp.a_cache < 0) {
+            if (gp.a_cache < 0 && gp.a_split) {
               split_detail(L->sa, "A");
               timed(gp.a_cols ? "ozaki_split_cols_kernel" : "ozaki_split_rows_kernel", 0.0, abytes,
                     [&] { launch_ozaki_split(L->sa, gp.a_cols, s); });
             }
             const double bbytes = (8.0 + gp.S) * L->sb.planes * (double)L->sb.Rb * L->sb.K;
-            if (gp.b_cache < 0) {
+            if (gp.b_cache < 0 && gp.b_split) {
               split_detail(L->sb, "B");
               timed(gp.b_cols ? "ozaki_split_cols_kernel" : "ozaki_split_rows_kernel", 0.0, bbytes,
                     [&] { launch_ozaki_split(L->sb, gp.b_cols, s); });

@@ -271,6 +271,20 @@ class Graph {
   };
   std::vector<DigitCache> cache_;
   std::map<std::string, int> cache_index_;
+  // digit planes of an operand computed by the program that several split
+  // launches read in the same layout: cut by the first launch into an arena
+  // block that stays allocated until the last one
+  struct DigitShare {
+    int64_t off = 0, bytes = 0;
+    int first = -1, last = -1;
+  };
+  std::vector<DigitShare> shares_;
+  struct DynUse {
+    int node, side, src;
+    int64_t bytes;
+  };
+  std::map<std::string, std::vector<DynUse>> dyn_uses_;  // layout key -> launches in node order
+  void plan_shares();
   int64_t cache_bytes_ = 0;
   char* cache_buf_ = nullptr;
   double* ones_buf_ = nullptr;

@@ -215,6 +215,28 @@ struct GemmScatterArgs {
 };
 void launch_gemm_scatter(const GemmScatterArgs& g, hipStream_t s);
 
+// ---------------------------------------------------------------------------
+// Dot-product contractions (both free groups small, the summed group long) on
+// FP64 MFMA, read through the offset tables without a gather:
+//   Z[s][r][c] = sum_{k in split s} A'[r][k] * B'[c][k]
+// with A' = [A_re; A_im] (row r = p M + m reads plane p at am[m] + ak[k]) and
+// B' likewise (pb N rows); pa M <= kDotMaxRows.  launch_dot then sums the
+// `splits` partial results in a fixed order into split 0, so Z[0] is what
+// launch_gemm_scatter (splits = 1, ldz = pb N) recombines and stores.
+constexpr int kDotMaxRows = 64;
+struct DotArgs {
+  const double* a_re; const double* a_im;   // a_im / b_im null: real operand
+  const double* b_re; const double* b_im;
+  const int64_t* am; const int64_t* ak;
+  const int64_t* bn; const int64_t* bk;
+  int M, N;
+  int64_t K;
+  int64_t k_chunk;                          // multiple of 32
+  int splits;
+  double* Z;                                // [splits][pa M][pb N]
+};
+void launch_dot(const DotArgs& a, hipStream_t s);
+
 // Store of one output element and its mirror images (see GemmScatterArgs):
 // y at (om, on), -y at the mirrored row or column, y at both.
 __device__ __forceinline__ void store_mirrored(double* re, double* im, int64_t om, int64_t on, int64_t om2,

@@ -1,9 +1,12 @@
 // Gather / scatter kernels that put a tensor contraction into dense GEMM form
-// (gfx950).  The operands of a contraction are addressed through the same
-// per-index-group offset tables as the strided contraction kernel.
+// (gfx950), and the dot-product contraction kernel.  The operands of a
+// contraction are addressed through the same per-index-group offset tables as
+// the strided contraction kernel.
 #include "kernels.hpp"
 
 namespace tdcc {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
 
 static int grid_of(int64_t total) {
   int64_t b = (total + 255) / 256;
@@ -75,6 +78,149 @@ __global__ void gemm_scatter_kernel(GemmScatterArgs g) {
 
 void launch_gemm_scatter(const GemmScatterArgs& g, hipStream_t s) {
   hipLaunchKernelGGL(gemm_scatter_kernel, dim3(grid_of((int64_t)g.M * g.N)), dim3(256), 0, s, g);
+}
+
+// ---------------------------------------------------------------------------
+// Dot-product contraction: a block owns all pa M rows of A' (zero padded to
+// 64) and 64 rows of B' over one K range, in 32-wide K steps staged through
+// LDS (two buffers, one barrier per step); wave w computes rows 16w .. 16w+15
+// as four 16 x 16 blocks on v_mfma_f64_16x16x4_f64 (lane l: A[row l%16][k
+// l/16], B[k l/16][col l%16], D[row l/16 + 4r][col l%16], as in the DGEMM
+// kernel).  The loader thread owns k = tid % 32 of every step and the rows
+// tid / 32 + 8p, p < 8, of both tiles (row offsets read once per block).  The
+// shapes are HBM-bound, so the loads run two steps ahead of the MFMAs in two
+// register sets (steps of even and odd parity), and the k offsets one step
+// ahead of the loads.
+// ---------------------------------------------------------------------------
+constexpr int DK_BK = 32, DK_BN = 64, DK_LD = 65;
+
+template <int PA, int PB>
+__global__ __launch_bounds__(256) void dot_kernel(DotArgs g) {
+  __shared__ double As[2][DK_BK][DK_LD];
+  __shared__ double Bs[2][DK_BK][DK_LD];
+  const int Mr = PA * g.M, Nc = PB * g.N;
+  const int n0 = blockIdx.x * DK_BN;
+  const int64_t kb = (int64_t)blockIdx.y * g.k_chunk;
+  const int64_t ke = kb + g.k_chunk < g.K ? kb + g.k_chunk : g.K;
+  const int64_t nsteps = (ke - kb + DK_BK - 1) / DK_BK;
+  const int tid = threadIdx.x, kk = tid & 31, r8 = tid >> 5;
+  const double* arow[8];
+  const double* brow[8];
+#pragma unroll
+  for (int p = 0; p < 8; ++p) {
+    const int r = r8 + 8 * p;
+    arow[p] = nullptr;
+    if (r < Mr) {
+      const bool im = PA == 2 && r >= g.M;
+      arow[p] = (im ? g.a_im : g.a_re) + g.am[im ? r - g.M : r];
+    }
+    const int c = n0 + r8 + 8 * p;
+    brow[p] = nullptr;
+    if (c < Nc) {
+      const bool im = PB == 2 && c >= g.N;
+      brow[p] = (im ? g.b_im : g.b_re) + g.bn[im ? c - g.N : c];
+    }
+  }
+  // k offsets of step t for this thread (-1: past the range)
+  auto offsets = [&](int64_t t, int64_t& oa, int64_t& ob) {
+    const int64_t k = kb + t * DK_BK + kk;
+    const bool ok = t < nsteps && k < ke;
+    oa = ok ? g.ak[k] : -1;
+    ob = ok ? g.bk[k] : -1;
+  };
+  auto load = [&](int64_t oa, int64_t ob, double* ra, double* rb) {
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      ra[p] = (oa >= 0 && arow[p]) ? arow[p][oa] : 0.0;
+      rb[p] = (ob >= 0 && brow[p]) ? brow[p][ob] : 0.0;
+    }
+  };
+  auto store = [&](int buf, const double* ra, const double* rb) {
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      As[buf][kk][r8 + 8 * p] = ra[p];
+      Bs[buf][kk][r8 + 8 * p] = rb[p];
+    }
+  };
+  d4 acc[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) acc[j] = d4{0.0, 0.0, 0.0, 0.0};
+  const int lane = tid & 63, w = tid >> 6, lr = lane & 15, lk = lane >> 4;
+  const bool active = 16 * w < Mr;
+  auto compute = [&](int buf) {
+    if (!active) return;
+#pragma unroll
+    for (int k4 = 0; k4 < DK_BK / 4; ++k4) {
+      const double a = As[buf][4 * k4 + lk][16 * w + lr];
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Bs[buf][4 * k4 + lk][16 * j + lr], acc[j], 0, 0, 0);
+    }
+  };
+  double xa[8], xb[8], ya[8], yb[8];
+  int64_t oxa, oxb, oya, oyb;
+  offsets(0, oxa, oxb);
+  load(oxa, oxb, xa, xb);
+  store(0, xa, xb);
+  offsets(1, oya, oyb);
+  load(oya, oyb, ya, yb);
+  offsets(2, oxa, oxb);
+  __syncthreads();
+  // entering step t (even): LDS 0 holds step t, set y step t + 1 (in flight),
+  // ox the offsets of step t + 2
+  for (int64_t t = 0; t < nsteps; t += 2) {
+    load(oxa, oxb, xa, xb);        // step t + 2
+    offsets(t + 3, oya, oyb);
+    compute(0);                    // step t
+    if (t + 1 < nsteps) store(1, ya, yb);
+    __syncthreads();
+    if (t + 1 >= nsteps) break;
+    load(oya, oyb, ya, yb);        // step t + 3
+    offsets(t + 4, oxa, oxb);
+    compute(1);                    // step t + 1
+    if (t + 2 < nsteps) store(0, xa, xb);
+    __syncthreads();
+  }
+  if (!active) return;
+  double* Zs = g.Z + (int64_t)blockIdx.y * Mr * Nc;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int col = n0 + 16 * j + lr;
+    if (col >= Nc) continue;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = 16 * w + lk + 4 * r;
+      if (row < Mr) Zs[(int64_t)row * Nc + col] = acc[j][r];
+    }
+  }
+}
+
+// Z[0][e] = sum over splits s of Z[s][e]: one wave per element, lane l sums
+// the splits l, l + 64, ... in order, then a butterfly over the lanes.
+__global__ __launch_bounds__(256) void dot_reduce_kernel(double* Z, int splits, int64_t n) {
+  const int lane = threadIdx.x & 63;
+  const int64_t e = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (e >= n) return;  // whole wave leaves together
+  double s = 0.0;
+  for (int k = lane; k < splits; k += 64) s += Z[(int64_t)k * n + e];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if (lane == 0) Z[e] = s;
+}
+
+void launch_dot(const DotArgs& g, hipStream_t s) {
+  if (g.M <= 0 || g.N <= 0) return;
+  const int pa = g.a_im ? 2 : 1, pb = g.b_im ? 2 : 1;
+  const int64_t Mr = pa * (int64_t)g.M, Nc = pb * (int64_t)g.N;
+  const dim3 grid((unsigned)((Nc + DK_BN - 1) / DK_BN), (unsigned)g.splits), block(256);
+  if (pa == 2 && pb == 2) hipLaunchKernelGGL((dot_kernel<2, 2>), grid, block, 0, s, g);
+  else if (pa == 2) hipLaunchKernelGGL((dot_kernel<2, 1>), grid, block, 0, s, g);
+  else if (pb == 2) hipLaunchKernelGGL((dot_kernel<1, 2>), grid, block, 0, s, g);
+  else hipLaunchKernelGGL((dot_kernel<1, 1>), grid, block, 0, s, g);
+  if (g.splits > 1) {
+    const int64_t n = Mr * Nc;
+    hipLaunchKernelGGL(dot_reduce_kernel, dim3((unsigned)((n + 3) / 4)), block, 0, s, g.Z, g.splits, n);
+  }
 }
 
 }  // namespace tdcc

@@ -36,10 +36,13 @@ def bound(S):
 # skinny contraction on an integral block through the split GEMM path (at
 # large sizes only the blocks above 8M elements go there).  "default" also
 # restricts outputs antisymmetric in two column letters to their ordered pairs
-# at every summed length (at large sizes only from K = 1024 on).
+# at every summed length (at large sizes only from K = 1024 on).  "dot" sends
+# the dot-product shapes (at most 64 A rows, 512 B rows) to the FP64 dot
+# kernel at every summed length (at large sizes only from K = 8192 on).
 MODES = {"default": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_NPAIR_MIN_K": "0"},
          "multipass": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_SPLIT_MULTIPASS": "1"},
-         "stream": {"TDCC_STATIC_STREAM_MIN": "0"}}
+         "stream": {"TDCC_STATIC_STREAM_MIN": "0"},
+         "dot": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_DOT_MIN_K": "0"}}
 
 
 @pytest.fixture(scope="module", params=[(n, m) for m in MODES for n in ("o4v6", "c1_o10v4")],
@@ -48,7 +51,8 @@ def case(request):
     name, mode = request.param
     g = dict(np.load(os.path.join(HERE, "golden", f"golden_{name}.npz")))
     print(f"\n[split case {name} {mode}]", flush=True)
-    keys = ("TDCC_GEMM_MIN_FLOPS", "TDCC_SPLIT_MULTIPASS", "TDCC_STATIC_STREAM_MIN", "TDCC_NPAIR_MIN_K")
+    keys = ("TDCC_GEMM_MIN_FLOPS", "TDCC_SPLIT_MULTIPASS", "TDCC_STATIC_STREAM_MIN", "TDCC_NPAIR_MIN_K",
+            "TDCC_DOT_MIN_K")
     saved = {k: os.environ.get(k) for k in keys}
     for k in keys:
         os.environ.pop(k, None)
@@ -97,6 +101,35 @@ def test_split_is_deterministic(case):
     a = [ctx.rhs(kind, g["Fc"], g["t1"], g["t2"], g["lam1"], g["lam2"]) for _, kind in RHS]
     b = [ctx.rhs(kind, g["Fc"], g["t1"], g["t2"], g["lam1"], g["lam2"]) for _, kind in RHS]
     for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_shared_digits_are_bitwise_neutral(case):
+    """Launches that split one computed operand in one layout share its digit
+    planes (cut once, at the first of them); without sharing every launch cuts
+    its own copy of the same digits, so the results must be bitwise equal."""
+    g, ctx = case
+    ctx.set_precision(4)
+
+    def evaluate(c):
+        out = [c.rhs(kind, g["Fc"], g["t1"], g["t2"], g["lam1"], g["lam2"]) for _, kind in RHS]
+        c.set_state(g["t1"], g["t2"], g["lam1"], g["lam2"])
+        mu, _ = c.rk4_run("CCSD", int(g["rk_steps"]), 0.0, float(g["rk_dt"]), float(g["rk_w0"]), float(g["rk_A"]))
+        return out + [mu]
+
+    shared = evaluate(ctx)
+    os.environ["TDCC_SHARE_DIGITS"] = "0"
+    try:
+        c2 = _lib.Context(int(g["o"]), int(g["v"]))
+        try:
+            c2.set_system(g["TEI"], g["F"], g["mu"])
+            c2.set_precision(4)
+            own = evaluate(c2)
+        finally:
+            c2.close()
+    finally:
+        os.environ.pop("TDCC_SHARE_DIGITS", None)
+    for x, y in zip(shared, own):
         assert np.array_equal(x, y)
 
 
