@@ -214,7 +214,8 @@ def main():
 
     f_ref, f_alg = step_flops(o, v, args.method)
     prec = (f"split: int8 digit planes ({args.digits} digits, Ozaki scheme, exact int32 accumulation) for the "
-            "ladders and contractions above 2 GFLOP; FP64 state, field, small contractions and dipole"
+            "ladders and contractions above 2 GFLOP; FP64 state, field, small and dot-product contractions "
+            "and dipole"
             if split else "fp64")
     out = {
         "metric": METRIC, "value": sps, "unit": "steps/s", "n_gpus": 1, "steps": args.steps,

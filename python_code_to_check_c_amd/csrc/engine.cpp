@@ -1194,8 +1194,12 @@ void Graph::lower_contract(int id) {
   g.accumulate = 0;
   const int64_t tiles = (int64_t)g.tiles_m * g.tiles_n * Bt;
   int splits = 1;
-  if (tiles < 256 && K >= 512) {
-    splits = (int)std::min<int64_t>((K + 255) / 256, std::max<int64_t>(1, 1024 / tiles));
+  // Few tiles: split K so that each block runs a couple of 16-wide steps.
+  // Such launches are bound by the chain of dependent global reads per step
+  // (offset tables, operands), not by bandwidth or MFMA rate; at most 64
+  // partial products keep the ordered reduction cheap.
+  if (tiles < 256 && K >= 64) {
+    splits = (int)std::min<int64_t>((K + 31) / 32, std::min<int64_t>(64, std::max<int64_t>(1, 1024 / tiles)));
   }
   int64_t chunk = (K + splits - 1) / splits;
   chunk = (chunk + 15) / 16 * 16;
@@ -1896,8 +1900,7 @@ void Graph::run(hipStream_t s) const {
       case Launch::GETT:
         if (L->g.M == 0 || L->g.N == 0) break;
         if (L->gp.dot) {
-          launch_dot(L->dt, s);
-          launch_gemm_scatter(L->sc, s);
+          launch_dot(L->dt, L->sc, s);
           break;
         }
         if (L->gp.on && L->gp.S > 0) {
@@ -1970,9 +1973,7 @@ void Graph::run_profiled(hipStream_t s, std::vector<LaunchRecord>& out) const {
             std::snprintf(dt, sizeof dt, "%s rows=%d N=%d K=%lld splits=%d", nodes_[L->node].spec.c_str(),
                           (int)(pa * d.M), (int)(pb * d.N), (long long)d.K, d.splits);
             detail = dt;
-            timed("dot_kernel", fl, dbytes, [&] { launch_dot(L->dt, s); });
-            const double sbytes = 8.0 * L->sc.M * (double)L->sc.N * (L->sc.kind == 2 ? 6 : (L->sc.kind == 1 ? 4 : 2));
-            timed("gemm_scatter_kernel", 0.0, sbytes, [&] { launch_gemm_scatter(L->sc, s); });
+            timed("dot_kernel", fl, dbytes, [&] { launch_dot(L->dt, L->sc, s); });
             break;
           }
           if (L->gp.on && L->gp.S > 0) {

@@ -958,80 +958,138 @@ __device__ __forceinline__ int64_t pair_index(int a, int b, int v) {
 }
 
 // Rows of the packed ladder operands: all (i, j) (r = i o + j), or with
-// ij_pairs only i < j (r = pair index of (i, j) among o), the (i, j)
-// antisymmetry of the amplitudes making the other half redundant.
-__device__ __forceinline__ bool ladder_row(int i, int j, int o, bool ij_pairs, int64_t& r) {
-  if (!ij_pairs) {
-    r = (int64_t)i * o + j;
-    return true;
+// ij_pairs only i < j (r = pair index of (i, j) among o, the order in which
+// lincomb_tri_index enumerates the strict pairs), the (i, j) antisymmetry of
+// the amplitudes making the other half redundant.
+
+// Both kernels work on 32 x 32 tiles (A, B), A <= B, of the (v, v) plane at
+// one leading row, 256 threads, element k of a thread at (A 32 + ly + 8k,
+// B 32 + lx); the transposed tile (B, A) goes through LDS, so every global
+// access runs along a contiguous axis.  pack reads x[e][f] and x[f][e] of the
+// tile pair and writes the e < f entries; unpack reads the a < b entries of
+// its row, writes y[a][b] and y[b][a] (and, with ij_pairs, the mirrored row
+// y[j][i] with the opposite sign; blocks past the last pair write the zero
+// rows y[i][i]).  The values are those of the elementwise definition above.
+__global__ __launch_bounds__(256) void pack_pairs_kernel(const double* xr, const double* xi, double* X, int o, int v,
+                                                         int ij_pairs, int64_t ntiles) {
+  __shared__ double tr[kLinTile][kLinTile + 1];
+  const int64_t P = (int64_t)v * (v - 1) / 2, vv = (int64_t)v * v;
+  const int64_t rows = ij_pairs ? (int64_t)o * (o - 1) / 2 : (int64_t)o * o;
+  const int64_t nt = (v + kLinTile - 1) / kLinTile;
+  const int64_t blk = blockIdx.x;
+  const int64_t pr = blk / ntiles, t = blk - pr * ntiles;  // plane * rows + r, tile
+  const int64_t pl = pr / rows, r = pr - pl * rows;
+  int64_t i, j, tA, tB;
+  if (ij_pairs) lincomb_tri_index(r, o, true, i, j);
+  else {
+    i = r / o;
+    j = r - i * o;
   }
-  if (i >= j) return false;
-  r = pair_index(i, j, o);
-  return true;
+  lincomb_tri_index(t, nt, false, tA, tB);
+  const double* x = (pl ? xi : xr) + (i * o + j) * vv;
+  const int lx = threadIdx.x & 31, ly = threadIdx.x >> 5;
+  const int64_t r0 = tA * kLinTile, c0 = tB * kLinTile;
+  // x[f][e] of this tile: row f = c0 + .., column e = r0 + ..
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int64_t f = c0 + ly + 8 * k, e = r0 + lx;
+    tr[ly + 8 * k][lx] = (f < v && e < v) ? x[f * v + e] : 0.0;
+  }
+  __syncthreads();
+  double* Xrow = X + (pl * rows + r) * P;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int64_t e = r0 + ly + 8 * k, f = c0 + lx;
+    if (e >= f || f >= v) continue;
+    const double xfe = tr[lx][ly + 8 * k];
+    Xrow[pair_index((int)e, (int)f, v)] = __dsub_rn(x[e * v + f], xfe);
+  }
 }
 
-__global__ void pack_pairs_kernel(const double* xr, const double* xi, double* X, int o, int v, int ij_pairs) {
-  const int64_t P = (int64_t)v * (v - 1) / 2, vv = (int64_t)v * v, oo = (int64_t)o * o;
-  const int64_t rows = ij_pairs ? (int64_t)o * (o - 1) / 2 : oo, total = 2 * oo * vv;
-  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
-       idx += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t pij = idx / vv;  // plane * o^2 + (i o + j)
-    const int64_t ef = idx - pij * vv;
-    const int e = (int)(ef / v), f = (int)(ef - (int64_t)e * v);
-    if (e >= f) continue;
-    const int64_t pl = pij / oo, ij = pij - pl * oo;
-    const int i = (int)(ij / o), j = (int)(ij - (int64_t)i * o);
-    int64_t r;
-    if (!ladder_row(i, j, o, ij_pairs, r)) continue;
-    const double* x = pl ? xi : xr;
-    const double val = __dsub_rn(x[ij * vv + (int64_t)e * v + f], x[ij * vv + (int64_t)f * v + e]);
-    X[(pl * rows + r) * P + pair_index(e, f, v)] = val;
+__global__ __launch_bounds__(256) void unpack_pairs_kernel(const double* Y, double* yr, double* yi, int o, int v,
+                                                           double alpha, int ij_pairs, int64_t npairs, int64_t ntiles) {
+  __shared__ double tr[kLinTile][kLinTile + 1];
+  const int64_t P = (int64_t)v * (v - 1) / 2, vv = (int64_t)v * v;
+  const int64_t rows = ij_pairs ? (int64_t)o * (o - 1) / 2 : (int64_t)o * o;
+  const int64_t nt = (v + kLinTile - 1) / kLinTile;
+  const int64_t blk = blockIdx.x;
+  const int64_t pp = blk / ntiles, t = blk - pp * ntiles;  // plane * npairs + pair, tile
+  const int64_t pl = pp / npairs, p = pp - pl * npairs;
+  int64_t tA, tB;
+  lincomb_tri_index(t, nt, false, tA, tB);
+  const int lx = threadIdx.x & 31, ly = threadIdx.x >> 5;
+  const int64_t r0 = tA * kLinTile, c0 = tB * kLinTile;
+  const bool diag_tile = tA == tB;
+  double* yp = pl ? yi : yr;
+  if (p >= rows) {  // zero row y[i][i] (ij_pairs)
+    const int64_t i = p - rows;
+    double* y = yp + (i * o + i) * vv;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int64_t a = r0 + ly + 8 * k, b = c0 + lx;
+      if (a < v && b < v) y[a * v + b] = 0.0;
+      const int64_t a2 = c0 + ly + 8 * k, b2 = r0 + lx;
+      if (!diag_tile && a2 < v && b2 < v) y[a2 * v + b2] = 0.0;
+    }
+    return;
   }
-}
-
-__global__ void unpack_pairs_kernel(const double* Y, double* yr, double* yi, int o, int v, double alpha, int ij_pairs) {
-  const int64_t P = (int64_t)v * (v - 1) / 2, vv = (int64_t)v * v, oo = (int64_t)o * o;
-  const int64_t rows = ij_pairs ? (int64_t)o * (o - 1) / 2 : oo, total = 2 * oo * vv;
-  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
-       idx += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t pij = idx / vv;
-    const int64_t ab = idx - pij * vv;
-    const int a = (int)(ab / v), b = (int)(ab - (int64_t)a * v);
-    const int64_t pl = pij / oo, ij = pij - pl * oo;
-    const int i = (int)(ij / o), j = (int)(ij - (int64_t)i * o);
-    double sign = alpha;
-    int64_t r = 0;
-    bool live = true;
-    if (ij_pairs) {
-      if (i == j) live = false;
-      else if (i < j) r = pair_index(i, j, o);
-      else {
-        r = pair_index(j, i, o);
-        sign = -sign;
-      }
-    } else {
-      r = ij;
+  int64_t i, j;
+  if (ij_pairs) lincomb_tri_index(p, o, true, i, j);
+  else {
+    i = p / o;
+    j = p - i * o;
+  }
+  const double* Yrow = Y + (pl * rows + p) * P;
+  // w = alpha * Y[pair(a, b)] for a < b of the tile, 0 elsewhere
+  double w[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int64_t a = r0 + ly + 8 * k, b = c0 + lx;
+    w[k] = (a < b && b < v) ? __dmul_rn(alpha, Yrow[pair_index((int)a, (int)b, v)]) : 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) tr[ly + 8 * k][lx] = w[k];
+  __syncthreads();
+  double* y = yp + (i * o + j) * vv;
+  double* ym = yp + (j * o + i) * vv;  // mirrored row (ij_pairs)
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int64_t a = r0 + ly + 8 * k, b = c0 + lx;
+    if (a >= v || b >= v) continue;
+    // on a diagonal tile the lower triangle is minus the upper one
+    const double val = (diag_tile && a > b) ? -tr[lx][ly + 8 * k] : w[k];
+    y[a * v + b] = val;
+    if (ij_pairs) ym[a * v + b] = -val;
+  }
+  if (!diag_tile) {
+    // y[b][a] = -y[a][b]: thread stores row c0 + ly + 8k, column r0 + lx
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int64_t a = c0 + ly + 8 * k, b = r0 + lx;
+      if (a >= v || b >= v) continue;
+      const double val = -tr[lx][ly + 8 * k];
+      y[a * v + b] = val;
+      if (ij_pairs) ym[a * v + b] = -val;
     }
-    double val = 0.0;
-    if (live && a != b) {
-      const double* yrow = Y + (pl * rows + r) * P;
-      val = a < b ? __dmul_rn(sign, yrow[pair_index(a, b, v)]) : -__dmul_rn(sign, yrow[pair_index(b, a, v)]);
-    }
-    double* y = pl ? yi : yr;
-    y[ij * vv + ab] = val;
   }
 }
 
 void launch_pack_pairs(const double* xr, const double* xi, double* X, int o, int v, bool ij_pairs, hipStream_t s) {
-  const int64_t total = 2 * (int64_t)o * o * v * v;
-  hipLaunchKernelGGL(pack_pairs_kernel, dim3(grid_for(total)), dim3(256), 0, s, xr, xi, X, o, v, (int)ij_pairs);
+  const int64_t nt = (v + kLinTile - 1) / kLinTile, ntiles = nt * (nt + 1) / 2;
+  const int64_t rows = ij_pairs ? (int64_t)o * (o - 1) / 2 : (int64_t)o * o;
+  const int64_t blocks = 2 * rows * ntiles;
+  if (blocks <= 0) return;
+  hipLaunchKernelGGL(pack_pairs_kernel, dim3((unsigned)blocks), dim3(256), 0, s, xr, xi, X, o, v, (int)ij_pairs, ntiles);
 }
 
 void launch_unpack_pairs(const double* Y, double* yr, double* yi, int o, int v, double alpha, bool ij_pairs,
                          hipStream_t s) {
-  const int64_t total = 2 * (int64_t)o * o * v * v;
-  hipLaunchKernelGGL(unpack_pairs_kernel, dim3(grid_for(total)), dim3(256), 0, s, Y, yr, yi, o, v, alpha,
-                     (int)ij_pairs);
+  const int64_t nt = (v + kLinTile - 1) / kLinTile, ntiles = nt * (nt + 1) / 2;
+  const int64_t npairs = ij_pairs ? (int64_t)o * (o - 1) / 2 + o : (int64_t)o * o;
+  const int64_t blocks = 2 * npairs * ntiles;
+  if (blocks <= 0) return;
+  hipLaunchKernelGGL(unpack_pairs_kernel, dim3((unsigned)blocks), dim3(256), 0, s, Y, yr, yi, o, v, alpha,
+                     (int)ij_pairs, npairs, ntiles);
 }
 
 __global__ void sum_splits_kernel(double* y, int splits, int64_t stride, int64_t n) {

@@ -221,8 +221,9 @@ void launch_gemm_scatter(const GemmScatterArgs& g, hipStream_t s);
 //   Z[s][r][c] = sum_{k in split s} A'[r][k] * B'[c][k]
 // with A' = [A_re; A_im] (row r = p M + m reads plane p at am[m] + ak[k]) and
 // B' likewise (pb N rows); pa M <= kDotMaxRows.  launch_dot then sums the
-// `splits` partial results in a fixed order into split 0, so Z[0] is what
-// launch_gemm_scatter (splits = 1, ldz = pb N) recombines and stores.
+// `splits` partial results in a fixed order and stores them as
+// launch_gemm_scatter would (complex recombination by kind, alpha, output
+// offset tables; no mirrors, no accumulation).
 constexpr int kDotMaxRows = 64;
 struct DotArgs {
   const double* a_re; const double* a_im;   // a_im / b_im null: real operand
@@ -235,7 +236,7 @@ struct DotArgs {
   int splits;
   double* Z;                                // [splits][pa M][pb N]
 };
-void launch_dot(const DotArgs& a, hipStream_t s);
+void launch_dot(const DotArgs& a, const GemmScatterArgs& out, hipStream_t s);
 
 // Store of one output element and its mirror images (see GemmScatterArgs):
 // y at (om, on), -y at the mirrored row or column, y at both.

@@ -195,32 +195,52 @@ __global__ __launch_bounds__(256) void dot_kernel(DotArgs g) {
   }
 }
 
-// Z[0][e] = sum over splits s of Z[s][e]: one wave per element, lane l sums
-// the splits l, l + 64, ... in order, then a butterfly over the lanes.
-__global__ __launch_bounds__(256) void dot_reduce_kernel(double* Z, int splits, int64_t n) {
+// Sum of the split partials and the scatter of gemm_scatter_kernel in one
+// pass: one wave per output element (m, n); lane l sums the splits l, l + 64,
+// ... in order for each of the (up to four) Z entries the element combines,
+// then a butterfly over the lanes; lane 0 stores alpha * value.
+__global__ __launch_bounds__(256) void dot_reduce_kernel(DotArgs d, GemmScatterArgs g) {
   const int lane = threadIdx.x & 63;
   const int64_t e = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (e >= n) return;  // whole wave leaves together
-  double s = 0.0;
-  for (int k = lane; k < splits; k += 64) s += Z[(int64_t)k * n + e];
+  if (e >= (int64_t)g.M * g.N) return;  // whole wave leaves together
+  const int64_t m = e / g.N, n = e - m * g.N;
+  const int pa = d.a_im ? 2 : 1, pb = d.b_im ? 2 : 1;
+  const int64_t ldz = pb * (int64_t)g.N, zs = pa * (int64_t)g.M * ldz;
+  double rr = 0.0, ii = 0.0, ri = 0.0, ir = 0.0;
+  for (int s = lane; s < d.splits; s += 64) {
+    const double* z = d.Z + (int64_t)s * zs;
+    rr += z[m * ldz + n];
+    if (g.kind >= 1) ir += z[(m + g.M) * ldz + n];
+    if (g.kind == 2) {
+      ri += z[m * ldz + g.N + n];
+      ii += z[(m + g.M) * ldz + g.N + n];
+    }
+  }
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-  if (lane == 0) Z[e] = s;
+  for (int off = 32; off > 0; off >>= 1) {
+    rr += __shfl_xor(rr, off);
+    ir += __shfl_xor(ir, off);
+    ri += __shfl_xor(ri, off);
+    ii += __shfl_xor(ii, off);
+  }
+  if (lane != 0) return;
+  const double vr = g.kind == 2 ? rr - ii : rr;
+  const double vi = g.kind == 2 ? ri + ir : ir;
+  store_mirrored(g.out_re, g.out_im, g.cm[m], g.cn[n], -1, -1, g.alpha_re * vr - g.alpha_im * vi,
+                 g.alpha_re * vi + g.alpha_im * vr);
 }
 
-void launch_dot(const DotArgs& g, hipStream_t s) {
-  if (g.M <= 0 || g.N <= 0) return;
-  const int pa = g.a_im ? 2 : 1, pb = g.b_im ? 2 : 1;
-  const int64_t Mr = pa * (int64_t)g.M, Nc = pb * (int64_t)g.N;
-  const dim3 grid((unsigned)((Nc + DK_BN - 1) / DK_BN), (unsigned)g.splits), block(256);
-  if (pa == 2 && pb == 2) hipLaunchKernelGGL((dot_kernel<2, 2>), grid, block, 0, s, g);
-  else if (pa == 2) hipLaunchKernelGGL((dot_kernel<2, 1>), grid, block, 0, s, g);
-  else if (pb == 2) hipLaunchKernelGGL((dot_kernel<1, 2>), grid, block, 0, s, g);
-  else hipLaunchKernelGGL((dot_kernel<1, 1>), grid, block, 0, s, g);
-  if (g.splits > 1) {
-    const int64_t n = Mr * Nc;
-    hipLaunchKernelGGL(dot_reduce_kernel, dim3((unsigned)((n + 3) / 4)), block, 0, s, g.Z, g.splits, n);
-  }
+void launch_dot(const DotArgs& d, const GemmScatterArgs& sc, hipStream_t s) {
+  if (d.M <= 0 || d.N <= 0) return;
+  const int pa = d.a_im ? 2 : 1, pb = d.b_im ? 2 : 1;
+  const int64_t Nc = pb * (int64_t)d.N;
+  const dim3 grid((unsigned)((Nc + DK_BN - 1) / DK_BN), (unsigned)d.splits), block(256);
+  if (pa == 2 && pb == 2) hipLaunchKernelGGL((dot_kernel<2, 2>), grid, block, 0, s, d);
+  else if (pa == 2) hipLaunchKernelGGL((dot_kernel<2, 1>), grid, block, 0, s, d);
+  else if (pb == 2) hipLaunchKernelGGL((dot_kernel<1, 2>), grid, block, 0, s, d);
+  else hipLaunchKernelGGL((dot_kernel<1, 1>), grid, block, 0, s, d);
+  const int64_t n = (int64_t)sc.M * sc.N;
+  hipLaunchKernelGGL(dot_reduce_kernel, dim3((unsigned)((n + 3) / 4)), block, 0, s, d, sc);
 }
 
 }  // namespace tdcc

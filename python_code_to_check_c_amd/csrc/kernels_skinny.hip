@@ -122,10 +122,25 @@ __global__ __launch_bounds__(256) void skinny_rows_kernel(SkinnyArgs a) {
 }
 
 // The big operand's contiguous index lies in k: one wave per big row, lanes
-// over k, then a lane reduction.  4 rows per block share the LDS chunk.
+// over k, then a lane reduction.  4 rows per block share the LDS chunk of the
+// small operand, which holds KC k (32 KB); lane l takes k = k0 + l + 64u of the
+// chunk, u < KC / 64, in order, with the offsets of all its k and then eight
+// big-operand loads at a time in flight.
+constexpr int sk_kchunk(bool sc, int it) {
+  const int kc = 32768 / (it * 8 * (sc ? 2 : 1)) / 64 * 64;
+  return kc < 64 ? 64 : (kc > 1024 ? 1024 : kc);
+}
+
+template <bool SC, int IT, int KC>
+struct SmallChunkK {
+  double re[IT][KC + 1];
+  double im[SC ? IT : 1][KC + 1];
+};
+
 template <bool SC, bool GC, int IT>
 __global__ __launch_bounds__(256) void skinny_k_kernel(SkinnyArgs a) {
-  __shared__ SmallChunk<SC, IT> sm;
+  constexpr int KC = sk_kchunk(SC, IT), KU = KC / 64;
+  __shared__ SmallChunkK<SC, IT, KC> sm;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int64_t j = (int64_t)blockIdx.x * 4 + w;
   const bool live = j < a.J;
@@ -135,23 +150,48 @@ __global__ __launch_bounds__(256) void skinny_k_kernel(SkinnyArgs a) {
   double acc_re[IT], acc_im[IT];
 #pragma unroll
   for (int i = 0; i < IT; ++i) acc_re[i] = acc_im[i] = 0.0;
-  for (int64_t k0 = kb; k0 < ke; k0 += SK_KC) {
+  for (int64_t k0 = kb; k0 < ke; k0 += KC) {
     __syncthreads();
-    load_small<SC, IT>(sm, a, k0, ke);
+    for (int e = threadIdx.x; e < IT * KC; e += blockDim.x) {
+      const int i = e / KC, kk = e - i * KC;
+      const int64_t k = k0 + kk;
+      const bool ok = i < a.I && k < ke;
+      const int64_t off = ok ? a.sr[i] + a.sk[k] : 0;
+      sm.re[i][kk] = ok ? a.s_re[off] : 0.0;
+      if (SC) sm.im[i][kk] = ok ? a.s_im[off] : 0.0;
+    }
     __syncthreads();
-    const int64_t k = k0 + lane;
-    if (!live || k >= ke) continue;
-    const int64_t go = a.gk[k];
-    const double br = gre[go];
-    const double bi = GC ? gim[go] : 0.0;
+    if (!live) continue;
+    int64_t go[KU];
 #pragma unroll
-    for (int i = 0; i < IT; ++i) {
-      const double sr = sm.re[i][lane];
-      const double si = SC ? sm.im[i][lane] : 0.0;
-      acc_re[i] = fma(sr, br, acc_re[i]);
-      if (SC && GC) acc_re[i] = fma(-si, bi, acc_re[i]);
-      if (GC) acc_im[i] = fma(sr, bi, acc_im[i]);
-      if (SC) acc_im[i] = fma(si, br, acc_im[i]);
+    for (int u = 0; u < KU; ++u) {
+      const int64_t k = k0 + 64 * u + lane;
+      go[u] = k < ke ? a.gk[k] : -1;
+    }
+#pragma unroll
+    for (int u0 = 0; u0 < KU; u0 += 8) {
+      double br[8], bi[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const bool ok = u0 + u < KU && go[u0 + u < KU ? u0 + u : 0] >= 0;
+        const int64_t o = ok ? go[u0 + u] : 0;
+        br[u] = ok ? gre[o] : 0.0;
+        bi[u] = (GC && ok) ? gim[o] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (u0 + u >= KU) break;
+        const int kk = 64 * (u0 + u) + lane;
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+          const double sr = sm.re[i][kk];
+          const double si = SC ? sm.im[i][kk] : 0.0;
+          acc_re[i] = fma(sr, br[u], acc_re[i]);
+          if (SC && GC) acc_re[i] = fma(-si, bi[u], acc_re[i]);
+          if (GC) acc_im[i] = fma(sr, bi[u], acc_im[i]);
+          if (SC) acc_im[i] = fma(si, br[u], acc_im[i]);
+        }
+      }
     }
   }
   // butterfly reduction: every lane ends with the same, order-fixed sum

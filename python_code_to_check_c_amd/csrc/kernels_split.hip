@@ -543,6 +543,11 @@ __global__ __launch_bounds__(256) void ozgemm_kernel(OzGemmArgs g) {
 // (checked against the 16x16x64 kernel's output).  Operands are read with
 // buffer loads from rows clamped to the matrix (the extra rows only reach
 // outputs that are not stored); LDS layout and swizzle as above.
+// With few K steps per tile the blocks are persistent: block b takes tiles b,
+// b + gridDim.x, ... (the same XCD for a grid that is a multiple of 8), and
+// the first K step of its next tile is loaded while the epilogue of the
+// current one stores (at 3 K steps per tile 8-17 % faster than one block per
+// tile).
 // ---------------------------------------------------------------------------
 template <int S>
 __global__ __launch_bounds__(512, 1) void ozgemm8_kernel(OzGemmArgs g) {
@@ -551,10 +556,7 @@ __global__ __launch_bounds__(512, 1) void ozgemm8_kernel(OzGemmArgs g) {
   __shared__ v4i Bs[2][S][BN * 4];
   __shared__ double sA[OZ_BM], sB[BN];
   const int tiles_m = (g.M + OZ_BM - 1) / OZ_BM, tiles_n = (g.N + BN - 1) / BN;
-  const int L = remap_tile(blockIdx.x, tiles_m * tiles_n);
-  int tm, tn;
-  grouped_tile(L, tiles_m, tiles_n, g.tile_group, tm, tn);
-  const int m0 = tm * OZ_BM, n0 = tn * BN;
+  const int tiles = tiles_m * tiles_n;
   const int tid = threadIdx.x;
   const int kb = g.k_chunk > 0 ? (int)blockIdx.y * g.k_chunk : 0;
   const int ke = g.k_chunk > 0 ? min(g.Kp, kb + g.k_chunk) : g.Kp;
@@ -563,9 +565,17 @@ __global__ __launch_bounds__(512, 1) void ozgemm8_kernel(OzGemmArgs g) {
       __builtin_amdgcn_make_buffer_rsrc((void*)g.A, (short)0, (int)(S * g.a_plane), 0x00020000);
   const __amdgpu_buffer_rsrc_t rB =
       __builtin_amdgcn_make_buffer_rsrc((void*)g.B, (short)0, (int)(S * g.b_plane), 0x00020000);
-  // one 16-byte chunk of A and one of B per thread and digit
-  const int va = min(m0 + (tid >> 2), g.M - 1) * g.Kp + (tid & 3) * 16;
-  const int vb = min(n0 + (tid >> 2), g.N - 1) * g.Kp + (tid & 3) * 16;
+  // tile origin and the thread's 16-byte chunk of A and of B per digit
+  int m0 = 0, n0 = 0, va = 0, vb = 0;
+  auto setup = [&](int t) {
+    const int L = remap_tile(t, tiles);
+    int tm, tn;
+    grouped_tile(L, tiles_m, tiles_n, g.tile_group, tm, tn);
+    m0 = tm * OZ_BM;
+    n0 = tn * BN;
+    va = min(m0 + (tid >> 2), g.M - 1) * g.Kp + (tid & 3) * 16;
+    vb = min(n0 + (tid >> 2), g.N - 1) * g.Kp + (tid & 3) * 16;
+  };
   v4i ra[S], rb[S];
   auto load_global = [&](int k0) {
 #pragma unroll
@@ -581,102 +591,117 @@ __global__ __launch_bounds__(512, 1) void ozgemm8_kernel(OzGemmArgs g) {
       Bs[buf][s][oz_swz(tid >> 2, tid & 3)] = rb[s];
     }
   };
-  v16i acc[S][2];
-#pragma unroll
-  for (int q = 0; q < S; ++q)
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int e = 0; e < 16; ++e) acc[q][i][e] = 0;
   const int lane = tid & 63, w = tid >> 6;
   const int wm = (w & 1) * 64, wn = (w >> 1) * 32;
   const int lr = lane & 31, lh = lane >> 5;
-
-  load_global(kb);
-  if (tid < OZ_BM) sA[tid] = m0 + tid < g.M ? g.sa[m0 + tid] : 0.0;
-  else if (tid - OZ_BM < BN) sB[tid - OZ_BM] = n0 + tid - OZ_BM < g.N ? g.sb[n0 + tid - OZ_BM] : 0.0;
-  store_lds(0);
-  __syncthreads();
-  int buf = 0;
-  for (int k0 = kb; k0 < ke; k0 += OZ_BK) {
-    const bool more = k0 + OZ_BK < ke;
-    if (more) load_global(k0 + OZ_BK);
-#pragma unroll
-    for (int kk = 0; kk < 2; ++kk) {
-      v4i a[S][2], b[S];
-#pragma unroll
-      for (int s = 0; s < S; ++s) {
-#pragma unroll
-        for (int i = 0; i < 2; ++i) a[s][i] = As[buf][s][oz_swz(wm + i * 32 + lr, 2 * kk + lh)];
-        b[s] = Bs[buf][s][oz_swz(wn + lr, 2 * kk + lh)];
-      }
-#pragma unroll
-      for (int t = 0; t < S; ++t)
-#pragma unroll
-        for (int s = 0; s + t < S; ++s)
-#pragma unroll
-          for (int i = 0; i < 2; ++i)
-            acc[s + t][i] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[s][i], b[t], acc[s + t][i], 0, 0, 0);
-    }
-    if (more) store_lds(buf ^ 1);
-    __syncthreads();
-    buf ^= 1;
-  }
-  // digit group q (pairs with s + t = q + 2, 1-based digits) weighs 128^-(q+2)
-  const int col = n0 + wn + lr;
-  const bool colok = col < g.N;
-  const double sbc = sB[wn + lr];
   const double ar = g.alpha_re, ai = g.alpha_im;
+
+  int t = blockIdx.x;
+  setup(t);
+  load_global(kb);
+  for (;;) {
+    v16i acc[S][2];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+    for (int q = 0; q < S; ++q)
 #pragma unroll
-    for (int gq = 0; gq < 4; ++gq) {
-      // rows row0 .. row0 + 3 of this lane; row0 is even
-      const int row0 = m0 + wm + i * 32 + 8 * gq + 4 * lh;
-      double z[4];
+      for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        double v = 0.0;
+        for (int e = 0; e < 16; ++e) acc[q][i][e] = 0;
+    if (tid < OZ_BM) sA[tid] = m0 + tid < g.M ? g.sa[m0 + tid] : 0.0;
+    else if (tid - OZ_BM < BN) sB[tid - OZ_BM] = n0 + tid - OZ_BM < g.N ? g.sb[n0 + tid - OZ_BM] : 0.0;
+    store_lds(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = kb; k0 < ke; k0 += OZ_BK) {
+      const bool more = k0 + OZ_BK < ke;
+      if (more) load_global(k0 + OZ_BK);
 #pragma unroll
-        for (int q = S - 1; q >= 0; --q) v += ldexp((double)acc[q][i][4 * gq + r], -7 * (q + 2));
-        z[r] = (colok && row0 + r < g.M) ? v * (sA[row0 + r - m0] * sbc) : 0.0;
-      }
-      if (g.epi < 0) {
+      for (int kk = 0; kk < 2; ++kk) {
+        v4i a[S][2], b[S];
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-          if (colok && row0 + r < g.M) Cout[(int64_t)(row0 + r) * g.ldc + col] = z[r];
-      } else if (g.epi == 0) {
+        for (int s = 0; s < S; ++s) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-          if (colok && row0 + r < g.M)
-            store_mirrored(g.out_re, nullptr, g.cm[row0 + r], g.cn[col], g.cm2 ? g.cm2[row0 + r] : -1,
-                           g.cn2 ? g.cn2[col] : -1, ar * z[r], 0.0);
-      } else if (g.epi == 1) {
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          if (!colok || row0 + 2 * h >= g.M) continue;
-          const double re = z[2 * h], im = z[2 * h + 1];
-          const int mr = (row0 >> 1) + h;
-          store_mirrored(g.out_re, g.out_im, g.cm[mr], g.cn[col], g.cm2 ? g.cm2[mr] : -1, g.cn2 ? g.cn2[col] : -1,
-                         ar * re - ai * im, ar * im + ai * re);
+          for (int i = 0; i < 2; ++i) a[s][i] = As[buf][s][oz_swz(wm + i * 32 + lr, 2 * kk + lh)];
+          b[s] = Bs[buf][s][oz_swz(wn + lr, 2 * kk + lh)];
         }
-      } else {
-        double x[4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) x[r] = __shfl_xor(z[r], 1);
-        if ((lr & 1) == 0 && colok) {
+        for (int tt = 0; tt < S; ++tt)
+#pragma unroll
+          for (int s = 0; s + tt < S; ++s)
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+              acc[s + tt][i] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[s][i], b[tt], acc[s + tt][i], 0, 0, 0);
+      }
+      if (more) store_lds(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+    }
+    // this tile's origin; the next tile's first K step goes in flight now
+    const int em0 = m0, en0 = n0;
+    const int tnext = t + (int)gridDim.x;
+    if (tnext < tiles) {
+      setup(tnext);
+      load_global(kb);
+    }
+    // digit group q (pairs with s + t = q + 2, 1-based digits) weighs 128^-(q+2)
+    const int col = en0 + wn + lr;
+    const bool colok = col < g.N;
+    const double sbc = sB[wn + lr];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int gq = 0; gq < 4; ++gq) {
+        // rows row0 .. row0 + 3 of this lane; row0 is even
+        const int row0 = em0 + wm + i * 32 + 8 * gq + 4 * lh;
+        double z[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          double v = 0.0;
+#pragma unroll
+          for (int q = S - 1; q >= 0; --q) v += ldexp((double)acc[q][i][4 * gq + r], -7 * (q + 2));
+          z[r] = (colok && row0 + r < g.M) ? v * (sA[row0 + r - em0] * sbc) : 0.0;
+        }
+        if (g.epi < 0) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (colok && row0 + r < g.M) Cout[(int64_t)(row0 + r) * g.ldc + col] = z[r];
+        } else if (g.epi == 0) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (colok && row0 + r < g.M)
+              store_mirrored(g.out_re, nullptr, g.cm[row0 + r], g.cn[col], g.cm2 ? g.cm2[row0 + r] : -1,
+                             g.cn2 ? g.cn2[col] : -1, ar * z[r], 0.0);
+        } else if (g.epi == 1) {
 #pragma unroll
           for (int h = 0; h < 2; ++h) {
-            if (row0 + 2 * h >= g.M) continue;
-            const double re = z[2 * h] - x[2 * h + 1];  // rr - ii
-            const double im = x[2 * h] + z[2 * h + 1];  // ri + ir
-            const int mr = (row0 >> 1) + h, nc = col >> 1;
-            store_mirrored(g.out_re, g.out_im, g.cm[mr], g.cn[nc], g.cm2 ? g.cm2[mr] : -1, g.cn2 ? g.cn2[nc] : -1,
+            if (!colok || row0 + 2 * h >= g.M) continue;
+            const double re = z[2 * h], im = z[2 * h + 1];
+            const int mr = (row0 >> 1) + h;
+            store_mirrored(g.out_re, g.out_im, g.cm[mr], g.cn[col], g.cm2 ? g.cm2[mr] : -1, g.cn2 ? g.cn2[col] : -1,
                            ar * re - ai * im, ar * im + ai * re);
+          }
+        } else {
+          double x[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) x[r] = __shfl_xor(z[r], 1);
+          if ((lr & 1) == 0 && colok) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+              if (row0 + 2 * h >= g.M) continue;
+              const double re = z[2 * h] - x[2 * h + 1];  // rr - ii
+              const double im = x[2 * h] + z[2 * h + 1];  // ri + ir
+              const int mr = (row0 >> 1) + h, nc = col >> 1;
+              store_mirrored(g.out_re, g.out_im, g.cm[mr], g.cn[nc], g.cm2 ? g.cm2[mr] : -1, g.cn2 ? g.cn2[nc] : -1,
+                             ar * re - ai * im, ar * im + ai * re);
+            }
           }
         }
       }
-    }
+    if (tnext >= tiles) break;
+    // every wave has read this tile's scales before they are replaced
+    __syncthreads();
+    t = tnext;
+  }
 }
 
 int64_t ozgemm_max_k(int S) {
@@ -698,17 +723,39 @@ static int oz_tile_group(int tiles_m, int tiles_n) {
   return tiles_n >= 40 ? 4 : tiles_m;
 }
 
+// Blocks of the persistent 4-digit kernel with one split: one per CU (the
+// kernel's LDS holds one block per CU).  TDCC_OZ_PERSIST overrides it: 0 gives
+// one block per tile, n > 0 about n blocks (a multiple of 8).
+static int oz_persistent_blocks() {
+  if (const char* e = std::getenv("TDCC_OZ_PERSIST")) {
+    const int n = std::atoi(e);
+    return n <= 0 ? 0 : std::max(8, n / 8 * 8);
+  }
+  static const int cus = [] {
+    int dev = 0, c = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      return 0;
+    return c / 8 * 8;
+  }();
+  return cus;
+}
+
 void launch_ozgemm(const OzGemmArgs& args, hipStream_t s) {
   if (args.M <= 0 || args.N <= 0) return;
   OzGemmArgs g = args;
   g.tile_group = oz_tile_group((g.M + OZ_BM - 1) / OZ_BM, (g.N + 127) / 128);
   const int bn = g.S <= 4 ? 128 : 64;
   const int tiles = ((g.M + OZ_BM - 1) / OZ_BM) * ((g.N + bn - 1) / bn);
-  const dim3 grid(tiles, g.k_chunk > 0 ? g.splits : 1), block(256), block8(512);
+  const int splits = g.k_chunk > 0 ? g.splits : 1;
+  const int pb = oz_persistent_blocks();
+  // persistent only with few K steps per tile (at 50 K steps one block per
+  // tile measured 12 % faster: the dispatcher balances long tiles better)
+  const int gx8 = (g.S <= 4 && splits == 1 && pb > 0 && tiles > pb && g.Kp <= 8 * OZ_BK) ? pb : tiles;
+  const dim3 grid(tiles, splits), grid8(gx8, splits), block(256), block8(512);
   switch (g.S) {
-    case 2: hipLaunchKernelGGL((ozgemm8_kernel<2>), grid, block8, 0, s, g); break;
-    case 3: hipLaunchKernelGGL((ozgemm8_kernel<3>), grid, block8, 0, s, g); break;
-    case 4: hipLaunchKernelGGL((ozgemm8_kernel<4>), grid, block8, 0, s, g); break;
+    case 2: hipLaunchKernelGGL((ozgemm8_kernel<2>), grid8, block8, 0, s, g); break;
+    case 3: hipLaunchKernelGGL((ozgemm8_kernel<3>), grid8, block8, 0, s, g); break;
+    case 4: hipLaunchKernelGGL((ozgemm8_kernel<4>), grid8, block8, 0, s, g); break;
     case 5: hipLaunchKernelGGL((ozgemm_kernel<5, 2>), grid, block, 0, s, g); break;
     case 6: hipLaunchKernelGGL((ozgemm_kernel<6, 2>), grid, block, 0, s, g); break;
     default: break;

@@ -133,6 +133,51 @@ def test_shared_digits_are_bitwise_neutral(case):
         assert np.array_equal(x, y)
 
 
+def test_persistent_digit_gemm_is_bitwise_neutral():
+    """The 4-digit GEMM runs persistent blocks that loop over the tiles
+    (TDCC_OZ_PERSIST blocks; 0: one block per tile).  A tile's arithmetic does
+    not depend on the block that computes it, so at a size with 64 tiles per
+    GEMM (o = 8, v = 64, every contraction on the GEMM path) 8 looping blocks
+    and one block per tile must give bitwise equal residuals."""
+    from python_code_to_check_c_amd import synthetic
+    o, v = 8, 64
+    sysm = synthetic.SyntheticSystem(3, o, v, with_tei=False)
+    rng = np.random.default_rng(5)
+
+    def amp(shape):
+        x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+        return 1e-2 * x
+
+    t1, l1 = amp((o, v)), amp((o, v))
+    t2, l2 = amp((o, o, v, v)), amp((o, o, v, v))
+    for x in (t2, l2):
+        x -= x.transpose(1, 0, 2, 3)
+        x -= x.transpose(0, 1, 3, 2)
+    Fc = sysm.F + 0j
+    out = {}
+    saved = {k: os.environ.get(k) for k in ("TDCC_OZ_PERSIST", "TDCC_GEMM_MIN_FLOPS")}
+    try:
+        os.environ["TDCC_GEMM_MIN_FLOPS"] = "0"
+        for blocks in ("8", "0"):
+            os.environ["TDCC_OZ_PERSIST"] = blocks
+            c = _lib.Context(o, v)
+            try:
+                c.set_system(None, np.asarray(sysm.F), np.asarray(sysm.mu))
+                c.synth_tei(3, 0.05)
+                c.set_precision(4)
+                out[blocks] = [c.rhs(kind, Fc, t1, t2, l1, l2) for _, kind in RHS]
+            finally:
+                c.close()
+    finally:
+        for k, val in saved.items():
+            if val is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = val
+    for (name, _), x, y in zip(RHS, out["8"], out["0"]):
+        assert np.array_equal(x, y), name
+
+
 def test_back_to_fp64(case):
     g, ctx = case
     ctx.set_precision(5)
