@@ -751,6 +751,7 @@ void Graph::plan() {
     inputs_of(nodes_[i], in);
     for (int x : in) live[x] = 1;
   }
+  inline_lincombs(live);
   live_ = live;
   // liveness: uses of views count as uses of their base
   auto root = [&](int x) {
@@ -846,6 +847,87 @@ void Graph::plan() {
     cache_buf_ = nullptr;
   }
   if (cache_bytes_ > 0) TDCC_HIP(hipMalloc(&cache_buf_, cache_bytes_));
+}
+
+// A linear combination S whose only consumer is another linear combination L
+// is folded into L (its terms composed into L's, equal source readings merged)
+// while L stays within one launch, so S is neither written nor read back.
+// For an antisymmetric L (summed at its canonical elements, where the terms
+// reading one source through the antisymmetrizer's permutations cover it once)
+// every reading of S is folded; otherwise only an S that L reads once.
+// TDCC_INLINE_LINCOMB=0 disables it.
+void Graph::inline_lincombs(std::vector<char>& live) {
+  if (const char* e = std::getenv("TDCC_INLINE_LINCOMB"); e && std::atoi(e) == 0) return;
+  const int N = (int)nodes_.size();
+  std::vector<int> users(N, 0), user_of(N, -1);
+  std::vector<int> in;
+  for (int i = 0; i < N; ++i) {
+    if (!live[i]) continue;
+    inputs_of(nodes_[i], in);
+    std::sort(in.begin(), in.end());
+    in.erase(std::unique(in.begin(), in.end()), in.end());
+    for (int x : in) {
+      ++users[x];
+      user_of[x] = i;
+    }
+  }
+  for (int i = 0; i < N; ++i) {
+    if (!live[i] || nodes_[i].op != Op::Lincomb) continue;
+    for (bool changed = true; changed;) {
+      changed = false;
+      Node& L = nodes_[i];
+      const bool anti = L.shape.rank == 4 && (L.anti & (anti_bit(0, 1) | anti_bit(2, 3)));
+      for (size_t t = 0; t < L.terms.size() && !changed; ++t) {
+        const int s = L.terms[t].src;
+        const Node& S = nodes_[s];
+        if (S.op != Op::Lincomb || users[s] != 1 || user_of[s] != i) continue;
+        int refs = 0;
+        for (const Term& T : L.terms) refs += T.src == s;
+        if (!anti && refs > 1) continue;
+        std::vector<Term> nt;
+        auto put = [&](const Term& c) {
+          for (Term& u : nt)
+            if (u.src == c.src && std::equal(u.perm, u.perm + 4, c.perm)) {
+              u.cr += c.cr;
+              u.ci += c.ci;
+              return;
+            }
+          nt.push_back(c);
+        };
+        for (const Term& T : L.terms) {
+          if (T.src != s) {
+            put(T);
+            continue;
+          }
+          for (const Term& u : S.terms) {
+            Term c;
+            c.src = u.src;
+            for (int d = 0; d < 4; ++d) c.perm[d] = u.perm[T.perm[d]];
+            c.cr = T.cr * u.cr - T.ci * u.ci;
+            c.ci = T.cr * u.ci + T.ci * u.cr;
+            put(c);
+          }
+        }
+        if (nt.size() > (size_t)kMaxTerms) continue;
+        // S disappears and L reads its sources: a source that L read already
+        // loses one user, any other keeps its count with L as the reader
+        std::vector<int> lin, sin;
+        inputs_of(L, lin);
+        std::sort(lin.begin(), lin.end());
+        inputs_of(S, sin);
+        std::sort(sin.begin(), sin.end());
+        sin.erase(std::unique(sin.begin(), sin.end()), sin.end());
+        L.terms = std::move(nt);
+        for (int x : sin) {
+          if (std::binary_search(lin.begin(), lin.end(), x)) --users[x];
+          user_of[x] = i;
+        }
+        live[s] = 0;
+        users[s] = 0;
+        changed = true;
+      }
+    }
+  }
 }
 
 // Groups the launches that split one computed operand in one layout into

@@ -285,6 +285,7 @@ class Graph {
   };
   std::map<std::string, std::vector<DynUse>> dyn_uses_;  // layout key -> launches in node order
   void plan_shares();
+  void inline_lincombs(std::vector<char>& live);
   int64_t cache_bytes_ = 0;
   char* cache_buf_ = nullptr;
   double* ones_buf_ = nullptr;

@@ -59,7 +59,7 @@ void launch_gett_reduce(const GettArgs& g, bool c_cplx, hipStream_t s);
 // ---------------------------------------------------------------------------
 // Linear combination with per-term permuted strides:
 //   out[idx] (+)= sum_t coef_t * src_t[sum_d idx_d * st_t[d]]
-constexpr int kMaxTerms = 12;   // terms per launch
+constexpr int kMaxTerms = 24;   // terms per launch
 constexpr int kInlineTerms = 8;  // a source linear combination is inlined while the total stays within this
 struct LincombTerm {
   const double* re; const double* im;
