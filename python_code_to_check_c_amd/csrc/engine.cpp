@@ -674,7 +674,10 @@ static void inputs_of(const Node& n, std::vector<int>& in) {
   switch (n.op) {
     case Op::Contract: in = {n.a, n.b}; break;
     case Op::Lincomb:
-      for (const Term& t : n.terms) in.push_back(t.src);
+      for (const Term& t : n.terms) {
+        in.push_back(t.src);
+        if (t.src2 >= 0) in.push_back(t.src2);
+      }
       break;
     case Op::View: in = {n.base}; break;
     case Op::Finalize: in = {n.k[0], n.k[1], n.k[2], n.k[3], n.a, n.b}; break;
@@ -751,6 +754,7 @@ void Graph::plan() {
     inputs_of(nodes_[i], in);
     for (int x : in) live[x] = 1;
   }
+  fold_outer_products(live);
   inline_lincombs(live);
   live_ = live;
   // liveness: uses of views count as uses of their base
@@ -849,6 +853,77 @@ void Graph::plan() {
   if (cache_bytes_ > 0) TDCC_HIP(hipMalloc(&cache_buf_, cache_bytes_));
 }
 
+// A term of a linear combination that reads an outer product of two rank-2
+// tensors (a contraction 'ia,jb->ijab' with nothing summed, such as t1 t1) is
+// replaced by a product term reading the two factors, so the outer product is
+// computed where it is added instead of being written and read back; the
+// contraction is dropped when nothing else reads it.  TDCC_FOLD_OUTER=0
+// disables it.
+void Graph::fold_outer_products(std::vector<char>& live) {
+  if (const char* e = std::getenv("TDCC_FOLD_OUTER"); e && std::atoi(e) == 0) return;
+  const int N = (int)nodes_.size();
+  std::vector<int> users(N, 0), in;
+  auto unique_inputs = [&](const Node& n, std::vector<int>& v) {
+    inputs_of(n, v);
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+  };
+  for (int i = 0; i < N; ++i) {
+    if (!live[i]) continue;
+    unique_inputs(nodes_[i], in);
+    for (int x : in) ++users[x];
+  }
+  auto outer = [&](int c, std::string& la, std::string& lb, std::string& lc) {
+    const Node& C = nodes_[c];
+    if (C.op != Op::Contract || C.shape.rank != 4 || !C.kpair.empty()) return false;
+    if (nodes_[C.a].shape.rank != 2 || nodes_[C.b].shape.rank != 2) return false;
+    std::vector<std::string> ins;
+    parse_spec(C.spec, ins, lc);
+    if (ins.size() != 2) return false;
+    la = ins[0];
+    lb = ins[1];
+    if (la.size() != 2 || lb.size() != 2 || lc.size() != 4) return false;
+    for (char x : la)
+      if (lb.find(x) != std::string::npos) return false;
+    return true;
+  };
+  std::vector<int> before, after;
+  for (int i = 0; i < N; ++i) {
+    if (!live[i] || nodes_[i].op != Op::Lincomb || nodes_[i].shape.rank != 4) continue;
+    Node& L = nodes_[i];
+    unique_inputs(L, before);
+    bool any = false;
+    for (Term& T : L.terms) {
+      std::string la, lb, lc;
+      if (T.src2 >= 0 || !outer(T.src, la, lb, lc)) continue;
+      const Node& C = nodes_[T.src];
+      Term U;
+      U.src = C.a;
+      U.src2 = C.b;
+      for (int d = 0; d < 4; ++d) {
+        const char ch = lc[T.perm[d]];
+        const size_t pa = la.find(ch), pb = lb.find(ch);
+        U.perm[d] = pa == std::string::npos ? -1 : (int)pa;
+        U.perm2[d] = pb == std::string::npos ? -1 : (int)pb;
+      }
+      U.cr = T.cr * C.ar - T.ci * C.ai;
+      U.ci = T.cr * C.ai + T.ci * C.ar;
+      T = U;
+      any = true;
+    }
+    if (!any) continue;
+    unique_inputs(L, after);
+    for (int x : before)
+      if (!std::binary_search(after.begin(), after.end(), x)) --users[x];
+    for (int x : after)
+      if (!std::binary_search(before.begin(), before.end(), x)) ++users[x];
+  }
+  for (int c = 0; c < N; ++c) {
+    std::string la, lb, lc;
+    if (live[c] && users[c] == 0 && outer(c, la, lb, lc)) live[c] = 0;
+  }
+}
+
 // A linear combination S whose only consumer is another linear combination L
 // is folded into L (its terms composed into L's, equal source readings merged)
 // while L stays within one launch, so S is neither written nor read back.
@@ -881,13 +956,19 @@ void Graph::inline_lincombs(std::vector<char>& live) {
         const int s = L.terms[t].src;
         const Node& S = nodes_[s];
         if (S.op != Op::Lincomb || users[s] != 1 || user_of[s] != i) continue;
+        // (a factor of a product term stays a tensor of its own)
         int refs = 0;
-        for (const Term& T : L.terms) refs += T.src == s;
-        if (!anti && refs > 1) continue;
+        bool factor = false;
+        for (const Term& T : L.terms) {
+          refs += T.src == s;
+          factor = factor || (T.src2 >= 0 && (T.src == s || T.src2 == s));
+        }
+        if (factor || (!anti && refs > 1)) continue;
         std::vector<Term> nt;
         auto put = [&](const Term& c) {
           for (Term& u : nt)
-            if (u.src == c.src && std::equal(u.perm, u.perm + 4, c.perm)) {
+            if (u.src == c.src && u.src2 == c.src2 && std::equal(u.perm, u.perm + 4, c.perm) &&
+                std::equal(u.perm2, u.perm2 + 4, c.perm2)) {
               u.cr += c.cr;
               u.ci += c.ci;
               return;
@@ -902,7 +983,11 @@ void Graph::inline_lincombs(std::vector<char>& live) {
           for (const Term& u : S.terms) {
             Term c;
             c.src = u.src;
-            for (int d = 0; d < 4; ++d) c.perm[d] = u.perm[T.perm[d]];
+            c.src2 = u.src2;
+            for (int d = 0; d < 4; ++d) {
+              c.perm[d] = u.perm[T.perm[d]];
+              c.perm2[d] = u.src2 >= 0 ? u.perm2[T.perm[d]] : -1;
+            }
             c.cr = T.cr * u.cr - T.ci * u.ci;
             c.ci = T.cr * u.ci + T.ci * u.cr;
             put(c);
@@ -1001,8 +1086,11 @@ void Graph::dump_nodes() const {
     if (n.op == Op::View) std::fprintf(stderr, " base=%d", n.base);
     if (n.op == Op::Ladder) std::fprintf(stderr, " u=%d", n.a);
     if (n.op == Op::Lincomb)
-      for (const Term& t : n.terms)
-        std::fprintf(stderr, " %d(%d%d%d%d)*(%g,%g)", t.src, t.perm[0], t.perm[1], t.perm[2], t.perm[3], t.cr, t.ci);
+      for (const Term& t : n.terms) {
+        std::fprintf(stderr, " %d(%d%d%d%d)", t.src, t.perm[0], t.perm[1], t.perm[2], t.perm[3]);
+        if (t.src2 >= 0) std::fprintf(stderr, "x%d(%d%d%d%d)", t.src2, t.perm2[0], t.perm2[1], t.perm2[2], t.perm2[3]);
+        std::fprintf(stderr, "*(%g,%g)", t.cr, t.ci);
+      }
     if (n.op == Op::External) std::fprintf(stderr, " %s", n.key.c_str());
     std::fprintf(stderr, "\n");
   }
@@ -1732,9 +1820,18 @@ void Graph::bind(double* arena) {
             const Dev S = resolve(T.src);
             l.t[k].re = S.re;
             l.t[k].im = nodes_[T.src].cplx ? S.im : nullptr;
-            for (int d = 0; d < 4; ++d) l.t[k].st[d] = d < n.shape.rank ? S.st[T.perm[d]] : 0;
+            for (int d = 0; d < 4; ++d) l.t[k].st[d] = d < n.shape.rank && T.perm[d] >= 0 ? S.st[T.perm[d]] : 0;
             l.t[k].cr = T.cr;
             l.t[k].ci = T.ci;
+            l.t[k].re2 = nullptr;
+            l.t[k].im2 = nullptr;
+            for (int d = 0; d < 4; ++d) l.t[k].st2[d] = 0;
+            if (T.src2 >= 0) {
+              const Dev S2 = resolve(T.src2);
+              l.t[k].re2 = S2.re;
+              l.t[k].im2 = nodes_[T.src2].cplx ? S2.im : nullptr;
+              for (int d = 0; d < 4; ++d) l.t[k].st2[d] = d < n.shape.rank && T.perm2[d] >= 0 ? S2.st[T.perm2[d]] : 0;
+            }
           }
           launches_.push_back(std::move(L));
         }
@@ -2132,7 +2229,8 @@ void Graph::run_profiled(hipStream_t s, std::vector<LaunchRecord>& out) const {
                      ((L->l.anti & 2) ? d3 * (d3 + 1) / 2 : (double)L->l.dims[2] * d3);
           }
           double term_planes = 0.0;
-          for (int k = 0; k < L->l.nterms; ++k) term_planes += L->l.t[k].im ? 2.0 : 1.0;
+          for (int k = 0; k < L->l.nterms; ++k)
+            if (!L->l.t[k].re2) term_planes += L->l.t[k].im ? 2.0 : 1.0;  // product terms: two small factors
           const double bytes =
               8.0 * ((double)L->l.total * (L->l.out_im ? 2 : 1) * (1 + L->l.accumulate) + summed * term_planes);
           char dt[160];

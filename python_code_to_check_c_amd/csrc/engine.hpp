@@ -43,6 +43,11 @@ struct Term {
   int src;
   int perm[4];  // output axis d reads source axis perm[d]
   double cr, ci;
+  // product term (outer product of two tensors folded in at planning): the
+  // value is c * src * src2, output axis d reading axis perm[d] of src and
+  // perm2[d] of src2 (-1: not an axis of that factor)
+  int src2 = -1;
+  int perm2[4] = {-1, -1, -1, -1};
 };
 
 struct Node {
@@ -286,6 +291,7 @@ class Graph {
   std::map<std::string, std::vector<DynUse>> dyn_uses_;  // layout key -> launches in node order
   void plan_shares();
   void inline_lincombs(std::vector<char>& live);
+  void fold_outer_products(std::vector<char>& live);
   int64_t cache_bytes_ = 0;
   char* cache_buf_ = nullptr;
   double* ones_buf_ = nullptr;

@@ -288,6 +288,21 @@ void launch_gett_reduce(const GettArgs& g, bool c_cplx, hipStream_t s) {
 // product and sum individually rounded (no contraction into FMA), which is
 // how numpy evaluates `y + c * k`.
 // ===========================================================================
+// Value of term T at source offsets so (and so2 for a product term): x, or
+// x * y for a product term, each product and sum individually rounded.
+__device__ __forceinline__ void lincomb_term_value(const LincombTerm& T, int64_t so, int64_t so2, double& xr,
+                                                   double& xi) {
+  xr = T.re[so];
+  xi = T.im ? T.im[so] : 0.0;
+  if (T.re2) {
+    const double yr = T.re2[so2], yi = T.im2 ? T.im2[so2] : 0.0;
+    const double pr = __dsub_rn(__dmul_rn(xr, yr), __dmul_rn(xi, yi));
+    const double pi = __dadd_rn(__dmul_rn(xr, yi), __dmul_rn(xi, yr));
+    xr = pr;
+    xi = pi;
+  }
+}
+
 __global__ void lincomb_kernel(LincombArgs a) {
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < a.total;
        idx += (int64_t)gridDim.x * blockDim.x) {
@@ -307,10 +322,13 @@ __global__ void lincomb_kernel(LincombArgs a) {
     }
     for (int t = 0; t < a.nterms; ++t) {
       const LincombTerm& T = a.t[t];
-      int64_t so = 0;
-      for (int d = 0; d < a.rank; ++d) so += ix[d] * T.st[d];
-      const double xr = T.re[so];
-      const double xi = T.im ? T.im[so] : 0.0;
+      int64_t so = 0, so2 = 0;
+      for (int d = 0; d < a.rank; ++d) {
+        so += ix[d] * T.st[d];
+        so2 += ix[d] * T.st2[d];
+      }
+      double xr, xi;
+      lincomb_term_value(T, so, so2, xr, xi);
       double pr, pi;
       if (T.ci == 0.0) {
         pr = __dmul_rn(T.cr, xr);
@@ -349,7 +367,7 @@ static int grid_for(int64_t total) {
 constexpr int kLinTile = 32;
 
 __device__ __forceinline__ bool lincomb_transposed(const LincombTerm& T, int R) {
-  return T.st[R - 1] != 1 && T.st[R - 2] == 1;
+  return !T.re2 && T.st[R - 1] != 1 && T.st[R - 2] == 1;
 }
 
 // Adds the terms of `a` over the 32 x 32 tile (r0.., c0..) of the two
@@ -365,8 +383,11 @@ __device__ __forceinline__ void lincomb_tile_terms(const LincombArgs& a, const i
   const int lx = threadIdx.x & 31, ly = threadIdx.x >> 5;
   for (int t = 0; t < a.nterms; ++t) {
     const LincombTerm& T = a.t[t];
-    int64_t sbase = 0;
-    for (int d = 0; d < nlead; ++d) sbase += lead[d] * T.st[d];
+    int64_t sbase = 0, sbase2 = 0;
+    for (int d = 0; d < nlead; ++d) {
+      sbase += lead[d] * T.st[d];
+      sbase2 += lead[d] * T.st2[d];
+    }
     double xr[4], xi[4];
     if (lincomb_transposed(T, R)) {
       // stage: thread (lx, ly) loads trailing-axis element (r0 + lx, c0 + ly + 8k)
@@ -390,9 +411,11 @@ __device__ __forceinline__ void lincomb_tile_terms(const LincombArgs& a, const i
       for (int k = 0; k < 4; ++k) {
         const int64_t r = r0 + ly + 8 * k, c = c0 + lx;
         const bool ok = r < D2 && c < D3;
-        const int64_t so = sbase + r * T.st[R - 2] + c * T.st[R - 1];
-        xr[k] = ok ? T.re[so] : 0.0;
-        xi[k] = (ok && T.im) ? T.im[so] : 0.0;
+        xr[k] = 0.0;
+        xi[k] = 0.0;
+        if (ok)
+          lincomb_term_value(T, sbase + r * T.st[R - 2] + c * T.st[R - 1],
+                             sbase2 + r * T.st2[R - 2] + c * T.st2[R - 1], xr[k], xi[k]);
       }
     }
 #pragma unroll
@@ -596,7 +619,7 @@ static bool lincomb_use_anti(const LincombArgs& a) {
 static bool lincomb_use_tiled(const LincombArgs& a) {
   if (a.rank < 2 || a.ost[a.rank - 1] != 1) return false;
   for (int t = 0; t < a.nterms; ++t)
-    if (a.t[t].st[a.rank - 1] != 1 && a.t[t].st[a.rank - 2] == 1) return true;
+    if (!a.t[t].re2 && a.t[t].st[a.rank - 1] != 1 && a.t[t].st[a.rank - 2] == 1) return true;
   return false;
 }
 

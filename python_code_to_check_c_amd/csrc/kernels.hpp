@@ -65,6 +65,9 @@ struct LincombTerm {
   const double* re; const double* im;
   int64_t st[4];
   double cr, ci;
+  // product term: coef * x * y with y = (re2, im2) read at st2 (re2 null: none)
+  const double* re2 = nullptr; const double* im2 = nullptr;
+  int64_t st2[4] = {0, 0, 0, 0};
 };
 struct LincombArgs {
   double* out_re; double* out_im;

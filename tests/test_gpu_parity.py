@@ -33,16 +33,18 @@ RHS = [("T1eq_rhs", _lib.RHS_T1), ("T2eq_rhs", _lib.RHS_T2), ("lam_1eq_rhs", _li
 # the antisymmetric amplitudes; antisymmetric linear combinations summed at
 # every element instead of the canonical ones; every dense-GEMM contraction
 # with at most 64 A rows and 512 B rows (planes counted) on the dot-product
-# kernel (by default only from K = 8192 on); every linear combination
-# materialized (by default one used only by another is folded into it).
+# kernel (by default only from K = 8192 on); every linear combination and
+# outer product materialized (by default a linear combination used only by
+# another is folded into it, and outer products of two matrices are formed
+# where a linear combination adds them).
 MODES = {"default": {}, "skinny": {"TDCC_SKINNY_MIN": "0"}, "gett": {"TDCC_SKINNY_MIN": "-1"},
          "gemm": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_NPAIR_MIN_K": "0"},
          "gemm_splitk": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_GEMM_SPLITS": "3"},
          "full_rows": {"TDCC_IJ_PAIRS": "0"}, "plain_lincomb": {"TDCC_LINCOMB_ANTI": "0"},
          "dot": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_DOT_MIN_K": "0"},
-         "no_inline": {"TDCC_INLINE_LINCOMB": "0"}}
+         "no_inline": {"TDCC_INLINE_LINCOMB": "0", "TDCC_FOLD_OUTER": "0"}}
 ENV_KEYS = ("TDCC_GEMM_MIN_FLOPS", "TDCC_GEMM_SPLITS", "TDCC_SKINNY_MIN", "TDCC_IJ_PAIRS", "TDCC_NPAIR_MIN_K",
-            "TDCC_LINCOMB_ANTI", "TDCC_DOT_MIN_K", "TDCC_INLINE_LINCOMB")
+            "TDCC_LINCOMB_ANTI", "TDCC_DOT_MIN_K", "TDCC_INLINE_LINCOMB", "TDCC_FOLD_OUTER")
 
 
 @pytest.fixture(scope="module", params=[(n, m) for m in MODES for n in ("o4v6", "c1_o10v4")],
