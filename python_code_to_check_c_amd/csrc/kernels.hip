@@ -303,15 +303,27 @@ __device__ __forceinline__ void lincomb_term_value(const LincombTerm& T, int64_t
   }
 }
 
+// Index decomposition in 32-bit arithmetic when the element count allows
+// (64-bit division is a long instruction sequence on the GPU).
+template <bool I32>
 __global__ void lincomb_kernel(LincombArgs a) {
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < a.total;
        idx += (int64_t)gridDim.x * blockDim.x) {
     int64_t ix[4] = {0, 0, 0, 0};
-    int64_t rem = idx;
-    for (int d = a.rank - 1; d >= 0; --d) {
-      const int64_t q = rem / a.dims[d];
-      ix[d] = rem - q * a.dims[d];
-      rem = q;
+    if (I32) {
+      uint32_t rem = (uint32_t)idx;
+      for (int d = a.rank - 1; d >= 0; --d) {
+        const uint32_t n = (uint32_t)a.dims[d], q = rem / n;
+        ix[d] = rem - q * n;
+        rem = q;
+      }
+    } else {
+      int64_t rem = idx;
+      for (int d = a.rank - 1; d >= 0; --d) {
+        const int64_t q = rem / a.dims[d];
+        ix[d] = rem - q * a.dims[d];
+        rem = q;
+      }
     }
     int64_t oo = 0;
     for (int d = 0; d < a.rank; ++d) oo += ix[d] * a.ost[d];
@@ -348,6 +360,90 @@ __global__ void lincomb_kernel(LincombArgs a) {
     a.out_re[oo] = sr;
     if (a.out_im) a.out_im[oo] = si;
   }
+}
+
+// Every term laid out like the contiguous output (no permutation, no product
+// term): element e of each operand sits at offset e, so a thread handles two
+// consecutive elements with 16-byte accesses; same arithmetic and term order
+// as lincomb_kernel.
+__global__ __launch_bounds__(256) void lincomb_flat_kernel(LincombArgs a) {
+  const int64_t n2 = a.total >> 1;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q <= n2; q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t e = 2 * q;
+    if (e >= a.total) break;
+    const bool pair = e + 1 < a.total;
+    double sr[2] = {0.0, 0.0}, si[2] = {0.0, 0.0};
+    if (a.accumulate) {
+      sr[0] = a.out_re[e];
+      if (pair) sr[1] = a.out_re[e + 1];
+      if (a.out_im) {
+        si[0] = a.out_im[e];
+        if (pair) si[1] = a.out_im[e + 1];
+      }
+    }
+    for (int t = 0; t < a.nterms; ++t) {
+      const LincombTerm& T = a.t[t];
+      double xr[2], xi[2] = {0.0, 0.0};
+      if (pair) {
+        const double2 v = *reinterpret_cast<const double2*>(T.re + e);
+        xr[0] = v.x;
+        xr[1] = v.y;
+        if (T.im) {
+          const double2 w = *reinterpret_cast<const double2*>(T.im + e);
+          xi[0] = w.x;
+          xi[1] = w.y;
+        }
+      } else {
+        xr[0] = T.re[e];
+        xr[1] = 0.0;
+        if (T.im) xi[0] = T.im[e];
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        double pr, pi;
+        if (T.ci == 0.0) {
+          pr = __dmul_rn(T.cr, xr[h]);
+          pi = __dmul_rn(T.cr, xi[h]);
+        } else {
+          pr = __dsub_rn(__dmul_rn(T.cr, xr[h]), __dmul_rn(T.ci, xi[h]));
+          pi = __dadd_rn(__dmul_rn(T.cr, xi[h]), __dmul_rn(T.ci, xr[h]));
+        }
+        if (t == 0 && !a.accumulate) {
+          sr[h] = pr;
+          si[h] = pi;
+        } else {
+          sr[h] = __dadd_rn(sr[h], pr);
+          si[h] = __dadd_rn(si[h], pi);
+        }
+      }
+    }
+    if (pair) {
+      *reinterpret_cast<double2*>(a.out_re + e) = make_double2(sr[0], sr[1]);
+      if (a.out_im) *reinterpret_cast<double2*>(a.out_im + e) = make_double2(si[0], si[1]);
+    } else {
+      a.out_re[e] = sr[0];
+      if (a.out_im) a.out_im[e] = si[0];
+    }
+  }
+}
+
+// Whether every operand of `a` is laid out like its contiguous output, with
+// 16-byte aligned planes.
+static bool lincomb_is_flat(const LincombArgs& a) {
+  auto al16 = [](const double* p) { return p == nullptr || (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+  if (!al16(a.out_re) || !al16(a.out_im)) return false;
+  int64_t st = 1;
+  for (int d = a.rank - 1; d >= 0; --d) {
+    if (a.ost[d] != st && a.dims[d] != 1) return false;
+    st *= a.dims[d];
+  }
+  for (int t = 0; t < a.nterms; ++t) {
+    const LincombTerm& T = a.t[t];
+    if (T.re2 || !al16(T.re) || !al16(T.im)) return false;
+    for (int d = 0; d < a.rank; ++d)
+      if (T.st[d] != a.ost[d] && a.dims[d] != 1) return false;
+  }
+  return true;
 }
 
 static int grid_for(int64_t total) {
@@ -635,6 +731,10 @@ void launch_lincomb(const LincombArgs& a, hipStream_t s) {
       return;
     }
   }
+  if (lincomb_is_flat(a)) {
+    hipLaunchKernelGGL(lincomb_flat_kernel, dim3(grid_for((a.total + 1) / 2)), dim3(256), 0, s, a);
+    return;
+  }
   if (lincomb_use_tiled(a)) {
     const int R = a.rank;
     const int64_t n2 = (a.dims[R - 2] + kLinTile - 1) / kLinTile, n3 = (a.dims[R - 1] + kLinTile - 1) / kLinTile;
@@ -646,11 +746,13 @@ void launch_lincomb(const LincombArgs& a, hipStream_t s) {
       return;
     }
   }
-  hipLaunchKernelGGL(lincomb_kernel, dim3(grid_for(a.total)), dim3(256), 0, s, a);
+  if (a.total < ((int64_t)1 << 31)) hipLaunchKernelGGL(lincomb_kernel<true>, dim3(grid_for(a.total)), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL(lincomb_kernel<false>, dim3(grid_for(a.total)), dim3(256), 0, s, a);
 }
 
 const char* lincomb_kernel_name(const LincombArgs& a) {
   if (a.total > 0 && lincomb_use_anti(a)) return "lincomb_anti_kernel";
+  if (a.total > 0 && lincomb_is_flat(a)) return "lincomb_flat_kernel";
   if (a.total > 0 && lincomb_use_tiled(a)) return "lincomb_tiled_kernel";
   return "lincomb_kernel";
 }

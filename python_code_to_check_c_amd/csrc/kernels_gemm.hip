@@ -23,13 +23,14 @@ __global__ void gather2d_kernel(Gather2dArgs g) {
   const int64_t total = R * C;
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
        idx += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t r = idx / C, c = idx - r * C;
+    const bool i32 = total < ((int64_t)1 << 31);
+    const int64_t r = i32 ? (int64_t)((uint32_t)idx / (uint32_t)C) : idx / C, c = idx - r * C;
     int64_t rb = r, cb = c, p = 0;
     if (g.planes_along_rows) {
-      p = r / g.Rb;
+      p = r >= g.Rb;
       rb = r - p * g.Rb;
     } else {
-      p = c / g.Cb;
+      p = c >= g.Cb;
       cb = c - p * g.Cb;
     }
     const double* src = p ? g.src_im : g.src_re;
@@ -51,7 +52,10 @@ __global__ void gemm_scatter_kernel(GemmScatterArgs g) {
   const int64_t total = (int64_t)g.M * g.N;
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
        idx += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t m = idx / g.N, n = idx - m * g.N;
+    // 32-bit division when the element count allows (64-bit division is a
+    // long instruction sequence)
+    const int64_t m = total < ((int64_t)1 << 31) ? (int64_t)((uint32_t)idx / (uint32_t)g.N) : idx / g.N;
+    const int64_t n = idx - m * g.N;
     double vr = 0.0, vi = 0.0;
     for (int s = 0; s < g.splits; ++s) {
       const double* z = g.Z + s * g.z_stride;
