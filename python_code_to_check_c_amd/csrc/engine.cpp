@@ -1271,14 +1271,18 @@ void Graph::lower_contract(int id) {
   const char ia = innermost(la, sa), ib = innermost(lb, sb);
   const bool akc = ia && has(gk, ia);
   const bool bkc = ib && has(gk, ib);
-  // Dot products (A rows with planes at most 64, B rows at most 512, long
-  // summed group, both operands contiguous in the same summed letter) run in
-  // FP64 on the dot kernel, which streams both once through the offset
-  // tables: HBM-bound shapes on which neither the 128 x 128 GEMM tiles nor
-  // digit splits pay.
+  // Dot products (long summed group, both operands contiguous in the same
+  // summed letter, few rows) run in FP64 on the dot kernel, which streams
+  // both operands once through the offset tables: up to 64 A rows (planes
+  // counted) and 512 B rows whatever the precision, HBM-bound shapes on which
+  // neither the 128 x 128 GEMM tiles nor digit splits pay; up to 512 rows on
+  // both sides for a contraction computed in FP64 anyway (at 320 x 160 rows
+  // it beats the gather + DGEMM path, not the digit GEMM).
   const int64_t dot_k = dot_min_k();
-  const bool use_dot = use_gemm && dot_k >= 0 && akc && ia == ib &&
-                       (nodes_[a].cplx ? 2 : 1) * gsize(gm) <= kDotMaxRows && (nodes_[b].cplx ? 2 : 1) * gsize(gn) <= 512 &&
+  const int64_t a_rows = (nodes_[a].cplx ? 2 : 1) * gsize(gm), b_rows = (nodes_[b].cplx ? 2 : 1) * gsize(gn);
+  const bool in_fp64 = rt_->split_digits == 0 || n.fp64;
+  const bool use_dot = use_gemm && dot_k >= 0 && akc && ia == ib && b_rows <= kDotMaxRows &&
+                       (a_rows <= 64 || (in_fp64 && a_rows <= kDotMaxRows)) &&
                        (double)gsize(gk) * kpair_frac >= (double)dot_k;
   auto by_stride = [&](std::string g, const std::map<char, int64_t>& st) {
     std::stable_sort(g.begin(), g.end(), [&](char x, char y) { return st.at(x) > st.at(y); });
@@ -1419,10 +1423,10 @@ void Graph::lower_contract(int id) {
     gp.dot = true;
     gp.pa = L->ac ? 2 : 1;
     gp.pb = L->bc ? 2 : 1;
-    // about 512 blocks (two per CU, one round; TDCC_DOT_BLOCKS) over the column
-    // tiles and K ranges of 32-multiples: 12 % faster than 1024, 25 % faster
-    // than 256 or 2048 on this step's dot products
-    const int64_t ntn = (gp.pb * Nn + 63) / 64;
+    // about 512 blocks (two per CU, one round; TDCC_DOT_BLOCKS) over the row
+    // and column tiles and K ranges of 32-multiples: 12 % faster than 1024,
+    // 25 % faster than 256 or 2048 on this step's dot products
+    const int64_t ntn = ((gp.pb * Nn + 63) / 64) * ((gp.pa * M + 63) / 64);
     const char* dbe = std::getenv("TDCC_DOT_BLOCKS");
     const int64_t dblocks = dbe ? std::max<int64_t>(1, std::atoll(dbe)) : 512;
     int64_t sp = std::max<int64_t>(1, std::min<int64_t>((K + 255) / 256, (dblocks + ntn - 1) / ntn));

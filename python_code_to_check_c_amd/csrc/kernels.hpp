@@ -223,11 +223,11 @@ void launch_gemm_scatter(const GemmScatterArgs& g, hipStream_t s);
 // FP64 MFMA, read through the offset tables without a gather:
 //   Z[s][r][c] = sum_{k in split s} A'[r][k] * B'[c][k]
 // with A' = [A_re; A_im] (row r = p M + m reads plane p at am[m] + ak[k]) and
-// B' likewise (pb N rows); pa M <= kDotMaxRows.  launch_dot then sums the
+// B' likewise (pb N rows); pa M, pb N <= kDotMaxRows.  launch_dot then sums the
 // `splits` partial results in a fixed order and stores them as
 // launch_gemm_scatter would (complex recombination by kind, alpha, output
 // offset tables; no mirrors, no accumulation).
-constexpr int kDotMaxRows = 64;
+constexpr int kDotMaxRows = 512;
 struct DotArgs {
   const double* a_re; const double* a_im;   // a_im / b_im null: real operand
   const double* b_re; const double* b_im;

@@ -85,8 +85,8 @@ void launch_gemm_scatter(const GemmScatterArgs& g, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// Dot-product contraction: a block owns all pa M rows of A' (zero padded to
-// 64) and 64 rows of B' over one K range, in 32-wide K steps staged through
+// Dot-product contraction: a block owns 64 rows of A' (grid z, zero padded)
+// and 64 rows of B' (grid x) over one K range (grid y), in 32-wide K steps staged through
 // LDS (two buffers, one barrier per step); wave w computes rows 16w .. 16w+15
 // as four 16 x 16 blocks on v_mfma_f64_16x16x4_f64 (lane l: A[row l%16][k
 // l/16], B[k l/16][col l%16], D[row l/16 + 4r][col l%16], as in the DGEMM
@@ -103,7 +103,7 @@ __global__ __launch_bounds__(256) void dot_kernel(DotArgs g) {
   __shared__ double As[2][DK_BK][DK_LD];
   __shared__ double Bs[2][DK_BK][DK_LD];
   const int Mr = PA * g.M, Nc = PB * g.N;
-  const int n0 = blockIdx.x * DK_BN;
+  const int n0 = blockIdx.x * DK_BN, m0 = blockIdx.z * DK_BN;
   const int64_t kb = (int64_t)blockIdx.y * g.k_chunk;
   const int64_t ke = kb + g.k_chunk < g.K ? kb + g.k_chunk : g.K;
   const int64_t nsteps = (ke - kb + DK_BK - 1) / DK_BK;
@@ -112,7 +112,7 @@ __global__ __launch_bounds__(256) void dot_kernel(DotArgs g) {
   const double* brow[8];
 #pragma unroll
   for (int p = 0; p < 8; ++p) {
-    const int r = r8 + 8 * p;
+    const int r = m0 + r8 + 8 * p;
     arow[p] = nullptr;
     if (r < Mr) {
       const bool im = PA == 2 && r >= g.M;
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(256) void dot_kernel(DotArgs g) {
 #pragma unroll
   for (int j = 0; j < 4; ++j) acc[j] = d4{0.0, 0.0, 0.0, 0.0};
   const int lane = tid & 63, w = tid >> 6, lr = lane & 15, lk = lane >> 4;
-  const bool active = 16 * w < Mr;
+  const bool active = m0 + 16 * w < Mr;
   auto compute = [&](int buf) {
     if (!active) return;
 #pragma unroll
@@ -193,7 +193,7 @@ __global__ __launch_bounds__(256) void dot_kernel(DotArgs g) {
     if (col >= Nc) continue;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int row = 16 * w + lk + 4 * r;
+      const int row = m0 + 16 * w + lk + 4 * r;
       if (row < Mr) Zs[(int64_t)row * Nc + col] = acc[j][r];
     }
   }
@@ -237,8 +237,9 @@ __global__ __launch_bounds__(256) void dot_reduce_kernel(DotArgs d, GemmScatterA
 void launch_dot(const DotArgs& d, const GemmScatterArgs& sc, hipStream_t s) {
   if (d.M <= 0 || d.N <= 0) return;
   const int pa = d.a_im ? 2 : 1, pb = d.b_im ? 2 : 1;
-  const int64_t Nc = pb * (int64_t)d.N;
-  const dim3 grid((unsigned)((Nc + DK_BN - 1) / DK_BN), (unsigned)d.splits), block(256);
+  const int64_t Mr = pa * (int64_t)d.M, Nc = pb * (int64_t)d.N;
+  const dim3 grid((unsigned)((Nc + DK_BN - 1) / DK_BN), (unsigned)d.splits, (unsigned)((Mr + DK_BN - 1) / DK_BN));
+  const dim3 block(256);
   if (pa == 2 && pb == 2) hipLaunchKernelGGL((dot_kernel<2, 2>), grid, block, 0, s, d);
   else if (pa == 2) hipLaunchKernelGGL((dot_kernel<2, 1>), grid, block, 0, s, d);
   else if (pb == 2) hipLaunchKernelGGL((dot_kernel<1, 2>), grid, block, 0, s, d);

@@ -32,7 +32,7 @@ RHS = [("T1eq_rhs", _lib.RHS_T1), ("T2eq_rhs", _lib.RHS_T2), ("lam_1eq_rhs", _li
 # particle-particle ladders on all (i, j) rows instead of the i < j rows of
 # the antisymmetric amplitudes; antisymmetric linear combinations summed at
 # every element instead of the canonical ones; every dense-GEMM contraction
-# with at most 64 A rows and 512 B rows (planes counted) on the dot-product
+# with at most 512 A rows and 512 B rows (planes counted) on the dot-product
 # kernel (by default only from K = 8192 on); every linear combination and
 # outer product materialized (by default a linear combination used only by
 # another is folded into it, and outer products of two matrices are formed
@@ -102,6 +102,49 @@ def test_rk4_trajectory(case, method):
     state = ctx.get_state(0)
     for k, key in enumerate(("t1", "t2", "lam1", "lam2")):
         assert np.abs(state[k] - g[f"rk_{method}_{key}"][-1]).max() <= TOL, key
+
+
+def test_dot_kernel_row_tiles_match_gemm_path():
+    """Dot-product contractions with more than 64 rows on a side run on several
+    64-row tiles of the dot kernel.  At o = 8, v = 64 (up to 128 rows), with every
+    batch-free contraction on the dense-GEMM path, the residuals computed with
+    the dot-product shapes on the dot kernel must agree with the dense GEMMs to
+    FP64 rounding."""
+    from python_code_to_check_c_amd import synthetic
+    o, v = 8, 64
+    sysm = synthetic.SyntheticSystem(3, o, v, with_tei=False)
+    rng = np.random.default_rng(11)
+
+    def amp(shape):
+        return 1e-2 * (rng.standard_normal(shape) + 1j * rng.standard_normal(shape))
+
+    t1, l1 = amp((o, v)), amp((o, v))
+    t2, l2 = amp((o, o, v, v)), amp((o, o, v, v))
+    for x in (t2, l2):
+        x -= x.transpose(1, 0, 2, 3)
+        x -= x.transpose(0, 1, 3, 2)
+    Fc = sysm.F + 0j
+    out = {}
+    saved = {k: os.environ.get(k) for k in ("TDCC_DOT_MIN_K", "TDCC_GEMM_MIN_FLOPS")}
+    try:
+        os.environ["TDCC_GEMM_MIN_FLOPS"] = "0"
+        for mode in ("0", "-1"):
+            os.environ["TDCC_DOT_MIN_K"] = mode
+            c = _lib.Context(o, v)
+            try:
+                c.set_system(None, np.asarray(sysm.F), np.asarray(sysm.mu))
+                c.synth_tei(3, 0.05)
+                out[mode] = [c.rhs(kind, Fc, t1, t2, l1, l2) for _, kind in RHS]
+            finally:
+                c.close()
+    finally:
+        for k, val in saved.items():
+            if val is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = val
+    for (name, _), x, y in zip(RHS, out["0"], out["-1"]):
+        assert np.abs(x - y).max() <= 1e-12 * max(np.abs(y).max(), 1e-300), name
 
 
 @pytest.mark.parametrize("kind,name", [(_lib.RHS_T2, "T2eq_rhs"), (_lib.RHS_L2, "lam2eq_rhs")])
