@@ -1271,8 +1271,7 @@ void Graph::lower_contract(int id) {
   const char ia = innermost(la, sa), ib = innermost(lb, sb);
   const bool akc = ia && has(gk, ia);
   const bool bkc = ib && has(gk, ib);
-  // Dot products (long summed group, both operands contiguous in the same
-  // summed letter, few rows) run in FP64 on the dot kernel, which streams
+  // Dot products (long summed group, few rows) run in FP64 on the dot kernel, which streams
   // both operands once through the offset tables: up to 64 A rows (planes
   // counted) and 512 B rows whatever the precision, HBM-bound shapes on which
   // neither the 128 x 128 GEMM tiles nor digit splits pay; up to 512 rows on
@@ -1281,7 +1280,10 @@ void Graph::lower_contract(int id) {
   const int64_t dot_k = dot_min_k();
   const int64_t a_rows = (nodes_[a].cplx ? 2 : 1) * gsize(gm), b_rows = (nodes_[b].cplx ? 2 : 1) * gsize(gn);
   const bool in_fp64 = rt_->split_digits == 0 || n.fp64;
-  const bool use_dot = use_gemm && dot_k >= 0 && akc && ia == ib && b_rows <= kDotMaxRows &&
+  // (an operand contiguous along K streams along the summed letters, one
+  // contiguous along its rows along them; two contiguous along different
+  // summed letters cannot both stream)
+  const bool use_dot = use_gemm && dot_k >= 0 && !(akc && bkc && ia != ib) && b_rows <= kDotMaxRows &&
                        (a_rows <= 64 || (in_fp64 && a_rows <= kDotMaxRows)) &&
                        (double)gsize(gk) * kpair_frac >= (double)dot_k;
   auto by_stride = [&](std::string g, const std::map<char, int64_t>& st) {
@@ -1603,6 +1605,8 @@ void Graph::bind(double* arena) {
           d.K = g.K;
           d.k_chunk = gp.k_chunk;
           d.splits = gp.splits;
+          d.a_kc = L->akc ? 1 : 0;
+          d.b_kc = L->bkc ? 1 : 0;
           d.Z = arena_ + n.scratch_off / (int64_t)sizeof(double);
           GemmScatterArgs& sc = L->sc;
           sc.Z = d.Z;

@@ -237,6 +237,7 @@ struct DotArgs {
   int64_t K;
   int64_t k_chunk;                          // multiple of 32
   int splits;
+  int a_kc = 1, b_kc = 1;                   // operand contiguous along k (else along its rows)
   double* Z;                                // [splits][pa M][pb N]
 };
 void launch_dot(const DotArgs& a, const GemmScatterArgs& out, hipStream_t s);

@@ -98,7 +98,7 @@ void launch_gemm_scatter(const GemmScatterArgs& g, hipStream_t s) {
 // ---------------------------------------------------------------------------
 constexpr int DK_BK = 32, DK_BN = 64, DK_LD = 65;
 
-template <int PA, int PB>
+template <int PA, int PB, bool AKC, bool BKC>
 __global__ __launch_bounds__(256) void dot_kernel(DotArgs g) {
   __shared__ double As[2][DK_BK][DK_LD];
   __shared__ double Bs[2][DK_BK][DK_LD];
@@ -107,62 +107,68 @@ __global__ __launch_bounds__(256) void dot_kernel(DotArgs g) {
   const int64_t kb = (int64_t)blockIdx.y * g.k_chunk;
   const int64_t ke = kb + g.k_chunk < g.K ? kb + g.k_chunk : g.K;
   const int64_t nsteps = (ke - kb + DK_BK - 1) / DK_BK;
-  const int tid = threadIdx.x, kk = tid & 31, r8 = tid >> 5;
+  const int tid = threadIdx.x;
+  // loader element p of an operand contiguous along k: row tid/32 + 8p, k
+  // tid%32; of one contiguous along its rows: row tid%64, k tid/64 + 4p
+  auto lrow = [&](bool kc, int p) { return kc ? (tid >> 5) + 8 * p : (tid & 63); };
+  auto lk = [&](bool kc, int p) { return kc ? (tid & 31) : (tid >> 6) + 4 * p; };
   const double* arow[8];
   const double* brow[8];
 #pragma unroll
   for (int p = 0; p < 8; ++p) {
-    const int r = m0 + r8 + 8 * p;
+    const int r = m0 + lrow(AKC, p);
     arow[p] = nullptr;
     if (r < Mr) {
       const bool im = PA == 2 && r >= g.M;
       arow[p] = (im ? g.a_im : g.a_re) + g.am[im ? r - g.M : r];
     }
-    const int c = n0 + r8 + 8 * p;
+    const int c = n0 + lrow(BKC, p);
     brow[p] = nullptr;
     if (c < Nc) {
       const bool im = PB == 2 && c >= g.N;
       brow[p] = (im ? g.b_im : g.b_re) + g.bn[im ? c - g.N : c];
     }
   }
-  // k offsets of step t for this thread (-1: past the range)
-  auto offsets = [&](int64_t t, int64_t& oa, int64_t& ob) {
-    const int64_t k = kb + t * DK_BK + kk;
-    const bool ok = t < nsteps && k < ke;
-    oa = ok ? g.ak[k] : -1;
-    ob = ok ? g.bk[k] : -1;
-  };
-  auto load = [&](int64_t oa, int64_t ob, double* ra, double* rb) {
+  // k offsets of step t for this thread's elements (-1: past the range)
+  auto offsets = [&](int64_t t, int64_t* oa, int64_t* ob) {
 #pragma unroll
     for (int p = 0; p < 8; ++p) {
-      ra[p] = (oa >= 0 && arow[p]) ? arow[p][oa] : 0.0;
-      rb[p] = (ob >= 0 && brow[p]) ? brow[p][ob] : 0.0;
+      const int64_t ka = kb + t * DK_BK + lk(AKC, p), kbb = kb + t * DK_BK + lk(BKC, p);
+      oa[p] = (t < nsteps && ka < ke) ? g.ak[ka] : -1;
+      ob[p] = (t < nsteps && kbb < ke) ? g.bk[kbb] : -1;
+    }
+  };
+  auto load = [&](const int64_t* oa, const int64_t* ob, double* ra, double* rb) {
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      ra[p] = (oa[p] >= 0 && arow[p]) ? arow[p][oa[p]] : 0.0;
+      rb[p] = (ob[p] >= 0 && brow[p]) ? brow[p][ob[p]] : 0.0;
     }
   };
   auto store = [&](int buf, const double* ra, const double* rb) {
 #pragma unroll
     for (int p = 0; p < 8; ++p) {
-      As[buf][kk][r8 + 8 * p] = ra[p];
-      Bs[buf][kk][r8 + 8 * p] = rb[p];
+      As[buf][lk(AKC, p)][lrow(AKC, p)] = ra[p];
+      Bs[buf][lk(BKC, p)][lrow(BKC, p)] = rb[p];
     }
   };
   d4 acc[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) acc[j] = d4{0.0, 0.0, 0.0, 0.0};
-  const int lane = tid & 63, w = tid >> 6, lr = lane & 15, lk = lane >> 4;
+  const int lane = tid & 63, w = tid >> 6, lr = lane & 15, lq = lane >> 4;
   const bool active = m0 + 16 * w < Mr;
   auto compute = [&](int buf) {
     if (!active) return;
 #pragma unroll
     for (int k4 = 0; k4 < DK_BK / 4; ++k4) {
-      const double a = As[buf][4 * k4 + lk][16 * w + lr];
+      const double a = As[buf][4 * k4 + lq][16 * w + lr];
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Bs[buf][4 * k4 + lk][16 * j + lr], acc[j], 0, 0, 0);
+        acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Bs[buf][4 * k4 + lq][16 * j + lr], acc[j], 0, 0, 0);
     }
   };
   double xa[8], xb[8], ya[8], yb[8];
-  int64_t oxa, oxb, oya, oyb;
+  int64_t oxa[8], oxb[8], oya[8], oyb[8];
   offsets(0, oxa, oxb);
   load(oxa, oxb, xa, xb);
   store(0, xa, xb);
@@ -193,7 +199,7 @@ __global__ __launch_bounds__(256) void dot_kernel(DotArgs g) {
     if (col >= Nc) continue;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int row = m0 + 16 * w + lk + 4 * r;
+      const int row = m0 + 16 * w + lq + 4 * r;
       if (row < Mr) Zs[(int64_t)row * Nc + col] = acc[j][r];
     }
   }
@@ -240,10 +246,19 @@ void launch_dot(const DotArgs& d, const GemmScatterArgs& sc, hipStream_t s) {
   const int64_t Mr = pa * (int64_t)d.M, Nc = pb * (int64_t)d.N;
   const dim3 grid((unsigned)((Nc + DK_BN - 1) / DK_BN), (unsigned)d.splits, (unsigned)((Mr + DK_BN - 1) / DK_BN));
   const dim3 block(256);
-  if (pa == 2 && pb == 2) hipLaunchKernelGGL((dot_kernel<2, 2>), grid, block, 0, s, d);
-  else if (pa == 2) hipLaunchKernelGGL((dot_kernel<2, 1>), grid, block, 0, s, d);
-  else if (pb == 2) hipLaunchKernelGGL((dot_kernel<1, 2>), grid, block, 0, s, d);
-  else hipLaunchKernelGGL((dot_kernel<1, 1>), grid, block, 0, s, d);
+  const int kc = (d.a_kc ? 2 : 0) + (d.b_kc ? 1 : 0);
+#define TDCC_DOT_LAUNCH(PA, PB)                                                                    \
+  switch (kc) {                                                                                    \
+    case 3: hipLaunchKernelGGL((dot_kernel<PA, PB, true, true>), grid, block, 0, s, d); break;     \
+    case 2: hipLaunchKernelGGL((dot_kernel<PA, PB, true, false>), grid, block, 0, s, d); break;    \
+    case 1: hipLaunchKernelGGL((dot_kernel<PA, PB, false, true>), grid, block, 0, s, d); break;    \
+    default: hipLaunchKernelGGL((dot_kernel<PA, PB, false, false>), grid, block, 0, s, d); break;  \
+  }
+  if (pa == 2 && pb == 2) { TDCC_DOT_LAUNCH(2, 2) }
+  else if (pa == 2) { TDCC_DOT_LAUNCH(2, 1) }
+  else if (pb == 2) { TDCC_DOT_LAUNCH(1, 2) }
+  else { TDCC_DOT_LAUNCH(1, 1) }
+#undef TDCC_DOT_LAUNCH
   const int64_t n = (int64_t)sc.M * sc.N;
   hipLaunchKernelGGL(dot_reduce_kernel, dim3((unsigned)((n + 3) / 4)), block, 0, s, d, sc);
 }
