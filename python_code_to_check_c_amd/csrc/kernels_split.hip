@@ -305,7 +305,12 @@ template <int S>
 static void split_launch(const SplitArgs& a, bool by_columns, hipStream_t s) {
   const int64_t R = (int64_t)a.planes * a.Rb;
   const dim3 block(256);
-  if (!a.multipass && !by_columns && a.Kp <= 256) {
+  // Short rows: one wave per row.  Column-oriented operands with few rows go
+  // here too: 64-row column tiles would give them a handful of blocks whose
+  // serial loads dominate, while the whole operand is small enough that
+  // uncoalesced loads cost little.
+  constexpr int64_t kWaveColsMaxRows = 2048;
+  if (!a.multipass && a.Kp <= 256 && (!by_columns || R <= kWaveColsMaxRows)) {
     hipLaunchKernelGGL(split_rows_wave_kernel<S>, dim3((unsigned)((R + 3) / 4)), block, 0, s, a);
     return;
   }
