@@ -49,7 +49,12 @@ def _anti_both(x):
 class CCEquations:
     """State-free CCSD/CC2 equations over one set of integrals."""
 
-    def __init__(self, TEI, nocc, contract=None):
+    def __init__(self, TEI, nocc, contract=None, vchunk=None):
+        """vchunk: None evaluates every v^4 intermediate (Wabef, Wabef_2,
+        LSWabef, the <ab||ef> t1 term of L1eq_rhs_CC2) whole, as the
+        reference does; an integer builds it `vchunk` leading virtual indices
+        at a time and contracts each slab at once (the same einsum strings on
+        index slices), so that o=20, v=160 fits in host memory."""
         self.TEI = np.asarray(TEI, dtype=np.float64)
         n = self.TEI.shape[0]
         self.nmo = n // 2
@@ -58,6 +63,24 @@ class CCEquations:
         self.occ = slice(0, nocc)
         self.vir = slice(nocc, n)
         self.contract = contract or np_contract
+        self.vchunk = vchunk
+
+    def _chunks(self):
+        step = self.vchunk or self.nvir
+        return [slice(a, min(a + step, self.nvir)) for a in range(0, self.nvir, step)]
+
+    def _sub(self, spec, ops, letter, rows):
+        """contract(spec, *ops) with the output letter `letter` restricted to
+        `rows` (every operand axis carrying that letter is sliced)."""
+        ins = spec.split("->")[0].split(",")
+        cut = [op[tuple(rows if ch == letter else slice(None) for ch in lab)] for lab, op in zip(ins, ops)]
+        return self.contract(spec, *cut)
+
+    def _rows_anti01(self, spec, ops, rows):
+        """Rows `rows` of _anti01(contract(spec, *ops)) for a rank-4 output."""
+        ins, out = spec.split("->")
+        swapped = ins + "->" + out[1] + out[0] + out[2:]
+        return self._sub(spec, ops, out[0], rows) - self._sub(swapped, ops, out[1], rows)
 
     def blk(self, key):
         """TEI block by orbital classes, e.g. 'oovv' -> TEI[o, o, v, v]."""
@@ -121,19 +144,30 @@ class CCEquations:
                 + _anti23(c("mnie,je->mnij", self.blk("ooov"), t1))
                 + _anti23(0.25 * c("mnef,ijef->mnij", self.blk("oovv"), tau)))
 
-    def Wabef(self, t1, t2, F):
-        """CCSD_Helper.py:227-239"""
+    def Wabef(self, t1, t2, F, rows=None):
+        """CCSD_Helper.py:227-239 (rows: the slab a in rows only)"""
         c = self.contract
         tau = self._t1t1(t1)
+        if rows is not None:
+            vovv, oovv = self.blk("vovv"), self.blk("oovv")
+            return (self.blk("vvvv")[rows].copy()
+                    + self._rows_anti01("amef,mb->abef", (vovv, -t1), rows)
+                    + 0.25 * self._sub("mnef,mnab->abef", (oovv, t2), "a", rows)
+                    + self._rows_anti01("mnef,mnab->abef", (oovv, 0.5 * tau), rows))
         return (self.blk("vvvv").copy()
                 + _anti01(-c("amef,mb->abef", self.blk("vovv"), t1))
                 + 0.25 * c("mnef,mnab->abef", self.blk("oovv"), t2)
                 + _anti01(0.5 * c("mnef,mnab->abef", self.blk("oovv"), tau)))
 
-    def Wabef_2(self, t1, t2, F):
-        """CCSD_Helper.py:242-252"""
+    def Wabef_2(self, t1, t2, F, rows=None):
+        """CCSD_Helper.py:242-252 (rows: the slab a in rows only)"""
         c = self.contract
         tau = self._t1t1(t1) - c("ib,ja->ijab", t1, t1)
+        if rows is not None:
+            vovv, oovv = self.blk("vovv"), self.blk("oovv")
+            return (self.blk("vvvv")[rows].copy()
+                    + self._rows_anti01("amef,mb->abef", (vovv, -t1), rows)
+                    + 0.25 * self._sub("mnef,mnab->abef", (oovv, tau), "a", rows))
         return (self.blk("vvvv").copy()
                 + _anti01(-c("amef,mb->abef", self.blk("vovv"), t1))
                 + 0.25 * c("mnef,mnab->abef", self.blk("oovv"), tau))
@@ -154,17 +188,23 @@ class CCEquations:
         fmi = self.Fmi(t1, t2, F)
         fme = self.Fme(t1, t2, F)
         wmnij = self.Wmnij(t1, t2, F)
-        wabef = self.Wabef(t1, t2, F)
         wmbej = self.Wmbej(t1, t2, F)
-        wabef_2 = self.Wabef_2(t1, t2, F)
         wmnij_2 = self.Wmnij_2(t1, t2, F)
         fvv = fae - 0.5 * c("me,mb->be", fme, t1)
         foo = fmi + 0.5 * c("me,je->mj", fme, t1)
         tau = c("ma,nb->mnab", t1, t1) - c("na,mb->mnab", t1, t1)
         oooo_ladder = (0.5 * c("mnij,mnab->ijab", wmnij, t2)
                        + 0.5 * c("mnij,mnab->ijab", wmnij_2, tau))
-        vvvv_ladder = (0.5 * c("abef,ijef->ijab", wabef, t2)
-                       + 0.5 * c("abef,ijef->ijab", wabef_2, tau))
+        if self.vchunk is None:
+            wabef = self.Wabef(t1, t2, F)
+            wabef_2 = self.Wabef_2(t1, t2, F)
+            vvvv_ladder = (0.5 * c("abef,ijef->ijab", wabef, t2)
+                           + 0.5 * c("abef,ijef->ijab", wabef_2, tau))
+        else:
+            vvvv_ladder = np.empty_like(t2)
+            for A in self._chunks():
+                vvvv_ladder[:, :, A] = (0.5 * c("abef,ijef->ijab", self.Wabef(t1, t2, F, A), t2)
+                                        + 0.5 * c("abef,ijef->ijab", self.Wabef_2(t1, t2, F, A), tau))
         ring = (c("mbej,imae->ijab", wmbej, t2)
                 - c("mbej,ie,ma->ijab", self.blk("ovvo"), t1, t1))
         return (self.blk("oovv").copy()
@@ -215,12 +255,17 @@ class CCEquations:
         """CCSD_Helper.py:488-505"""
         c = self.contract
         fme = self.Fme(t1, t2, F)
-        wabef = self.LSWabef(t1, t2, F)
+        if self.vchunk is None:
+            w_t1 = c("efab,mb->efam", self.LSWabef(t1, t2, F), t1)
+        else:
+            w_t1 = np.empty((self.nvir, self.nvir, self.nvir, self.nocc), dtype=np.result_type(t1, t2, F))
+            for E in self._chunks():
+                w_t1[E] = c("efab,mb->efam", self.LSWabef(t1, t2, F, E), t1)
         tau = 0.25 * t2 + 0.5 * self._t1t1(t1)
         ring_a = -self.blk("ovvo").copy() + c("jnab,nmfb->jfam", self.blk("oovv"), t2)
         return 0.5 * self.blk("vvvo").copy() + (
             0.5 * c("na,mnef->efam", fme, t2)
-            + c("efab,mb->efam", wabef, t1)
+            + w_t1
             + c("jfam,je->efam", ring_a, t1)
             + c("jnam,jnef->efam", self.blk("oovo"), tau)
             - c("jfab,jmeb->efam", self.blk("ovvv"), t2))
@@ -285,10 +330,14 @@ class CCEquations:
                 + c("ijme,ne->ijmn", self.blk("ooov"), t1)
                 + c("ijfe,mnfe->ijmn", self.blk("oovv"), tau))
 
-    def LSWabef(self, t1, t2, F):
-        """CCSD_Helper.py:598-607"""
+    def LSWabef(self, t1, t2, F, rows=None):
+        """CCSD_Helper.py:598-607 (rows: the slab e in rows only)"""
         c = self.contract
         tau = 0.25 * t2 + 0.5 * self._t1t1(t1)
+        if rows is not None:
+            return (0.5 * self.blk("vvvv")[rows].copy()
+                    - self._sub("emab,mf->efab", (self.blk("vovv"), t1), "e", rows)
+                    + self._sub("nmab,nmef->efab", (self.blk("oovv"), tau), "e", rows))
         return (0.5 * self.blk("vvvv").copy()
                 - c("emab,mf->efab", self.blk("vovv"), t1)
                 + c("nmab,nmef->efab", self.blk("oovv"), tau))
@@ -299,7 +348,6 @@ class CCEquations:
         feb = self.LRFea(t1, t2, F)
         fjm = self.LRFim(t1, t2, F)
         wijmn = self.LSWmnij(t1, t2, F)
-        wefab = self.LSWabef(t1, t2, F)
         wjebm = self.LSWieam(t1, t2, F)
         wejab = self.LWfiea(t1)
         wijmb = self.LWmina(t1)
@@ -312,13 +360,25 @@ class CCEquations:
                  + _anti23(c("eb,ijae->ijab", feb, lam2))
                  + _anti01(-c("jm,imab->ijab", fjm, lam2))
                  + c("ijmn,mnab->ijab", wijmn, lam2)
-                 + (c("efab,ijef->ijab", wefab, lam2)
+                 + (self._lam2_ladder(t1, t2, lam2, F)
                     + _anti01(c("ejab,ie->ijab", wejab, lam1)))
                  + _anti23(-c("ijmb,ma->ijab", wijmb, lam1))
                  + _anti_both(ring))
         return (total
                 + _anti23(c("ijfb,af->ijab", oovv, gbe))
                 + _anti01(-c("mjab,im->ijab", oovv, gmj)))
+
+    def _lam2_ladder(self, t1, t2, lam2, F):
+        """c("efab,ijef->ijab", LSWabef, lam2) (CCSD_Helper.py:631); chunked,
+        the sum over e is taken slab by slab."""
+        c = self.contract
+        if self.vchunk is None:
+            return c("efab,ijef->ijab", self.LSWabef(t1, t2, F), lam2)
+        out = None
+        for E in self._chunks():
+            part = c("efab,ijef->ijab", self.LSWabef(t1, t2, F, E), lam2[:, :, E])
+            out = part if out is None else out + part
+        return out
 
     def CCSD_pseudo_E(self, t1, t2, lam1, lam2, F):
         """CCSD_Helper.py:650-656"""
@@ -330,15 +390,20 @@ class CCEquations:
     def T2eq_rhs_CC2(self, t1, t2, F):
         """CC2_Helper.py:12-47"""
         c, o, v = self.contract, self.occ, self.vir
-        wabef_2 = self.Wabef_2(t1, t2, F)
         wmnij_2 = self.Wmnij_2(t1, t2, F)
         tau = c("ma,nb->mnab", t1, t1) - c("na,mb->mnab", t1, t1)
+        if self.vchunk is None:
+            ladder = 0.5 * c("abef,ijef->ijab", self.Wabef_2(t1, t2, F), tau)
+        else:
+            ladder = np.empty(tau.shape, dtype=np.result_type(t1, t2, F))
+            for A in self._chunks():
+                ladder[:, :, A] = 0.5 * c("abef,ijef->ijab", self.Wabef_2(t1, t2, F, A), tau)
         ring = -c("mbej,ie,ma->ijab", self.blk("ovvo"), t1, t1)
         return (self.blk("oovv").copy()
                 + _anti23(c("be,ijae->ijab", F[v, v].copy(), t2))
                 + _anti01(-c("mj,imab->ijab", F[o, o].copy(), t2))
                 + 0.5 * c("mnij,mnab->ijab", wmnij_2, tau)
-                + 0.5 * c("abef,ijef->ijab", wabef_2, tau)
+                + ladder
                 + _anti_both(ring)
                 + _anti01(c("abej,ie->ijab", self.blk("vvvo"), t1))
                 + _anti23(-c("mbij,ma->ijab", self.blk("ovoo"), t1)))
@@ -371,7 +436,13 @@ class CCEquations:
                     - c("mbej,ie->mbij", zmejb, t1))
         wmbij = wmbij - zmejb_t1
         rhs = rhs - 0.5 * c("iemn,mnae->ia", wmbij, lam2)
-        wabei = self.blk("vvvo").copy() + c("abef,if->abei", self.blk("vvvv"), t1)
+        if self.vchunk is None:
+            vvvv_t1 = c("abef,if->abei", self.blk("vvvv"), t1)
+        else:
+            vvvv_t1 = np.empty((self.nvir, self.nvir, self.nvir, self.nocc), dtype=np.result_type(t1, t2, F))
+            for A in self._chunks():
+                vvvv_t1[A] = c("abef,if->abei", self.blk("vvvv")[A], t1)
+        wabei = self.blk("vvvo").copy() + vvvv_t1
         zmbej = (self.blk("ovvo").copy()
                  + c("mbef,jf->mbej", self.blk("ovvv"), t1)
                  - 0.5 * c("mnej,nb->mbej", self.blk("oovo"), t1)

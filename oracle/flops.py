@@ -129,6 +129,7 @@ class _SymEquations(CCEquations):
         self.contract = counter.contract
         self.occ = slice(0, 1)
         self.vir = slice(1, 2)
+        self.vchunk = None  # whole v^4 intermediates, as written
 
     def blk(self, key):
         return Sym(key, False, "TEI" + key)

@@ -215,11 +215,20 @@ class CCSD_Helper(object):
                    pol=2, max_steps=None, chunk_seconds=0.5, max_chunk=256):
         """The reference loop on the device, in chunks of graph-replayed steps.
 
-        The stop rules are checked after every step of a chunk: on divergence
-        (|Re mu_z| > 100) the state is restored to the chunk start and the
-        chunk is replayed up to the diverging step (kernels are deterministic),
-        so the saved rounded base is the one of that step; the wall-clock rule
-        is applied at chunk ends.  Running to t >= tf saves nothing.
+        The stop rules are applied after every step, in the reference's order
+        (CCSD_Helper.py:1234-1251): first the wall clock (elapsed time at the
+        end of the step's update above 0.9 * timeout minutes), then divergence
+        (|Re mu_z| > 100).  Inside a chunk the end of step j is taken as the
+        chunk's start plus (j + 1) times its mean step time.  When a rule fires
+        before the chunk's last step, the state is restored to the chunk start
+        and the chunk is replayed up to that step (the kernels are
+        deterministic), so the saved rounded base and time are those of the
+        stopping step.  Running to t >= tf saves nothing.
+
+        Timing rows: `total` is the mean step time of the step's chunk; the
+        per-chain columns t1, t2, l1, l2 are NaN, because one captured graph
+        runs the four chains together, sharing the intermediates the
+        reference recomputes per chain, so there is no per-chain time to take.
         """
         ctx = self.ctx
         method = self.method
@@ -248,11 +257,12 @@ class CCSD_Helper(object):
             per = (time.time() - c0) / k
             stop_at, reason = None, None
             for j in range(k):
-                if abs(mu[j, 2].real) > 100:
+                if abs(c0 + (j + 1) * per - start) > 0.9 * timeout * 60.0:
+                    stop_at, reason = j, "timeout"
+                elif abs(mu[j, 2].real) > 100:
                     stop_at, reason = j, "divergence"
+                if stop_at is not None:
                     break
-            if stop_at is None and abs(time.time() - start) > 0.9 * timeout * 60.0:
-                stop_at, reason = k - 1, "timeout"
             if stop_at is not None and stop_at < k - 1:
                 ctx.checkpoint(1)
                 mu, ts = ctx.rk4_run(method, stop_at + 1, t, dt, w0, A, pol, precs)

@@ -172,15 +172,24 @@ class Context:
         return tuple(out)
 
     # ------------------------------------------------------------- evaluation
+    def _fock_ok(self, F):
+        if F.shape != (self.n, self.n):
+            raise ValueError(f"F has shape {F.shape}, expected ({self.n}, {self.n})")
+
     def rhs(self, kind, F, t1, t2, lam1=None, lam2=None):
         F_, pF = _c(F)
         a, pa = _c(t1)
         b, pb = _c(t2)
         self._shape_ok(a, b)
+        self._fock_ok(F_)
         c, pc = _opt(lam1)
         d, pd = _opt(lam2)
-        if F_.shape != (self.n, self.n):
-            raise ValueError("F must be (n,n)")
+        if kind in (RHS_L1, RHS_L2, RHS_L1_CC2, RHS_L2_CC2):
+            if c is None or d is None:
+                raise ValueError("the Lambda residuals need lam1 and lam2")
+            self._shape_ok(c, d)
+        elif (c is None) != (d is None) or (c is not None and (c.shape, d.shape) != (a.shape, b.shape)):
+            raise ValueError("lam1/lam2 must both be given with the shapes of t1/t2, or both be None")
         one_body = kind in (RHS_T1, RHS_L1, RHS_L1_CC2)
         shape = (self.o, self.v) if one_body else (self.o, self.o, self.v, self.v)
         out = np.empty(shape, complex)
@@ -194,6 +203,9 @@ class Context:
         b, pb = _c(t2)
         c, pc = _c(lam1)
         d, pd = _c(lam2)
+        self._fock_ok(F_)
+        self._shape_ok(a, b)
+        self._shape_ok(c, d)
         out = np.empty(shape, complex)
         self._check(self.lib.tdcc_intermediate(self._h, name.encode(), pF, pa, pb, pc, pd,
                                                out.ctypes.data_as(_dp), out.size), f"tdcc_intermediate({name})")
@@ -204,6 +216,8 @@ class Context:
         b, pb = _c(t2)
         c, pc = _c(lam1)
         d, pd = _c(lam2)
+        self._shape_ok(a, b)
+        self._shape_ok(c, d)
         out = np.empty(3, complex)
         self._check(self.lib.tdcc_dipole(self._h, pa, pb, pc, pd, out.ctypes.data_as(_dp)), "tdcc_dipole")
         return out

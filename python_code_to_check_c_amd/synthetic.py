@@ -23,6 +23,8 @@ Distributions:
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
 KEY0 = 0x20261203
@@ -34,6 +36,7 @@ COMP_FOCK = 2
 COMP_TEI = 3
 COMP_MU = (4, 5, 6)
 COMP_W0 = 7
+COMP_AMP = 8  # 8..15: re/im of t1, t2, lam1, lam2 perturbations (test states)
 
 _M0 = np.uint64(0xD2511F53)
 _M1 = np.uint64(0xCD9E8D57)
@@ -95,6 +98,23 @@ def normal(config_id: int, component: int, idx):
     return np.sqrt(-2.0 * np.log(u1)) * np.cos(TWO_PI * u2)
 
 
+def normals(config_id: int, component: int, size: int, chunk: int = 1 << 20, workers: int | None = None):
+    """normal(config_id, component, 0..size-1), generated in chunks by a thread pool."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    out = np.empty(size)
+
+    def fill(start):
+        stop = min(size, start + chunk)
+        out[start:stop] = normal(config_id, component, np.arange(start, stop, dtype=np.uint64))
+
+    if workers is None:
+        workers = min(16, len(os.sched_getaffinity(0)))
+    with ThreadPoolExecutor(max(1, workers)) as ex:
+        list(ex.map(fill, range(0, size, chunk)))
+    return out
+
+
 def orbital_energies(config_id: int, o: int, v: int):
     eo = -1.5 + 1.2 * uniform(config_id, COMP_EPS_OCC, np.arange(o))
     ev = 0.2 + 1.8 * uniform(config_id, COMP_EPS_VIR, np.arange(v))
@@ -133,12 +153,19 @@ def tei_canonical(p, q, r, s, n: int):
     return lo * (n * n) + hi, sign
 
 
-def tei(config_id: int, o: int, v: int, sigma: float = 0.05, chunk: int = 1 << 22):
-    """Dense antisymmetrized TEI (n, n, n, n) float64."""
+def tei(config_id: int, o: int, v: int, sigma: float = 0.05, chunk: int = 1 << 20, workers: int | None = None):
+    """Dense antisymmetrized TEI (n, n, n, n) float64.
+
+    Chunks of the flat index are generated by a thread pool (numpy releases
+    the GIL inside its array operations); every element is a pure function of
+    its index, so the result does not depend on the chunking or the threads."""
+    from concurrent.futures import ThreadPoolExecutor
+
     n = o + v
     out = np.empty(n ** 4, dtype=np.float64)
     total = n ** 4
-    for start in range(0, total, chunk):
+
+    def fill(start):
         lin = np.arange(start, min(total, start + chunk), dtype=np.int64)
         s = lin % n
         r = (lin // n) % n
@@ -146,6 +173,16 @@ def tei(config_id: int, o: int, v: int, sigma: float = 0.05, chunk: int = 1 << 2
         p = lin // (n * n * n)
         idx, sign = tei_canonical(p, q, r, s, n)
         out[start:start + lin.size] = sign * (sigma * normal(config_id, COMP_TEI, idx))
+
+    if workers is None:
+        workers = min(16, len(os.sched_getaffinity(0)))
+    starts = range(0, total, chunk)
+    if workers <= 1 or total <= chunk:
+        for st in starts:
+            fill(st)
+    else:
+        with ThreadPoolExecutor(workers) as ex:
+            list(ex.map(fill, starts))
     return out.reshape(n, n, n, n)
 
 
@@ -172,6 +209,37 @@ def tei_block(config_id: int, o: int, v: int, classes: str, sigma: float = 0.05)
     p, q, r, s = np.meshgrid(*axes, indexing="ij")
     idx, sign = tei_canonical(p, q, r, s, n)
     return sign * (sigma * normal(config_id, COMP_TEI, idx))
+
+
+def mp2_t2(config_id: int, o: int, v: int, sigma: float = 0.05):
+    """MP2 amplitudes <ij||ab> / (e_i + e_j - e_a - e_b) from the sorted
+    orbital energies (the denominators of the reference's MP2_E,
+    CCSD_Helper.py:102-124, with eigh(F) eigenvalues)."""
+    eps = np.linalg.eigh(fock(config_id, o, v))[0]
+    eo, ev = eps[:o], eps[o:]
+    D = eo.reshape(-1, 1, 1, 1) + eo.reshape(-1, 1, 1) - ev.reshape(-1, 1) - ev
+    return tei_block(config_id, o, v, "oovv", sigma) / D
+
+
+def perturbed_state(config_id: int, o: int, v: int, sigma: float = 0.05, scale1: float = 0.02,
+                    scale2: float = 0.005):
+    """A seeded complex test state (t1, t2, lam1, lam2): t1, lam1 ~ scale1 (N + iN);
+    t2, lam2 = MP2 amplitudes + scale2 (N + iN) antisymmetrized in (i, j) and
+    (a, b).  Every element is a pure function of its index (components
+    COMP_AMP..COMP_AMP+7), so host and tests on any machine build the same state."""
+    mp2 = mp2_t2(config_id, o, v, sigma)
+    out = []
+    for k, shape in enumerate(((o, v), (o, o, v, v), (o, v), (o, o, v, v))):
+        size = int(np.prod(shape))
+        x = (normals(config_id, COMP_AMP + 2 * k, size) + 1j * normals(config_id, COMP_AMP + 2 * k + 1, size))
+        x = x.reshape(shape)
+        if len(shape) == 2:
+            out.append(scale1 * x)
+        else:
+            x = x - x.transpose(1, 0, 2, 3)
+            x = x - x.transpose(0, 1, 3, 2)
+            out.append(mp2 + (0.25 * scale2) * x)
+    return tuple(out)
 
 
 class SyntheticSystem:

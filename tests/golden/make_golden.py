@@ -53,6 +53,7 @@ def load_reference(ref_dir):
     oe.contract = lambda *a, **k: np.einsum(*a, optimize=True)
     sys.modules["opt_einsum"] = oe
     sys.path.insert(0, ref_dir)
+    sys.dont_write_bytecode = True  # leave no .pyc inside the read-only reference tree
     import CC2_Helper  # noqa: E402
     return CC2_Helper.CC2_Helper
 

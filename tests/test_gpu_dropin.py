@@ -87,3 +87,28 @@ def test_timeout_stop_then_restart(g, tmp_path, monkeypatch):
     tr2 = rk4.propagate(eq, r["F"], r["t1"], r["t2"], r["lam1"], r["lam2"], g["mu"], r["w0"], r["A"], r["t0"],
                         r["dt"], method="CCSD", max_steps=n2)
     assert np.abs(mol2.mu_history - np.array(tr2.mu)).max() <= 1e-11
+
+
+def test_divergence_stop_saves_rounded_base(g, tmp_path, monkeypatch):
+    """A = 100 drives |Re mu_z| above 100 at step 22, inside the fifth chunk
+    (steps 16..31): the run is replayed up to that step, and the files hold the
+    rounded base of step 22 and the time t - dt, as the reference saves them
+    (CCSD_Helper.py:1248-1251)."""
+    monkeypatch.chdir(tmp_path)
+    print("\n[dropin divergence stop CCSD]", flush=True)
+    o = int(g["o"])
+    eq = CCEquations(g["TEI"], o)
+    big_A = 100.0
+    mol = CCSD_Helper(provider(g))
+    mol.verbose = False
+    mol.Runge_Kutta_solver(g["F"], *amplitudes(g), W0, big_A, 0.0, 10.1, DT, 1000.0, 15)
+    tr = rk4.propagate(eq, g["F"], *amplitudes(g), g["mu"], W0, big_A, 0.0, DT, method="CCSD", tf=10.1)
+    assert tr.stop_reason == "divergence" and tr.steps == 22
+    assert mol.stop_reason == "divergence"
+    assert len(mol.data) == 22
+    scale = np.abs(np.array(tr.mu)).max()
+    assert np.abs(mol.mu_history - np.array(tr.mu)).max() <= 1e-10 * scale
+    r = restart_io.read_restart()
+    assert abs(r["t0"] - DT - tr.saved_time) <= 1e-10  # the reader adds dt back (CC_Calculator.py:80)
+    for got, ref in zip((r["t1"], r["t2"], r["lam1"], r["lam2"]), tr.saved):
+        assert np.abs(got - ref).max() <= 1e-10 * max(1.0, np.abs(ref).max())
