@@ -98,6 +98,23 @@ int tdcc_intermediate(tdcc_ctx* ctx, const char* name, const double* F, const do
 int tdcc_dipole(tdcc_ctx* ctx, const double* t1, const double* t2, const double* l1,
                 const double* l2, double* mu_out);
 
+/* No-allocation variant of tdcc_dipole for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  It has no counterpart in the reference.
+ * t1, t2, lam1, lam2 and mu_out are DEVICE pointers: the amplitudes
+ * interleaved complex in C order (the memory of a complex128 tensor), mu_out
+ * 3 complex (x, y, z; 6 doubles).  The result is that of tdcc_dipole for the
+ * same amplitudes (the same program, bit for bit).  The call allocates
+ * nothing: the program is planned at tdcc_create, and its scratch (density
+ * blocks, contraction temporaries, reduction partials) lives in a static
+ * device array of 64 MiB per GPU, shared by every call on that GPU and kept
+ * until the process ends (calls that overlap on one GPU race).  A system
+ * whose dipole program needs more scratch (o, v far beyond the workloads
+ * here) gets an error naming the size.  The amplitudes pass through the
+ * context's own input planes; the call returns after the result is written. */
+int tdcc_dipole_device_noalloc(tdcc_ctx* ctx, const double* t1, const double* t2, const double* l1,
+                               const double* l2, double* mu_out);
+
 /* Run `nsteps` iterations of the reference's RK4 loop body on the current state
  * (CCSD_Helper.py:1206-1240 / CC2_Helper.py:453-485 without the stop rules and
  * the file output, which the caller applies): time starts at t0 and is

@@ -19,6 +19,7 @@ EXPORTS = (
     "tdcc_rk4_run", "tdcc_profile_step", "tdcc_checkpoint", "tdcc_step_info",
     "tdcc_set_batch", "tdcc_set_state_slot", "tdcc_get_state_slot", "tdcc_rk4_run_batch",
     "tdcc_dipole_gradient", "tdcc_write_restart_block", "tdcc_io_error",
+    "tdcc_dipole_device_noalloc",
 )
 
 RHS_T1, RHS_T2, RHS_L1, RHS_L2, RHS_T2_CC2, RHS_L1_CC2, RHS_L2_CC2 = range(7)
@@ -56,6 +57,7 @@ def load():
         "tdcc_rhs": (ctypes.c_int, [vp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
         "tdcc_intermediate": (ctypes.c_int, [vp, ctypes.c_char_p, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int64]),
         "tdcc_dipole": (ctypes.c_int, [vp, _dp, _dp, _dp, _dp, _dp]),
+        "tdcc_dipole_device_noalloc": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),
         "tdcc_rk4_run": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp]),
         "tdcc_profile_step": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -220,6 +222,31 @@ class Context:
         self._shape_ok(c, d)
         out = np.empty(3, complex)
         self._check(self.lib.tdcc_dipole(self._h, pa, pb, pc, pd, out.ctypes.data_as(_dp)), "tdcc_dipole")
+        return out
+
+    def dipole_device_noalloc(self, t1, t2, lam1, lam2, out):
+        """tdcc_dipole_device_noalloc on device memory: the amplitudes as
+        interleaved complex128 in C order with the shapes (o,v), (o,o,v,v),
+        (o,v), (o,o,v,v), and `out` 6 doubles (x, y, z as re, im pairs).  Each
+        argument is a device address (int) or a tensor with data_ptr() (then
+        contiguous complex128 of that shape, or 6 float64 for `out`).  A torch
+        tensor must come from the same HIP runtime as this library: import
+        torch before the library is first loaded (torch bundles its own
+        runtime, which is loaded as a second copy otherwise)."""
+        o, v = self.o, self.v
+
+        def addr(x, shape, dtype_ok):
+            if isinstance(x, int):
+                return ctypes.c_void_p(x)
+            if tuple(x.shape) != shape or not x.is_contiguous() or not dtype_ok(x):
+                raise ValueError(f"expected a contiguous device tensor of shape {shape}")
+            return ctypes.c_void_p(x.data_ptr())
+
+        cplx = lambda x: str(x.dtype).endswith("complex128")
+        f64 = lambda x: str(x.dtype).endswith("float64")
+        ptrs = [addr(x, sh, cplx) for x, sh in ((t1, (o, v)), (t2, (o, o, v, v)), (lam1, (o, v)), (lam2, (o, o, v, v)))]
+        po = addr(out, (6,), f64)
+        self._check(self.lib.tdcc_dipole_device_noalloc(self._h, *ptrs, po), "tdcc_dipole_device_noalloc")
         return out
 
     def rk4_run(self, method, nsteps, t0, dt, w0, A, pol=2, precs=15):

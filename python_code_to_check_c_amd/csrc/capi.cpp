@@ -7,6 +7,7 @@
 #include <array>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -89,6 +90,14 @@ struct tdcc_ctx {
   double* ttab = nullptr;
   int64_t vcap = 0;
   double* grad = nullptr;  // [4] dL/dA, dL/dw0 (complex), then (n,n) re and im planes of dL/dmu_pol
+  // no-allocation dipole (tdcc_dipole_device_noalloc): its own runtime (no
+  // counter; the trace writes row 0 of na_mu), a program planned and bound at
+  // tdcc_create with its arena in the device's static scratch array, or the
+  // reason it could not be
+  Runtime rt_na;
+  double* na_mu = nullptr;
+  std::unique_ptr<Graph> na_prog;
+  std::string na_err;
 };
 
 namespace {
@@ -706,6 +715,57 @@ Program* prepare_run(tdcc_ctx* c, int method, int nsteps, double t0, double dt, 
   return &p;
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of the no-allocation dipole entry point: the arena of its program
+// (density blocks, contraction temporaries, reduction partials) lives in this
+// file-scope device array instead of a hipMalloc'd buffer.  Each GPU holds its
+// own copy for the life of the process; the host-side name is not a device
+// address, so host code uses only the pointer that hipGetSymbolAddress returns
+// for the current device (scratch_ptr).  Every call on one GPU shares it:
+// calls that overlap on one GPU race.
+constexpr int64_t kNoAllocScratchBytes = int64_t(64) << 20;
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch_ptr;  // device -> address of its scratch copy
+
+// Address of the scratch array on `device`, which must be the current device.
+void* scratch_ptr(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(device);
+  if (it != g_scratch_ptr.end()) return it->second;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    throw Error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+  g_scratch_ptr[device] = p;
+  return p;
+}
+
+// The dipole program of tdcc_dipole, planned once at tdcc_create and bound to
+// the static scratch (when its arena fits there).
+void build_noalloc_dipole(tdcc_ctx* c) {
+  c->rt_na = c->rt;
+  c->rt_na.counter = nullptr;
+  c->rt_na.mu_out = c->na_mu;
+  c->rt_na.batch.clear();
+  auto g = std::make_unique<Graph>(c->o, c->v, &c->rt_na);
+  const int tei = ext_tei(c, *g);
+  int y[4];
+  for (int k = 0; k < 4; ++k) y[k] = ext_planes(*g, c->io_in[k], shape_of_block(k), "in");
+  CCBuilder b(*g, tei);
+  g->trace(b.Dij(y[0], y[1], y[2], y[3]), y[2], b.Dai(y[0], y[1], y[2], y[3]), b.Dab(y[0], y[1], y[2], y[3]));
+  g->plan();
+  if (g->arena_bytes() > kNoAllocScratchBytes) {
+    c->na_err = "the dipole program of o=" + std::to_string(c->o) + ", v=" + std::to_string(c->v) + " needs " +
+                std::to_string(g->arena_bytes()) + " bytes of scratch; the static scratch holds " +
+                std::to_string(kNoAllocScratchBytes);
+    return;
+  }
+  g->bind(static_cast<double*>(scratch_ptr(c->device)));
+  c->na_prog = std::move(g);
+}
+
 }  // namespace
 
 extern "C" {
@@ -747,6 +807,8 @@ int tdcc_create(int nocc, int nvir, int device, tdcc_ctx** out) {
     c->rt.mu = c->mu;
     c->rt.counter = c->counter;
     c->pairs = c->backup_pairs = antisym_ij(nullptr, c->o, c->v);  // zero amplitudes
+    TDCC_HIP(hipMalloc(&c->na_mu, 6 * sizeof(double)));
+    build_noalloc_dipole(c);
     TDCC_HIP(hipStreamSynchronize(c->stream));
   });
   if (rc) {
@@ -763,6 +825,8 @@ void tdcc_destroy(tdcc_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto& kv : c->progs) invalidate_exec(kv.second);
   c->progs.clear();
+  c->na_prog.reset();
+  if (c->na_mu) (void)hipFree(c->na_mu);
   for (int k = 0; k < 4; ++k) {
     free_planes(c->state[k]);
     free_planes(c->base[k]);
@@ -1139,6 +1203,21 @@ int tdcc_dipole(tdcc_ctx* c, const double* t1, const double* t2, const double* l
     launch_counter_set(c->counter, 1, c->stream);
     run_program(c, p, false);
     TDCC_HIP(hipMemcpyAsync(mu_out, c->mu_out, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    TDCC_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int tdcc_dipole_device_noalloc(tdcc_ctx* c, const double* t1, const double* t2, const double* l1, const double* l2,
+                               double* mu_out) {
+  return guarded(c, [&] {
+    if (!c->have_system) throw Error("system not set: call tdcc_set_system");
+    if (!t1 || !t2 || !l1 || !l2 || !mu_out) throw Error("all arguments are required");
+    if (!c->na_prog) throw Error(c->na_err.empty() ? "no-allocation dipole program missing" : c->na_err);
+    const double* d[4] = {t1, t2, l1, l2};
+    for (int k = 0; k < 4; ++k) launch_split_complex(d[k], c->io_in[k].re(), c->io_in[k].im(), c->io_in[k].n, c->stream);
+    c->na_prog->run(c->stream);
+    TDCC_HIP(hipMemcpyAsync(mu_out, c->na_mu, 6 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    TDCC_HIP(hipGetLastError());
     TDCC_HIP(hipStreamSynchronize(c->stream));
   });
 }
