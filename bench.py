@@ -54,36 +54,72 @@ def initial_state(sysm, o, v):
     return t1, t2, t1.copy(), t2.copy()
 
 
-def cpu_baseline(method, o_full, v_full, budget_s=20.0):
-    """Time the oracle's RK4 step at a reduced shape and scale by F_ref."""
+def _host_info():
+    """Host BLAS, CPU model and thread count, for the CPU-baseline record."""
+    info = {"cores": len(os.sched_getaffinity(0))}
+    try:
+        import threadpoolctl
+        np.dot(np.ones((64, 64)), np.ones((64, 64)))
+        pools = [p for p in threadpoolctl.threadpool_info() if p.get("user_api") == "blas"]
+        if pools:
+            p = pools[0]
+            info["blas"] = f"{p.get('internal_api')} {p.get('version')} ({p.get('architecture')}), " \
+                           f"{p.get('num_threads')} threads"
+    except Exception as e:  # the record still carries the rest
+        info["blas"] = f"unknown ({e})"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    info["cpu_model"] = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return info
+
+
+def cpu_baseline(method, o_full, v_full, sample=(20, 96)):
+    """The oracle (numpy complex128, the reference's einsum strings with
+    optimize='optimal', host BLAS) on one evaluation of each of the method's
+    four right-hand sides at the sample shape: a quarter of an RK4 step's
+    as-written work (each chain's first stage).  The step time is four times
+    that plus the dipole; it is scaled to the workload by F_ref(full) /
+    F_ref(sample), the flop ratio of the two shapes' as-written steps."""
+    import resource
+
     from oracle import rk4
     from oracle.cc_equations import CCEquations
-    from oracle.flops import step_flops
+    from oracle.flops import rhs_flops, step_flops
     from python_code_to_check_c_amd import synthetic
-    o, v = min(o_full, 20), min(v_full, 24)
-    s = synthetic.SyntheticSystem(90, o, v, tei_sigma=0.02)
+    o, v = min(o_full, sample[0]), min(v_full, sample[1])
+    s = synthetic.SyntheticSystem(90, o, v, tei_sigma=0.05)
     eq = CCEquations(s.TEI, o)
-    t1, t2, l1, l2 = initial_state(s, o, v)
-    Vt = lambda x: rk4.field(s.mu[2], 0.002, 0.923, x)
-    steps, t0 = 0, time.perf_counter()
-    t = 0.0
-    state = (t1, t2, l1, l2)
-    while True:
-        t += 0.001
-        state, _ = rk4.rk4_step(eq, method, s.F, *state, t, 0.001, Vt, 15)
-        eq.dipole_moment(*state, s.mu)
-        steps += 1
-        if time.perf_counter() - t0 > budget_s or steps >= 50:
-            break
-    el = time.perf_counter() - t0
+    t1, t2, l1, l2 = synthetic.perturbed_state(90, o, v, 0.05)
+    F = s.F + rk4.field(s.mu[2], 0.002, 0.923, 0.001)
+    names = ("T1eq_rhs", "T2eq_rhs", "lam_1eq_rhs", "lam2eq_rhs") if method == "CCSD" else \
+        ("T1eq_rhs", "T2eq_rhs_CC2", "L1eq_rhs_CC2", "L2eq_rhs_CC2")
+    t0 = time.perf_counter()
+    for nm in names:
+        f = getattr(eq, nm)
+        f(t1, t2, F) if nm in ("T1eq_rhs", "T2eq_rhs", "T2eq_rhs_CC2") else f(t1, t2, l1, l2, F)
+    el_rhs = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    eq.dipole_moment(t1, t2, l1, l2, s.mu)
+    el_dip = time.perf_counter() - t0
+    step_s = 4.0 * el_rhs + el_dip
     f_small = step_flops(o, v, method)[0]
     f_full = step_flops(o_full, v_full, method)[0]
-    sps = steps / el * f_small / f_full
-    cores = len(os.sched_getaffinity(0))
-    return {"value": sps, "unit": "steps/s", "cores": cores, "kind": "port",
-            "sample": f"oracle (numpy complex128 einsum, reference strings) {steps} {method} RK4 steps at "
-                      f"o={o}, v={v} in {el:.2f} s on {cores} host threads, scaled to o={o_full}, v={v_full} "
-                      f"by F_ref {f_small:.3e} -> {f_full:.3e} flop/step"}
+    f_quarter = sum(rhs_flops(o, v, nm) for nm in names)
+    info = _host_info()
+    rss_gb = resource.getrusage(resource.RUSAGE_SELF).ru_maxrss / 2**20
+    return {"value": (1.0 / step_s) * f_small / f_full, "unit": "steps/s", "cores": info["cores"], "kind": "port",
+            "sample": f"oracle (numpy complex128, reference einsum strings, optimize='optimal') at o={o}, v={v}: "
+                      f"one evaluation of each of {', '.join(names)} ({f_quarter:.3e} flop as written, a quarter "
+                      f"of the step) in {el_rhs:.2f} s ({f_quarter / el_rhs / 1e9:.1f} GFLOP/s) and the dipole in "
+                      f"{el_dip:.3f} s -> {step_s:.1f} s per RK4 step; scaled to o={o_full}, v={v_full} by F_ref "
+                      f"{f_small:.3e} -> {f_full:.3e} flop/step (x{f_full / f_small:.2f})",
+            "blas": info.get("blas"), "cpu_model": info.get("cpu_model"),
+            "process_peak_rss_GiB": round(rss_gb, 2)}
 
 
 def roofline(records, digits):
@@ -125,8 +161,8 @@ def main():
     ap.add_argument("--o", type=int, default=20)
     ap.add_argument("--v", type=int, default=160)
     ap.add_argument("--config-id", type=int, default=3)
-    ap.add_argument("--tei-sigma", type=float, default=0.005)
-    ap.add_argument("--digits", type=int, default=4,
+    ap.add_argument("--tei-sigma", type=float, default=0.05)
+    ap.add_argument("--digits", type=int, default=5,
                     help="int8 digits per operand of the split-precision path (0: FP64 only)")
     ap.add_argument("--error-steps", type=int, default=0,
                     help="also propagate this many steps in both precisions and report the dipole difference")
@@ -213,16 +249,16 @@ def main():
             sps, el, mu = sps64, el64, mu64
 
     f_ref, f_alg = step_flops(o, v, args.method)
-    prec = (f"split: int8 digit planes ({args.digits} digits, Ozaki scheme, exact int32 accumulation) for the "
-            "ladders and contractions above 2 GFLOP; FP64 state, field, small and dot-product contractions "
-            "and dipole"
+    prec = (f"split: int8 digit planes ({args.digits} digits of radix 254 per operand, Ozaki scheme, exact int32 "
+            "accumulation on int8 MFMA; not FP32 hi/lo) for the ladders and contractions above 2 GFLOP; FP64 state, "
+            "field, small and dot-product contractions and dipole"
             if split else "fp64")
     out = {
         "metric": METRIC, "value": sps, "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64+int8-split" if split else "f64",
         "data": "synthetic",
-        "config": {"workload": f"T{args.method} (T+Lambda) partial-RK4 propagation, o={o}, v={v} spin orbitals, "
+        "config": {"workload": f"TD{args.method} (T+Lambda) partial-RK4 propagation, o={o}, v={v} spin orbitals, "
                                f"dt={dt}, z field w0={w0} A={A}, precs={precs}",
                    "method": args.method, "precision": prec, "digits": args.digits, "o": o, "v": v,
                    "tei_sigma": args.tei_sigma, "config_id": args.config_id,
@@ -238,6 +274,9 @@ def main():
         err = mu[:, 2] - mu64[:, 2]
         out["config"]["dipole_z_err_vs_fp64"] = {"steps": args.steps, "re": float(np.abs(err.real).max()),
                                                  "im": float(np.abs(err.imag).max())}
+    if split and not args.split_only:
+        out["config"]["residual_err_vs_fp64_at_initial_state"] = residual_errors(
+            ctx, args.method, args.digits, sysm, o, v, rk4_field(sysm, A, w0, dt, pol))
     if split and args.error_steps > 0:
         out["config"]["dipole_z_err_vs_fp64_long"] = long_error(ctx, args, dt, w0, A, pol, precs)
     if not args.no_profile:
@@ -437,9 +476,42 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def rk4_field(sysm, A, w0, t, pol):
+    """F + V(t) with V(t) = -A mu_pol exp(i 2 pi w0 t) (CCSD_Helper.py:1197-1200)."""
+    import cmath
+    return sysm.F + (-A * sysm.mu[pol] * cmath.exp(1j * w0 * 2 * np.pi * t))
+
+
+def residual_errors(ctx, method, digits, sysm, o, v, F):
+    """Split-precision residuals against the FP64 ones at the initial state and
+    F + V(dt): max-abs difference and its ratio to the FP64 residual's largest
+    element, per residual."""
+    from python_code_to_check_c_amd import _lib
+    kinds = ((("T1eq_rhs", _lib.RHS_T1), ("T2eq_rhs", _lib.RHS_T2), ("lam_1eq_rhs", _lib.RHS_L1),
+              ("lam2eq_rhs", _lib.RHS_L2)) if method == "CCSD" else
+             (("T1eq_rhs", _lib.RHS_T1), ("T2eq_rhs_CC2", _lib.RHS_T2_CC2), ("L1eq_rhs_CC2", _lib.RHS_L1_CC2),
+              ("L2eq_rhs_CC2", _lib.RHS_L2_CC2)))
+    X0 = initial_state(sysm, o, v)
+    ref = {}
+    ctx.set_precision(0)
+    for name, kind in kinds:
+        ref[name] = ctx.rhs(kind, F, *X0)
+    ctx.set_precision(digits)
+    out = {}
+    for name, kind in kinds:
+        d = np.abs(ctx.rhs(kind, F, *X0) - ref[name]).max()
+        m = np.abs(ref[name]).max()
+        out[name] = {"max_abs": float(d), "rel_to_max": float(d / m) if m > 0 else None, "max_abs_fp64": float(m)}
+    return out
+
+
 def long_error(ctx, args, dt, w0, A, pol, precs, chunk=250):
     """Propagate error_steps steps from X0 in FP64 and in split precision and
-    return the max |mu_z| difference (re and im) over the run."""
+    return the max |mu_z| difference (re and im).  Where the FP64 trajectory
+    leaves the finite numbers the comparison ends at its last finite step;
+    `reference_stop_step` is the first step at which the reference's loop
+    would stop on divergence (|Re mu_z| > 100, CCSD_Helper.py:1248), and the
+    difference is also reported over the steps before it."""
     traj = {}
     for digits in (0, args.digits):
         ctx.set_precision(digits)
@@ -453,10 +525,26 @@ def long_error(ctx, args, dt, w0, A, pol, precs, chunk=250):
             t = float(ts[-1])
             done += n
             log(f"error run digits={digits}: {done}/{args.error_steps} steps")
+            if not np.all(np.isfinite(mu[:, 2])):
+                break
         traj[digits] = np.concatenate(mus)
-    d = traj[args.digits][:, 2] - traj[0][:, 2]
-    return {"steps": args.error_steps, "re": float(np.abs(d.real).max()), "im": float(np.abs(d.imag).max()),
-            "mu_z_excursion_fp64": float(np.abs(traj[0][:, 2] - traj[0][0, 2]).max())}
+    ref = traj[0][:, 2]
+    fin = np.isfinite(ref)
+    n_fin = int(np.argmin(fin)) if not fin.all() else len(ref)
+    big = np.abs(ref.real) > 100
+    n_stop = int(np.argmax(big)) + 1 if big.any() else None
+    n = min(n_fin, len(traj[args.digits]))
+    d = traj[args.digits][:n, 2] - ref[:n]
+    res = {"steps": args.error_steps, "compared_steps": n, "re": float(np.abs(d.real).max()),
+           "im": float(np.abs(d.imag).max()),
+           "mu_z_excursion_fp64": float(np.abs(ref[:n] - ref[0]).max()),
+           "fp64_first_nonfinite_step": None if fin.all() else n_fin + 1,
+           "reference_stop_step": n_stop}
+    if n_stop:
+        w = min(n_stop - 1, n)
+        res["before_reference_stop"] = ({"steps": w, "re": float(np.abs(d[:w].real).max()),
+                                         "im": float(np.abs(d[:w].imag).max())} if w > 0 else {"steps": 0})
+    return res
 
 
 if __name__ == "__main__":

@@ -175,6 +175,26 @@ int tdcc_step_info(tdcc_ctx* ctx, int method, double dt, int64_t* flops, int64_t
  * Status 0 = ok; tdcc_io_error() gives the message of the last failure on
  * the calling thread. */
 int tdcc_write_restart_block(const double* x, int rank, const int64_t* dims, int precs, const char* path);
+
+/* Restart block reader, replacing convert_2data / convert_4data
+ * (CC_Calculator.py:85-113): out (dims, C order, float64) is zeroed, then each
+ * row "i<TAB>a<TAB>value" or "i<TAB>j<TAB>a<TAB>b<TAB>value" assigns value
+ * (parsed to the nearest double) at the indices (numbers rounded to the
+ * nearest integer), in file order; blank lines are skipped, a malformed row or
+ * an index outside dims is an error.  No context, no device. */
+int tdcc_read_restart_block(const char* path, int rank, const int64_t* dims, double* out);
+
+/* Exact binary snapshot of a stopped run, for resuming bit for bit (the text
+ * blocks above quantize the state and the reference resumes one step later in
+ * time, CC_Calculator.py:77-80; there is no reference counterpart).  Layout,
+ * little-endian: 16-byte magic "TDCC-STATE-V1", int32 o, v, method, precs,
+ * float64 t, dt, w0, A (params[0..3]), float64 F[(o+v)^2], then t1, t2, lam1,
+ * lam2 as interleaved complex in C order, then a 64-bit FNV-1a checksum of all
+ * preceding bytes.  Written to path + ".tmp" and renamed. */
+int tdcc_write_state_file(const char* path, int o, int v, int method, int precs, const double* params,
+                          const double* F, const double* t1, const double* t2, const double* l1, const double* l2);
+int tdcc_read_state_file_header(const char* path, int* o, int* v, int* method, int* precs, double* params);
+int tdcc_read_state_file(const char* path, double* F, double* t1, double* t2, double* l1, double* l2);
 const char* tdcc_io_error(void);
 
 #ifdef __cplusplus

@@ -161,3 +161,18 @@ def step_flops(o, v, method="CCSD"):
     for f in (eq.Dij, eq.Dab, eq.Dai):
         f(t1, t2, l1, l2)
     return cnt.total, cnt.unique
+
+
+def rhs_flops(o, v, name):
+    """Flops of one evaluation of the named right-hand side (e.g. 'T2eq_rhs',
+    'lam2eq_rhs') at (o, v), each contraction costed as above."""
+    cnt = Counter(o, v)
+    eq = _SymEquations(cnt)
+    t1, t2 = Sym("ov", True, "t1"), Sym("oovv", True, "t2")
+    l1, l2 = Sym("ov", True, "l1"), Sym("oovv", True, "l2")
+    F = Sym("nn", True, "F")
+    if name in ("T1eq_rhs", "T2eq_rhs", "T2eq_rhs_CC2"):
+        getattr(eq, name)(t1, t2, F)
+    else:
+        getattr(eq, name)(t1, t2, l1, l2, F)
+    return cnt.total

@@ -215,6 +215,11 @@ class CCSD_Helper(object):
                    pol=2, max_steps=None, chunk_seconds=0.5, max_chunk=256):
         """The reference loop on the device, in chunks of graph-replayed steps.
 
+        On a stop the reference's text files are written (rounded base of the
+        stopping step, time t - dt) and, beside them, the exact snapshot of the
+        state after that step and its time t (restart_io.save_state), from
+        which CC_Calculator.TDCC_restart(exact=True) resumes bit for bit.
+
         The stop rules are applied after every step, in the reference's order
         (CCSD_Helper.py:1234-1251): first the wall clock (elapsed time at the
         end of the step's update above 0.9 * timeout minutes), then divergence
@@ -280,6 +285,7 @@ class CCSD_Helper(object):
                 data, timing = self._frames(rows, trows)
                 self.Save_data(F, *base, data, timing, precs, restart)
                 self.Save_parameters(w0, A, t0, t - dt, dt, precs, self.nocc, self.nvir, method)
+                restart_io.save_state(F, *ctx.get_state(0), t, dt, w0, A, precs, method)
                 saved = reason
                 break
             if per * chunk < chunk_seconds:

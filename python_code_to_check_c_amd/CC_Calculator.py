@@ -40,9 +40,19 @@ class CC_Calculator(object):
             print("Error in specifying whether it is a CCSD or CC2 calculation")
             print("Correct format is:\nmol.TDCC(stop_time, CCSD)\nwhere CCSD='CCSD' or 'CC2'")
 
-    def TDCC_restart(self, timeout):
-        """CC_Calculator.py:70-145: resume from the files a stopped run saved."""
-        r = restart_io.read_restart()
+    def TDCC_restart(self, timeout, exact=False):
+        """CC_Calculator.py:70-145: resume from the files a stopped run saved.
+
+        exact=True (no reference counterpart) resumes from the binary snapshot
+        instead: the state after the stopping step at its exact time, so the
+        resumed run continues the interrupted one bit for bit; tf is the
+        reference's 50 + the saved t0."""
+        if exact:
+            r = restart_io.read_state()
+            r["tf"] = restart_io.read_parameters()["tf"]
+            r["t0"] = r["t"]
+        else:
+            r = restart_io.read_restart()
         if r["method"] == 'CC2':
             mol = CC2_Helper(self.psi, digits=self.digits)
             mol.Runge_Kutta_solver_CC2(r["F"], r["t1"], r["t2"], r["lam1"], r["lam2"], r["w0"], r["A"],

@@ -19,7 +19,8 @@ EXPORTS = (
     "tdcc_rk4_run", "tdcc_profile_step", "tdcc_checkpoint", "tdcc_step_info",
     "tdcc_set_batch", "tdcc_set_state_slot", "tdcc_get_state_slot", "tdcc_rk4_run_batch",
     "tdcc_dipole_gradient", "tdcc_write_restart_block", "tdcc_io_error",
-    "tdcc_dipole_device_noalloc",
+    "tdcc_dipole_device_noalloc", "tdcc_read_restart_block", "tdcc_write_state_file",
+    "tdcc_read_state_file_header", "tdcc_read_state_file",
 )
 
 RHS_T1, RHS_T2, RHS_L1, RHS_L2, RHS_T2_CC2, RHS_L1_CC2, RHS_L2_CC2 = range(7)
@@ -75,6 +76,13 @@ def load():
                                               ctypes.POINTER(ctypes.c_int), _dp, _dp, ctypes.c_int, _dp, _dp]),
         "tdcc_write_restart_block": (ctypes.c_int, [_dp, ctypes.c_int, i64p, ctypes.c_int, ctypes.c_char_p]),
         "tdcc_io_error": (ctypes.c_char_p, []),
+        "tdcc_read_restart_block": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, i64p, _dp]),
+        "tdcc_write_state_file": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                 ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
+        "tdcc_read_state_file_header": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int),
+                                                       ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                                       ctypes.POINTER(ctypes.c_int), _dp]),
+        "tdcc_read_state_file": (ctypes.c_int, [ctypes.c_char_p, _dp, _dp, _dp, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -347,3 +355,54 @@ def write_restart_block(x, path, precs):
     rc = lib.tdcc_write_restart_block(a.ctypes.data_as(_dp), a.ndim, dims, int(precs), os.fsencode(path))
     if rc:
         raise TdccError(f"tdcc_write_restart_block: {lib.tdcc_io_error().decode()}")
+
+
+def read_restart_block(path, shape):
+    """One restart block file (rank 2 or 4) through the C++ reader: a dense
+    float64 array of `shape` (zero where the file has no row)."""
+    lib = load()
+    shape = tuple(int(x) for x in shape)
+    if len(shape) not in (2, 4):
+        raise ValueError("restart blocks are rank 2 or 4")
+    out = np.empty(shape, dtype=np.float64)
+    dims = (ctypes.c_int64 * len(shape))(*shape)
+    rc = lib.tdcc_read_restart_block(os.fsencode(path), len(shape), dims, out.ctypes.data_as(_dp))
+    if rc:
+        raise TdccError(f"tdcc_read_restart_block: {lib.tdcc_io_error().decode()}")
+    return out
+
+
+def write_state_file(path, method, precs, t, dt, w0, A, F, t1, t2, lam1, lam2):
+    """Exact binary snapshot of a stopped run (include/tdcc.h, tdcc_write_state_file)."""
+    lib = load()
+    o, v = np.shape(t1)
+    F_, pF = _r(np.real(F))
+    arrs = [_c(x) for x in (t1, t2, lam1, lam2)]
+    if F_.shape != (o + v, o + v) or arrs[1][0].shape != (o, o, v, v) or arrs[2][0].shape != (o, v) \
+            or arrs[3][0].shape != (o, o, v, v):
+        raise ValueError("state file arrays do not have the shapes of one (o, v) system")
+    params = np.array([t, dt, w0, A], dtype=np.float64)
+    rc = lib.tdcc_write_state_file(os.fsencode(path), o, v, METHODS[method], int(precs), params.ctypes.data_as(_dp),
+                                   pF, *[p for _, p in arrs])
+    if rc:
+        raise TdccError(f"tdcc_write_state_file: {lib.tdcc_io_error().decode()}")
+
+
+def read_state_file(path):
+    """Everything tdcc_write_state_file stored: dict with method, precs, t, dt,
+    w0, A, F (real), t1, t2, lam1, lam2."""
+    lib = load()
+    o, v, m, pr = (ctypes.c_int() for _ in range(4))
+    params = np.empty(4)
+    if lib.tdcc_read_state_file_header(os.fsencode(path), ctypes.byref(o), ctypes.byref(v), ctypes.byref(m),
+                                       ctypes.byref(pr), params.ctypes.data_as(_dp)):
+        raise TdccError(f"tdcc_read_state_file_header: {lib.tdcc_io_error().decode()}")
+    o, v = o.value, v.value
+    F = np.empty((o + v, o + v))
+    out = [np.empty((o, v), complex), np.empty((o, o, v, v), complex), np.empty((o, v), complex),
+           np.empty((o, o, v, v), complex)]
+    if lib.tdcc_read_state_file(os.fsencode(path), F.ctypes.data_as(_dp), *[x.ctypes.data_as(_dp) for x in out]):
+        raise TdccError(f"tdcc_read_state_file: {lib.tdcc_io_error().decode()}")
+    method = {v_: k for k, v_ in METHODS.items()}[m.value]
+    return dict(method=method, precs=pr.value, t=float(params[0]), dt=float(params[1]), w0=float(params[2]),
+                A=float(params[3]), F=F, t1=out[0], t2=out[1], lam1=out[2], lam2=out[3])

@@ -265,14 +265,26 @@ __device__ __forceinline__ void store_mirrored(double* re, double* im, int64_t o
 // ---------------------------------------------------------------------------
 // Split-precision GEMM on int8 MFMA (Ozaki scheme with exact accumulation).
 //
-// A row x of an FP64 matrix is written as x = 2^e * sum_{s=1..S} d_s 128^-s
-// (+ a remainder below 0.5 * 128^-S * 2^e), with the row exponent e chosen so
-// that |x| < 2^(e-1) and digits d_s = round-to-nearest in [-64, 64].  A product
-// of two split matrices keeps the digit pairs with s + t <= S + 1; pairs of
-// equal s + t share one int32 accumulator, which sums exactly while
-// pairs * 64 * 64 * k_chunk < 2^31.  The FP64 result is the weighted sum of the
-// accumulators times the two row scales.
+// A row x of an FP64 matrix is written as x = 2^e * sum_{s=1..S} d_s R^-s
+// (+ a remainder of at most 0.5 * R^-S * 2^e), radix R = kOzRadix = 254, with
+// the row exponent e chosen so that |x| < 2^(e-1) and digits d_s =
+// round-to-nearest in [-127, 127] (R = 254 is the largest radix whose rounded
+// digits of remainders in [-1/2, 1/2] always fit int8).  A product of two split
+// matrices keeps the digit pairs with s + t <= S + 1; pairs of equal s + t share
+// one int32 accumulator, which sums exactly while pairs * 127^2 * k_chunk < 2^31.
+// The FP64 result is the weighted sum of the accumulators (weights R^-(s+t))
+// times the two row scales.
 constexpr int kOzMaxSlices = 6;
+constexpr double kOzRadix = 254.0;
+constexpr int kOzDigitMax = 127;
+// R^-(q + 2) for digit group q = s + t - 2 (1-based digits), q < kOzMaxSlices;
+// 254^k is exact in FP64 for k <= 7, so each weight is one correctly rounded
+// division.
+__host__ __device__ constexpr double oz_group_weight(int q) {
+  double p = kOzRadix * kOzRadix;
+  for (int k = 0; k < q; ++k) p *= kOzRadix;
+  return 1.0 / p;
+}
 
 struct SplitArgs {
   // Element (r, k) of the R x K source, R = planes * Rb, is

@@ -17,10 +17,12 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 // Row split in three launches:
 //   1. partial maxima of |x| over k tiles:  pmax[kt][r]
 //   2. row scales: e = ilogb(max_k |x_rk|) + 2, scale[r] = 2^e
-//   3. digits: y = x 2^-e (|y| < 1/2), d_s = rint(128 y), y <- 128 y - d_s
-// so every digit lies in [-64, 64] and every remainder in [-1/2, 1/2].  All
-// steps are exact in FP64 (power-of-two scaling, rint, subtraction of the
-// integer part).  A row with a non-finite entry gets a NaN scale and zero
+//   3. digits: y = x 2^-e (|y| < 1/2), d_s = rint(254 y), y <- 254 y - d_s
+// so every digit lies in [-127, 127] and every remainder in [-1/2, 1/2].  The
+// scaling by 2^-e, rint and the subtraction of the integer part are exact in
+// FP64; the product 254 y rounds at 2^-53 of |254 y| <= 127, which moves the
+// represented value by about 2^-53 of the row scale, far below the truncation
+// 0.5 * 254^-S.  A row with a non-finite entry gets a NaN scale and zero
 // digits, so its products come out NaN.
 // Tiles: along k (rows orientation) 16 rows x 256 k, a thread owning 4
 // consecutive k of 4 rows; along r (columns orientation) 64 rows x 64 k, a
@@ -47,7 +49,7 @@ __device__ __forceinline__ uint32_t next_digit4(double v[4]) {
   uint32_t packed = 0;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    const double y = v[j] * 128.0;
+    const double y = v[j] * kOzRadix;
     const double d = rint(y);
     v[j] = y - d;
     packed |= (uint32_t)(uint8_t)(int8_t)(int)d << (8 * j);
@@ -462,7 +464,7 @@ __global__ __launch_bounds__(256) void ozgemm_kernel(OzGemmArgs g) {
     __syncthreads();
     buf ^= 1;
   }
-  // digit group q (pairs with s + t = q + 2, 1-based digits) weighs 128^-(q+2)
+  // digit group q (pairs with s + t = q + 2, 1-based digits) weighs 254^-(q+2)
   if (g.epi < 0) {
 #pragma unroll
     for (int j = 0; j < JB; ++j) {
@@ -477,7 +479,7 @@ __global__ __launch_bounds__(256) void ozgemm_kernel(OzGemmArgs g) {
           if (row >= g.M) continue;
           double v = 0.0;
 #pragma unroll
-          for (int q = S - 1; q >= 0; --q) v += ldexp((double)acc[q][i][j][r], -7 * (q + 2));
+          for (int q = S - 1; q >= 0; --q) v += (double)acc[q][i][j][r] * oz_group_weight(q);
           Cout[(int64_t)row * g.ldc + col] = v * (sA[row - m0] * sbc);
         }
     }
@@ -500,7 +502,7 @@ __global__ __launch_bounds__(256) void ozgemm_kernel(OzGemmArgs g) {
       for (int r = 0; r < 4; ++r) {
         double v = 0.0;
 #pragma unroll
-        for (int q = S - 1; q >= 0; --q) v += ldexp((double)acc[q][i][j][r], -7 * (q + 2));
+        for (int q = S - 1; q >= 0; --q) v += (double)acc[q][i][j][r] * oz_group_weight(q);
         z[r] = (colok && row0 + r < g.M) ? v * (sA[row0 + r - m0] * sbc) : 0.0;
       }
       if (g.epi == 0) {
@@ -648,7 +650,7 @@ __global__ __launch_bounds__(512, 1) void ozgemm8_kernel(OzGemmArgs g) {
       setup(tnext);
       load_global(kb);
     }
-    // digit group q (pairs with s + t = q + 2, 1-based digits) weighs 128^-(q+2)
+    // digit group q (pairs with s + t = q + 2, 1-based digits) weighs 254^-(q+2)
     const int col = en0 + wn + lr;
     const bool colok = col < g.N;
     const double sbc = sB[wn + lr];
@@ -663,7 +665,7 @@ __global__ __launch_bounds__(512, 1) void ozgemm8_kernel(OzGemmArgs g) {
         for (int r = 0; r < 4; ++r) {
           double v = 0.0;
 #pragma unroll
-          for (int q = S - 1; q >= 0; --q) v += ldexp((double)acc[q][i][4 * gq + r], -7 * (q + 2));
+          for (int q = S - 1; q >= 0; --q) v += (double)acc[q][i][4 * gq + r] * oz_group_weight(q);
           z[r] = (colok && row0 + r < g.M) ? v * (sA[row0 + r - em0] * sbc) : 0.0;
         }
         if (g.epi < 0) {
@@ -710,9 +712,9 @@ __global__ __launch_bounds__(512, 1) void ozgemm8_kernel(OzGemmArgs g) {
 }
 
 int64_t ozgemm_max_k(int S) {
-  // S pairs share the largest group; |d| <= 64 for every digit
+  // S pairs share the largest group; |d| <= 127 for every digit
   const int64_t lim = ((int64_t)1 << 31) - 1;
-  return lim / (4096LL * S) / OZ_BK * OZ_BK;
+  return lim / ((int64_t)kOzDigitMax * kOzDigitMax * S) / OZ_BK * OZ_BK;
 }
 
 // Bands of tile rows for the 4-digit kernels' tile order: bands of 4 on grids

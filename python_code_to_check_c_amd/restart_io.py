@@ -8,6 +8,9 @@ Writers follow CCSD_Helper.py:1102-1145, readers CC_Calculator.py:70-135:
                         value = str(numpy.around(x, precs)), csv module, '\\n' endings
   H2O.csv               ``,time,mu_real,mu_imag`` (appended without header on restart)
   timing.csv            ``,total,t1,t2,l1,l2``
+  restart_state.bin     exact binary snapshot (state after the last step, its
+                        time, F, field parameters) for a bit-for-bit resume;
+                        written beside the text files, read only on request
 """
 from __future__ import annotations
 
@@ -62,21 +65,40 @@ def save_data(F, t1, t2, lam1, lam2, data, timing, precs, restart, path="."):
     write_4data(lam2.imag, j("lam2_imag.dat"), precs)
 
 
+STATE_FILE = "restart_state.bin"
+
+
 def convert_2data(FileName, i, a):
-    """CC_Calculator.py:85-96: rows (i, a, value) -> dense (i, a)."""
-    df = pd.read_csv(FileName, sep="\t", header=None, names=["i", "a", "F"])
-    out = np.zeros((i, a))
-    out[np.around(df["i"].to_numpy()).astype(int), np.around(df["a"].to_numpy()).astype(int)] = df["F"].to_numpy()
-    return out
+    """CC_Calculator.py:85-96: rows (i, a, value) -> dense (i, a), by the
+    library's C++ reader."""
+    return _lib.read_restart_block(FileName, (i, a))
 
 
 def convert_4data(FileName, i, a):
-    """CC_Calculator.py:98-113: rows (i, j, a, b, value) -> dense (i, i, a, a)."""
-    df = pd.read_csv(FileName, sep="\t", header=None, names=["i", "j", "a", "b", "F"])
-    out = np.zeros((i, i, a, a))
-    idx = [np.around(df[k].to_numpy()).astype(int) for k in ("i", "j", "a", "b")]
-    out[idx[0], idx[1], idx[2], idx[3]] = df["F"].to_numpy()
-    return out
+    """CC_Calculator.py:98-113: rows (i, j, a, b, value) -> dense (i, i, a, a),
+    by the library's C++ reader."""
+    return _lib.read_restart_block(FileName, (i, i, a, a))
+
+
+def save_state(F, t1, t2, lam1, lam2, t, dt, w0, A, precs, method, path="."):
+    """Exact snapshot beside the text files: the state after the stopping step
+    and its time t (not the rounded base and t - dt the text files hold)."""
+    _lib.write_state_file(os.path.join(path, STATE_FILE), method, precs, t, dt, w0, A, F, t1, t2, lam1, lam2)
+
+
+def read_state(path="."):
+    """The exact snapshot as a dict (method, precs, t, dt, w0, A, F, t1, t2, lam1, lam2)."""
+    return _lib.read_state_file(os.path.join(path, STATE_FILE))
+
+
+def read_parameters(path="."):
+    """Parameters.csv as TDCC_restart reads it (CC_Calculator.py:71-83):
+    w0, A, t0 (advanced by dt), tf = 50 + the saved t0, dt, precs, method, o, v."""
+    p = pd.read_csv(os.path.join(path, "Parameters.csv"))
+    t0 = p["t0"][0]
+    dt = p["dt"][0]
+    return dict(w0=p["w0"][0], A=p["A"][0], t0=t0 + dt, tf=50.0 + t0, dt=dt, precs=int(p["precs"][0]),
+                method=p["CCSD_or_CC2"][0], o=int(p["i"][0]), v=int(p["a"][0]))
 
 
 def read_restart(path="."):
@@ -87,17 +109,9 @@ def read_restart(path="."):
     lam1, lam2.
     """
     j = lambda name: os.path.join(path, name)
-    p = pd.read_csv(j("Parameters.csv"))
-    w0 = p["w0"][0]
-    A = p["A"][0]
-    t0 = p["t0"][0]
-    method = p["CCSD_or_CC2"][0]
-    tf = 50.0 + t0
-    dt = p["dt"][0]
-    t0 = t0 + dt
-    precs = int(p["precs"][0])
-    i = int(p["i"][0])
-    a = int(p["a"][0])
+    prm = read_parameters(path)
+    w0, A, t0, tf, dt, precs, method = (prm[k] for k in ("w0", "A", "t0", "tf", "dt", "precs", "method"))
+    i, a = prm["o"], prm["v"]
     t1 = convert_2data(j("t1_real.dat"), i, a) + 1.0 * 1j * convert_2data(j("t1_imag.dat"), i, a)
     lam1 = convert_2data(j("lam1_real.dat"), i, a) + 1.0 * 1j * convert_2data(j("lam1_imag.dat"), i, a)
     t2 = convert_4data(j("t2_real.dat"), i, a) + 1.0 * 1j * convert_4data(j("t2_imag.dat"), i, a)

@@ -112,3 +112,38 @@ def test_divergence_stop_saves_rounded_base(g, tmp_path, monkeypatch):
     assert abs(r["t0"] - DT - tr.saved_time) <= 1e-10  # the reader adds dt back (CC_Calculator.py:80)
     for got, ref in zip((r["t1"], r["t2"], r["lam1"], r["lam2"]), tr.saved):
         assert np.abs(got - ref).max() <= 1e-10 * max(1.0, np.abs(ref).max())
+
+
+def test_exact_resume_is_bitwise_continuous(g, tmp_path, monkeypatch):
+    """A run stopped by the wall-clock rule also writes the exact snapshot
+    (restart_state.bin: the state after the stopping step and its time);
+    resuming from it continues the interrupted run bit for bit, through
+    Runge_Kutta_solver and through CC_Calculator.TDCC_restart(exact=True)."""
+    monkeypatch.chdir(tmp_path)
+    print("\n[dropin exact resume CCSD]", flush=True)
+    tf = 0.0205
+    full = CCSD_Helper(provider(g))
+    full.verbose = False
+    full.Runge_Kutta_solver(g["F"], *amplitudes(g), W0, A, 0.0, tf, DT, 1000.0, 15)
+    full_mu, full_state = full.mu_history.copy(), full.get_state()
+    assert full.stop_reason is None and len(full_mu) == 21
+    calc = CC_Calculator(provider(g), w0=W0, A=A, t0=0.0, dt=DT, precs=15)
+    first = calc.mol
+    first.verbose = False
+    first.Runge_Kutta_solver(g["F"], *amplitudes(g), W0, A, 0.0, tf, DT, 0.0, 15)
+    assert first.stop_reason == "timeout"
+    n1 = len(first.data)
+    r = restart_io.read_state()
+    assert r["t"] == first.final_time and r["method"] == "CCSD"
+    rest = CCSD_Helper(provider(g))
+    rest.verbose = False
+    rest.Runge_Kutta_solver(r["F"], r["t1"], r["t2"], r["lam1"], r["lam2"], r["w0"], r["A"], r["t"], tf, r["dt"],
+                            1000.0, r["precs"], "restart")
+    mu = np.concatenate([first.mu_history, rest.mu_history])
+    assert len(mu) == len(full_mu) and np.array_equal(mu, full_mu)
+    for x, y in zip(rest.get_state(), full_state):
+        assert np.array_equal(x, y)
+    # the calculator's exact resume (timeout 0: one more step) gives step n1 + 1 of the full run
+    calc.TDCC_restart(0.0, exact=True)
+    assert calc.last.stop_reason == "timeout"
+    assert np.array_equal(calc.last.mu_history[0], full_mu[n1])
