@@ -79,6 +79,10 @@ struct tdcc_ctx {
   // rounded base, per-slot field tables and dipole rows ([slot][cap][3][2])
   int nb = 0;
   std::vector<std::array<Planes, 4>> bstate, bbase;
+  // one allocation per amplitude block for all slots (state, then base):
+  // slot b of block k at bblk[.][k] + b * 2 * block_size(k), so that the
+  // stacked program reaches slot b at a fixed element stride
+  double* bblk[2][4] = {{nullptr, nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr, nullptr}};
   std::vector<char> slot_pairs;
   double* btable = nullptr;
   double* bmu = nullptr;
@@ -420,31 +424,58 @@ Program& step_program(tdcc_ctx* c, int method, double dt) {
   return finish_program(c, key, std::move(g), true);
 }
 
-// One program advancing every batch slot by one step: the slots' step
-// graphs side by side (integral-only nodes shared), one device counter.
+// One program advancing every batch slot by one step.  Stacked (default): the
+// step graph of one trajectory with a trajectory axis over the slots
+// (Graph::set_stack), so that a contraction of a shared integral with the
+// slots' amplitudes is one launch over all of them and a linear combination
+// covers every slot in one launch.  TDCC_STACK=0 puts the slots' step graphs
+// side by side instead (integral-only nodes shared), one launch per slot.
+bool stack_enabled() {
+  const char* e = std::getenv("TDCC_STACK");
+  return !(e && e[0] == '0');
+}
+
 Program& batch_step_program(tdcc_ctx* c, int method, double dt) {
   bool pairs = true;
   for (char f : c->slot_pairs) pairs = pairs && f;
-  const std::string key =
-      "bstep:" + std::to_string(method) + ":" + hexd(dt) + ":" + std::to_string(c->nb) + pairs_suffix(c, pairs);
+  const bool stacked = stack_enabled();
+  const std::string key = std::string(stacked ? "zstep:" : "bstep:") + std::to_string(method) + ":" + hexd(dt) + ":" +
+                          std::to_string(c->nb) + pairs_suffix(c, pairs);
   auto it = c->progs.find(key);
   if (it != c->progs.end()) return it->second;
   auto g = std::make_unique<Graph>(c->o, c->v, &c->rt);
   g->set_ij_pairs(pairs);
   g->set_k_pairs(pairs && c->tei_antisym);
   const int tei = ext_tei(c, *g);
-  for (int b = 0; b < c->nb; ++b) {
-    g->set_traj(b);
+  if (stacked) {
+    g->set_stack(c->nb);
     StepIO io;
     io.tei = tei;
+    auto ext = [&](const Planes& p, int k, const char* name) {
+      const int id = ext_planes(*g, p, shape_of_block(k), name);
+      g->set_zstride(id, 2 * block_size(c, k));
+      return id;
+    };
     for (int k = 0; k < 4; ++k) {
-      io.y[k] = mark_amplitude(*g, ext_planes(*g, c->bstate[b][k], shape_of_block(k), "y"), k, pairs);
-      io.base[k] = mark_amplitude(*g, ext_planes(*g, c->bbase[b][k], shape_of_block(k), "base"), k, pairs);
+      io.y[k] = mark_amplitude(*g, ext(c->bstate[0][k], k, "y"), k, pairs);
+      io.base[k] = mark_amplitude(*g, ext(c->bbase[0][k], k, "base"), k, pairs);
     }
-    for (int k = 0; k < 4; ++k) io.ynew[k] = ext_planes(*g, c->bstate[b][k], shape_of_block(k), "ynew");
+    for (int k = 0; k < 4; ++k) io.ynew[k] = ext(c->bstate[0][k], k, "ynew");
     build_rk4_step(*g, io, method, dt);
+  } else {
+    for (int b = 0; b < c->nb; ++b) {
+      g->set_traj(b);
+      StepIO io;
+      io.tei = tei;
+      for (int k = 0; k < 4; ++k) {
+        io.y[k] = mark_amplitude(*g, ext_planes(*g, c->bstate[b][k], shape_of_block(k), "y"), k, pairs);
+        io.base[k] = mark_amplitude(*g, ext_planes(*g, c->bbase[b][k], shape_of_block(k), "base"), k, pairs);
+      }
+      for (int k = 0; k < 4; ++k) io.ynew[k] = ext_planes(*g, c->bstate[b][k], shape_of_block(k), "ynew");
+      build_rk4_step(*g, io, method, dt);
+    }
+    g->set_traj(-1);
   }
-  g->set_traj(-1);
   Program& p = finish_program(c, key, std::move(g), true);
   p.batched = true;
   return p;
@@ -680,10 +711,11 @@ void check_run_args(int method, int nsteps, int pol, int precs, double dt) {
 }
 
 void free_batch(tdcc_ctx* c) {
-  for (auto& s : c->bstate)
-    for (auto& p : s) free_planes(p);
-  for (auto& s : c->bbase)
-    for (auto& p : s) free_planes(p);
+  for (auto& blk : c->bblk)
+    for (double*& p : blk) {
+      if (p) (void)hipFree(p);
+      p = nullptr;
+    }
   c->bstate.clear();
   c->bbase.clear();
   if (c->btable) (void)hipFree(c->btable);
@@ -936,10 +968,16 @@ int tdcc_set_batch(tdcc_ctx* c, int nb) {
     free_batch(c);
     c->bstate.resize(nb);
     c->bbase.resize(nb);
-    for (int b = 0; b < nb; ++b)
+    for (int w = 0; w < 2 && nb > 0; ++w)
       for (int k = 0; k < 4; ++k) {
-        alloc_planes(c->bstate[b][k], block_size(c, k));
-        alloc_planes(c->bbase[b][k], block_size(c, k));
+        const int64_t slot = 2 * block_size(c, k);
+        TDCC_HIP(hipMalloc(&c->bblk[w][k], (size_t)nb * slot * sizeof(double)));
+        TDCC_HIP(hipMemset(c->bblk[w][k], 0, (size_t)nb * slot * sizeof(double)));
+        for (int b = 0; b < nb; ++b) {
+          Planes& p = (w == 0 ? c->bstate : c->bbase)[b][k];
+          p.n = block_size(c, k);
+          p.d = c->bblk[w][k] + (int64_t)b * slot;
+        }
       }
     c->nb = nb;
     c->slot_pairs.assign(nb, antisym_ij(nullptr, c->o, c->v));  // zero amplitudes

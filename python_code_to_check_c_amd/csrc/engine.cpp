@@ -4,6 +4,7 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <set>
@@ -44,7 +45,9 @@ struct LadderArgs {
   bool ij_pairs = false;  // rows are the pairs i < j
   int splits = 1;         // K chunks of the GEMM (partial results summed before the unpack)
   int64_t k_chunk = 0;
-  int64_t rows = 0;
+  int64_t rows = 0;        // rows of one trajectory
+  int nz = 1;              // stacked trajectories: rows of trajectory z at z * 2 rows of up / rp
+  int64_t u_zs = 0, out_zs = 0;
   int v = 0;
   int64_t P = 0;
   // split precision (S > 0): digits of up, and the digit GEMM against vpack8
@@ -690,46 +693,47 @@ static void inputs_of(const Node& n, std::vector<int>& in) {
   }
 }
 
-// Ladder scratch: packed input up (2 rows x P doubles; rows = o^2, or
-// o (o - 1) / 2 with ij pairs) and the result rp (one 2 rows x P partial per
-// K chunk of the digit GEMM), then, in split precision, the digit planes of
-// up, its row scales and the split workspace.
+// Ladder scratch: packed input up (R2 = 2 rows x P doubles; rows = o^2, or
+// o (o - 1) / 2 with ij pairs, times the trajectories of a stacked program)
+// and the result rp (one R2 x P partial per K chunk of the digit GEMM), then,
+// in split precision, the digit planes of up, its row scales and the split
+// workspace.
 static int64_t al256(int64_t x) { return (x + 255) / 256 * 256; }
 static int64_t ladder_rows(int o, bool pr) { return pr ? (int64_t)o * (o - 1) / 2 : (int64_t)o * o; }
 static int64_t ladder_rows2(int o, bool pr) { return 2 * ladder_rows(o, pr); }
 static int64_t ladder_P(int v) { return (int64_t)v * (v - 1) / 2; }
 static int64_t ladder_Kp(int v) { return (ladder_P(v) + 63) / 64 * 64; }
-static int64_t ladder_plane(int o, bool pr, int v) { return al256(ladder_rows2(o, pr) * ladder_Kp(v)); }
+static int64_t ladder_plane(int64_t R2, int v) { return al256(R2 * ladder_Kp(v)); }
 // K chunk of the ladder GEMM (0: one chunk): a grid of fewer than two rounds
 // of 128 x 128 tiles over the 256 CUs is cut along K so the last round is
 // not mostly idle.  Digit GEMM: multiples of 64 within the exact int32
 // range; FP64 GEMM: multiples of 16 of K = P.
-static int64_t ladder_k_chunk(int o, bool pr, int v, int S) {
+static int64_t ladder_k_chunk(int64_t R2, int v, int S) {
   const int64_t gran = S > 0 ? 64 : 16;
   const int64_t K = S > 0 ? ladder_Kp(v) : ladder_P(v), steps = K / gran;
-  const int64_t tiles = ((ladder_rows2(o, pr) + 127) / 128) * ((ladder_P(v) + 127) / 128);
+  const int64_t tiles = ((R2 + 127) / 128) * ((ladder_P(v) + 127) / 128);
   int64_t sp = tiles >= 512 ? 1 : std::min<int64_t>(8, (1280 + tiles - 1) / tiles);
   sp = std::max<int64_t>(1, std::min<int64_t>(sp, steps / 2));
   int64_t chunk = (steps + sp - 1) / sp * gran;
   if (S > 0) chunk = std::min<int64_t>(chunk, ozgemm_max_k(S));
   return chunk >= K ? 0 : chunk;
 }
-static int64_t ladder_splits(int o, bool pr, int v, int S) {
-  const int64_t c = ladder_k_chunk(o, pr, v, S);
+static int64_t ladder_splits(int64_t R2, int v, int S) {
+  const int64_t c = ladder_k_chunk(R2, v, S);
   return c > 0 ? ((S > 0 ? ladder_Kp(v) : ladder_P(v)) + c - 1) / c : 1;
 }
-static int64_t ladder_digits_off(int o, bool pr, int v, int S) {
-  return al256((1 + ladder_splits(o, pr, v, S)) * ladder_rows2(o, pr) * ladder_P(v) * (int64_t)sizeof(double));
+static int64_t ladder_digits_off(int64_t R2, int v, int S) {
+  return al256((1 + ladder_splits(R2, v, S)) * R2 * ladder_P(v) * (int64_t)sizeof(double));
 }
-static int64_t ladder_scale_off(int o, bool pr, int v, int S) {
-  return ladder_digits_off(o, pr, v, S) + S * ladder_plane(o, pr, v);
+static int64_t ladder_scale_off(int64_t R2, int v, int S) {
+  return ladder_digits_off(R2, v, S) + S * ladder_plane(R2, v);
 }
-static int64_t ladder_pmax_off(int o, bool pr, int v, int S) {
-  return ladder_scale_off(o, pr, v, S) + al256(ladder_rows2(o, pr) * (int64_t)sizeof(double));
+static int64_t ladder_pmax_off(int64_t R2, int v, int S) {
+  return ladder_scale_off(R2, v, S) + al256(R2 * (int64_t)sizeof(double));
 }
-static int64_t ladder_scratch_bytes(int o, bool pr, int v, int S) {
-  if (S <= 0) return ladder_digits_off(o, pr, v, S);
-  return ladder_pmax_off(o, pr, v, S) + al256(ozaki_split_workspace(ladder_rows2(o, pr), ladder_Kp(v), false));
+static int64_t ladder_scratch_bytes(int64_t R2, int v, int S) {
+  if (S <= 0) return ladder_digits_off(R2, v, S);
+  return ladder_pmax_off(R2, v, S) + al256(ozaki_split_workspace(R2, ladder_Kp(v), false));
 }
 
 bool Graph::is_static(int id) const {
@@ -757,6 +761,7 @@ void Graph::plan() {
   fold_outer_products(live);
   inline_lincombs(live);
   live_ = live;
+  mark_stack();
   // liveness: uses of views count as uses of their base
   auto root = [&](int x) {
     while (nodes_[x].op == Op::View) x = nodes_[x].base;
@@ -815,7 +820,8 @@ void Graph::plan() {
   for (int i = 0; i < N; ++i) {
     if (!live[i]) continue;
     Node& n = nodes_[i];
-    if (n.op == Op::Ladder) n.scratch_len = ladder_scratch_bytes(o_, n.ij_pairs, v_, rt_->split_digits);
+    if (n.op == Op::Ladder)
+      n.scratch_len = ladder_scratch_bytes(ladder_rows2(o_, n.ij_pairs) * (n.zdep ? nz_ : 1), v_, rt_->split_digits);
     if (n.op == Op::Contract) lower_contract(i);
   }
   plan_shares();
@@ -827,14 +833,14 @@ void Graph::plan() {
   for (int i = 0; i < N; ++i) {
     if (!live[i]) continue;
     Node& n = nodes_[i];
-    if (owns_buffer(n.op)) n.arena_off = alloc(size(n.shape) * (n.cplx ? 2 : 1) * (int64_t)sizeof(double));
+    if (owns_buffer(n.op)) n.arena_off = alloc(zbytes(n) * (n.zdep ? nz_ : 1));
     for (int k : share_at[i]) shares_[k].off = alloc(shares_[k].bytes);
     if ((n.op == Op::Contract || n.op == Op::Ladder) && n.scratch_len > 0) {
       n.scratch_off = alloc(n.scratch_len);
       release(n.scratch_off, n.scratch_len);
     }
     for (int k : share_end[i]) release(shares_[k].off, shares_[k].bytes);
-    for (int x : ends[i]) release(nodes_[x].arena_off, size(nodes_[x].shape) * (nodes_[x].cplx ? 2 : 1) * (int64_t)sizeof(double));
+    for (int x : ends[i]) release(nodes_[x].arena_off, zbytes(nodes_[x]) * (nodes_[x].zdep ? nz_ : 1));
   }
   arena_bytes_ = top;
   if (const char* d = std::getenv("TDCC_DUMP_NODES"); d && d[0] == '1') dump_nodes();
@@ -1096,13 +1102,20 @@ void Graph::dump_nodes() const {
   }
 }
 
-Dev Graph::resolve(int id) const {
+Dev Graph::resolve(int id, int z) const {
   const Node& n = nodes_[id];
   Dev d;
+  const int64_t zo = n.zdep ? (int64_t)z * n.zs : 0;
   switch (n.op) {
-    case Op::External: return n.ext;
+    case Op::External:
+      d = n.ext;
+      if (zo) {
+        d.re += zo;
+        if (d.im) d.im += zo;
+      }
+      return d;
     case Op::View: {
-      Dev b = resolve(n.base);
+      Dev b = resolve(n.base, z);
       d.re = b.re + n.voff;
       d.im = b.im ? b.im + n.voff : nullptr;
       for (int k = 0; k < 4; ++k) d.st[k] = n.vst[k];
@@ -1110,11 +1123,47 @@ Dev Graph::resolve(int id) const {
     }
     default: {
       if (n.arena_off < 0) throw Error("node has no buffer");
-      d.re = arena_ + n.arena_off / (int64_t)sizeof(double);
+      d.re = arena_ + n.arena_off / (int64_t)sizeof(double) + zo;
       d.im = n.cplx ? d.re + size(n.shape) : nullptr;
       contiguous_strides(*this, n.shape, d.st);
       return d;
     }
+  }
+}
+
+int64_t Graph::zbytes(const Node& n) const {
+  return (size(n.shape) * (n.cplx ? 2 : 1) * (int64_t)sizeof(double) + 255) & ~int64_t(255);
+}
+
+// Which live nodes carry a trajectory axis: per-trajectory externals (zs set
+// by the caller), Field nodes (one field per trajectory) and everything
+// computed from them; their z-stride is the external's, the view base's, or
+// one trajectory's (256-byte aligned) buffer.
+void Graph::mark_stack() {
+  std::vector<int> in;
+  for (int i = 0; i < (int)nodes_.size(); ++i) {
+    Node& n = nodes_[i];
+    if (nz_ <= 1) {
+      n.zdep = false;
+      if (n.op != Op::External) n.zs = 0;
+      continue;
+    }
+    switch (n.op) {
+      case Op::External: n.zdep = n.zs > 0; break;
+      case Op::View:
+        n.zdep = nodes_[n.base].zdep;
+        n.zs = nodes_[n.base].zs;
+        break;
+      case Op::Field: n.zdep = true; break;
+      default: {
+        inputs_of(n, in);
+        n.zdep = false;
+        for (int x : in) n.zdep = n.zdep || nodes_[x].zdep;
+      }
+    }
+    if (n.op != Op::External && n.op != Op::View) n.zs = n.zdep ? zbytes(n) / (int64_t)sizeof(double) : 0;
+    if (n.zdep && (n.op == Op::FieldVjp || n.op == Op::Seed || n.op == Op::TraceVjp))
+      throw Error("adjoints of stacked trajectories are not supported");
   }
 }
 
@@ -1192,6 +1241,26 @@ void Graph::lower_contract(int id) {
       std::swap(la, lb);
       std::swap(a, b);
       std::swap(sa, sb);
+    }
+  }
+  // Stacked trajectories: the trajectory axis is the letter 'Z' of extent nz
+  // with each operand's z-stride.  With one per-trajectory operand it joins
+  // that operand's free letters, so the trajectories stack into its rows (or
+  // columns) of one launch against the shared operand; with two it is a batch
+  // letter (GETT batch) unless the one-trajectory contraction takes a
+  // dense-GEMM, dot or skinny route, which then runs once per trajectory.
+  const bool za = nodes_[a].zdep, zb = nodes_[b].zdep;
+  n.zloop = false;
+  bool zbatch = false;
+  if (nz_ > 1 && (za || zb)) {
+    if (za && zb) {
+      zbatch = true;  // decided below, once the one-trajectory route is known
+    } else {
+      (za ? la : lb).push_back('Z');
+      lc.push_back('Z');
+      ext['Z'] = nz_;
+      (za ? sa : sb)['Z'] = nodes_[a].zdep ? nodes_[a].zs : nodes_[b].zs;
+      sc['Z'] = n.zs;
     }
   }
   std::string gb, gm, gn, gk;
@@ -1286,6 +1355,22 @@ void Graph::lower_contract(int id) {
   const bool use_dot = use_gemm && dot_k >= 0 && !(akc && bkc && ia != ib) && b_rows <= kDotMaxRows &&
                        (a_rows <= 64 || (in_fp64 && a_rows <= kDotMaxRows)) &&
                        (double)gsize(gk) * kpair_frac >= (double)dot_k;
+  if (zbatch) {
+    const bool would_skinny = !use_gemm && std::min(gsize(gm), gsize(gn)) <= kSkinnyMax &&
+                              (double)std::max(gsize(gm), gsize(gn)) * gsize(gk) >= skinny_min_work();
+    if (use_gemm || use_dot || would_skinny) {
+      n.zloop = true;
+    } else {
+      la.push_back('Z');
+      lb.push_back('Z');
+      lc.push_back('Z');
+      ext['Z'] = nz_;
+      sa['Z'] = nodes_[a].zs;
+      sb['Z'] = nodes_[b].zs;
+      sc['Z'] = n.zs;
+      groups();
+    }
+  }
   auto by_stride = [&](std::string g, const std::map<char, int64_t>& st) {
     std::stable_sort(g.begin(), g.end(), [&](char x, char y) { return st.at(x) > st.at(y); });
     return g;
@@ -1303,7 +1388,8 @@ void Graph::lower_contract(int id) {
     auto find_pair = [&](const std::string& grp, std::string& pr) {
       for (size_t x = 0; x < grp.size() && pr.empty(); ++x)
         for (size_t y = x + 1; y < grp.size() && pr.empty(); ++y)
-          if (ext[grp[x]] == ext[grp[y]] && (n.anti & anti_bit((int)lc.find(grp[x]), (int)lc.find(grp[y]))))
+          if (grp[x] != 'Z' && grp[y] != 'Z' && ext[grp[x]] == ext[grp[y]] &&
+              (n.anti & anti_bit((int)lc.find(grp[x]), (int)lc.find(grp[y]))))
             pr = std::string{grp[x], grp[y]};
     };
     find_pair(gm, mpair);
@@ -1460,7 +1546,7 @@ void Graph::lower_contract(int id) {
       gp.Kp = (K + 63) / 64 * 64;
       gp.a_plane = al256(R * gp.Kp);
       gp.b_plane = al256(NB * gp.Kp);
-      const int64_t bn = S <= 4 ? 128 : 64;
+      const int64_t bn = S <= 5 ? 128 : 64;
       const int64_t tiles = ((R + 127) / 128) * ((NB + bn - 1) / bn);
       int64_t sp = 1;
       if (tiles < 240 && K >= 2048) sp = std::min<int64_t>((K + 1023) / 1024, (480 + tiles - 1) / tiles);
@@ -1507,8 +1593,9 @@ void Graph::lower_contract(int id) {
       const std::string bkey = layout_key(b, lb, gn, npair, gp.b_cols, NB);
       gp.a_cache = cache_entry(a, akey, R, gp.a_plane);
       gp.b_cache = cache_entry(b, bkey, NB, gp.b_plane);
-      if (gp.a_cache < 0) dyn_uses_[akey].push_back({id, 0, a, S * gp.a_plane + al256(8 * R)});
-      if (gp.b_cache < 0 && bkey != akey) dyn_uses_[bkey].push_back({id, 1, b, S * gp.b_plane + al256(8 * NB)});
+      if (gp.a_cache < 0 && !n.zloop) dyn_uses_[akey].push_back({id, 0, a, S * gp.a_plane + al256(8 * R)});
+      if (gp.b_cache < 0 && bkey != akey && !n.zloop)
+        dyn_uses_[bkey].push_back({id, 1, b, S * gp.b_plane + al256(8 * NB)});
       L->node = id;
       launches_.push_back(std::move(L));
       return;
@@ -1551,10 +1638,13 @@ void Graph::bind(double* arena) {
     const Node& n = nodes_[i];
     switch (n.op) {
       case Op::Contract: {
-        auto L = std::move(gett.at(gi++));
-        if (L->node != i) throw Error("internal: contraction launch order");
+        auto L0 = std::move(gett.at(gi++));
+        if (L0->node != i) throw Error("internal: contraction launch order");
+        const int zc = n.zloop ? nz_ : 1;  // once per trajectory, or once over the stacked rows
+        for (int z = 0; z < zc; ++z) {
+        auto L = z + 1 == zc ? std::move(L0) : std::make_unique<Launch>(*L0);
         GettArgs& g = L->g;
-        const Dev A = resolve(n.a), B = resolve(n.b), C = resolve(i);
+        const Dev A = resolve(n.a, z), B = resolve(n.b, z), C = resolve(i, z);
         g.a_re = A.re;
         g.a_im = L->ac ? A.im : nullptr;
         g.b_re = B.re;
@@ -1800,6 +1890,7 @@ void Graph::bind(double* arena) {
           sc.accumulate = 0;
         }
         launches_.push_back(std::move(L));
+        }
         break;
       }
       case Op::Lincomb:
@@ -1822,6 +1913,9 @@ void Graph::bind(double* arena) {
           }
           l.accumulate = t0 > 0;
           l.nterms = (int)std::min<size_t>(kMaxTerms, terms.size() - t0);
+          const Node& on = n.op == Op::Assign ? nodes_[n.b] : n;
+          l.nz = on.zdep ? nz_ : 1;
+          l.ozs = on.zdep ? on.zs : 0;
           // an antisymmetric result is summed at its canonical elements and
           // mirrored (one launch only)
           l.anti = 0;
@@ -1835,6 +1929,8 @@ void Graph::bind(double* arena) {
             for (int d = 0; d < 4; ++d) l.t[k].st[d] = d < n.shape.rank && T.perm[d] >= 0 ? S.st[T.perm[d]] : 0;
             l.t[k].cr = T.cr;
             l.t[k].ci = T.ci;
+            l.t[k].zs = nodes_[T.src].zdep ? nodes_[T.src].zs : 0;
+            l.t[k].zs2 = (T.src2 >= 0 && nodes_[T.src2].zdep) ? nodes_[T.src2].zs : 0;
             l.t[k].re2 = nullptr;
             l.t[k].im2 = nullptr;
             for (int d = 0; d < 4; ++d) l.t[k].st2[d] = 0;
@@ -1850,10 +1946,15 @@ void Graph::bind(double* arena) {
         break;
       }
       case Op::Ladder: {
+        // stacked trajectories: one GEMM over the packed rows of every
+        // trajectory (the packing and unpacking run once per trajectory)
+        const int zn = n.zdep ? nz_ : 1;
+        {
+        const int z = 0;
         auto L = std::make_unique<Launch>();
         L->kind = Launch::LADDER;
         L->node = i;
-        const Dev U = resolve(n.a), O = resolve(i);
+        const Dev U = resolve(n.a, z), O = resolve(i, z);
         if (!nodes_[n.a].cplx) throw Error("internal: real ladder input");
         LadderArgs& a = L->lad;
         a.u_re = U.re;
@@ -1863,32 +1964,36 @@ void Graph::bind(double* arena) {
         a.o = o_;
         a.ij_pairs = n.ij_pairs;
         a.rows = ladder_rows(o_, n.ij_pairs);
+        a.nz = zn;
+        a.u_zs = nodes_[n.a].zdep ? nodes_[n.a].zs : 0;
+        a.out_zs = n.zdep ? n.zs : 0;
+        const int64_t R2 = 2 * a.rows * zn;
         a.v = v_;
         a.P = (int64_t)v_ * (v_ - 1) / 2;
         a.up = arena_ + n.scratch_off / (int64_t)sizeof(double);
-        a.rp = a.up + 2 * a.rows * a.P;
+        a.rp = a.up + R2 * a.P;
         a.vp = rt_->vpack[n.stage];
-        a.k_chunk = ladder_k_chunk(o_, a.ij_pairs, v_, rt_->split_digits);
-        a.splits = (int)ladder_splits(o_, a.ij_pairs, v_, rt_->split_digits);
+        a.k_chunk = ladder_k_chunk(R2, v_, rt_->split_digits);
+        a.splits = (int)ladder_splits(R2, v_, rt_->split_digits);
         if (!a.vp) throw Error("packed <ab||ef> not built");
         a.S = rt_->split_digits;
         if (a.S > 0) {
           if (!rt_->vpack8[n.stage]) throw Error("digit planes of the packed <ab||ef> not built");
           char* scr = reinterpret_cast<char*>(arena_) + n.scratch_off;
-          const int64_t Kp = ladder_Kp(v_), plane = ladder_plane(o_, a.ij_pairs, v_);
+          const int64_t Kp = ladder_Kp(v_), plane = ladder_plane(R2, v_);
           if (Kp != rt_->vpack_Kp) throw Error("internal: ladder digit row length");
           SplitArgs& su = a.su;  // rows of up, dense
           su.src_re = a.up;
           su.ldx = a.P;
-          su.Rb = 2 * a.rows;
+          su.Rb = R2;
           su.planes = 1;
           su.K = a.P;
           su.Kp = Kp;
           su.S = a.S;
-          su.D = reinterpret_cast<int8_t*>(scr + ladder_digits_off(o_, a.ij_pairs, v_, a.S));
+          su.D = reinterpret_cast<int8_t*>(scr + ladder_digits_off(R2, v_, a.S));
           su.plane = plane;
-          su.scale = reinterpret_cast<double*>(scr + ladder_scale_off(o_, a.ij_pairs, v_, a.S));
-          su.pmax = reinterpret_cast<double*>(scr + ladder_pmax_off(o_, a.ij_pairs, v_, a.S));
+          su.scale = reinterpret_cast<double*>(scr + ladder_scale_off(R2, v_, a.S));
+          su.pmax = reinterpret_cast<double*>(scr + ladder_pmax_off(R2, v_, a.S));
           su.multipass = split_multipass();
           OzGemmArgs& oz = a.oz;
           oz.A = su.D;
@@ -1899,49 +2004,55 @@ void Graph::bind(double* arena) {
           oz.sb = rt_->vpack8_scale[n.stage];
           oz.C = a.rp;
           oz.ldc = a.P;
-          oz.M = (int)(2 * a.rows);
+          oz.M = (int)R2;
           oz.N = (int)a.P;
           oz.Kp = (int)Kp;
           oz.S = a.S;
-          const int64_t chunk = ladder_k_chunk(o_, a.ij_pairs, v_, a.S);
+          const int64_t chunk = ladder_k_chunk(R2, v_, a.S);
           if (chunk == 0 && Kp > ozgemm_max_k(a.S)) throw Error("ladder K beyond the exact int32 range of the digit GEMM");
           oz.k_chunk = (int)chunk;
-          oz.splits = (int)ladder_splits(o_, a.ij_pairs, v_, a.S);
-          oz.c_split_stride = 2 * a.rows * a.P;
+          oz.splits = (int)ladder_splits(R2, v_, a.S);
+          oz.c_split_stride = R2 * a.P;
           a.splits = oz.splits;
         }
         launches_.push_back(std::move(L));
+        }
         break;
       }
       case Op::Field: {
+        for (int z = 0; z < (n.zdep ? nz_ : 1); ++z) {
         auto L = std::make_unique<Launch>();
         L->kind = Launch::FIELD;
-        const Dev O = resolve(i);
+        const Dev O = resolve(i, z);
         if (n.traj >= (int)rt_->batch.size()) throw Error("internal: trajectory without parameters");
-        const bool bt = n.traj >= 0;
+        if (n.zdep && (int)rt_->batch.size() < nz_) throw Error("internal: stacked trajectories without parameters");
+        const int tj = n.zdep ? z : n.traj;
+        const bool bt = tj >= 0;
         L->f.F = rt_->F;
-        L->f.mu_pol = rt_->mu + (int64_t)(bt ? rt_->batch[n.traj].pol : rt_->pol) * n2;
+        L->f.mu_pol = rt_->mu + (int64_t)(bt ? rt_->batch[tj].pol : rt_->pol) * n2;
         L->f.out_re = O.re;
         L->f.out_im = O.im;
         L->f.n2 = n2;
-        L->f.negA = bt ? rt_->batch[n.traj].negA : rt_->negA;
-        L->f.table = bt ? rt_->batch[n.traj].field_table : rt_->field_table;
+        L->f.negA = bt ? rt_->batch[tj].negA : rt_->negA;
+        L->f.table = bt ? rt_->batch[tj].field_table : rt_->field_table;
         L->f.counter = rt_->counter;
         L->f.stage = n.stage;
         launches_.push_back(std::move(L));
+        }
         break;
       }
       case Op::Finalize: {
+        for (int z = 0; z < (n.zdep ? nz_ : 1); ++z) {
         auto L = std::make_unique<Launch>();
         L->kind = Launch::FIN;
-        const Dev Y = resolve(n.a);
-        const Dev Bs = resolve(n.b);
+        const Dev Y = resolve(n.a, z);
+        const Dev Bs = resolve(n.b, z);
         L->fin.y_re = Y.re;
         L->fin.y_im = Y.im;
         L->fin.base_re = Bs.re;
         L->fin.base_im = Bs.im;
         for (int s = 0; s < 4; ++s) {
-          const Dev K = resolve(n.k[s]);
+          const Dev K = resolve(n.k[s], z);
           L->fin.k_re[s] = K.re;
           L->fin.k_im[s] = K.im;
           if (!nodes_[n.k[s]].cplx) throw Error("internal: real stage derivative");
@@ -1951,12 +2062,14 @@ void Graph::bind(double* arena) {
         L->fin.scale = rt_->round_scale;
         L->fin.lambda_sign = n.lambda_sign;
         launches_.push_back(std::move(L));
+        }
         break;
       }
       case Op::Trace: {
+        for (int z = 0; z < (n.zdep ? nz_ : 1); ++z) {
         auto L = std::make_unique<Launch>();
         L->kind = Launch::TRACE;
-        const Dev d0 = resolve(n.k[0]), d1 = resolve(n.k[1]), d2 = resolve(n.k[2]), d3 = resolve(n.k[3]);
+        const Dev d0 = resolve(n.k[0], z), d1 = resolve(n.k[1], z), d2 = resolve(n.k[2], z), d3 = resolve(n.k[3], z);
         L->tr.dij_re = d0.re;
         L->tr.dij_im = d0.im;
         L->tr.l1_re = d1.re;
@@ -1969,9 +2082,12 @@ void Graph::bind(double* arena) {
         L->tr.o = o_;
         L->tr.v = v_;
         if (n.traj >= (int)rt_->batch.size()) throw Error("internal: trajectory without parameters");
-        L->tr.out = n.traj >= 0 ? rt_->batch[n.traj].mu_out : rt_->mu_out;
+        if (n.zdep && (int)rt_->batch.size() < nz_) throw Error("internal: stacked trajectories without parameters");
+        const int tj = n.zdep ? z : n.traj;
+        L->tr.out = tj >= 0 ? rt_->batch[tj].mu_out : rt_->mu_out;
         L->tr.counter = rt_->counter;
         launches_.push_back(std::move(L));
+        }
         break;
       }
       case Op::FieldVjp: {
@@ -2054,6 +2170,122 @@ void Graph::bind(double* arena) {
     TDCC_HIP(hipGetLastError());
     TDCC_HIP(hipStreamSynchronize(rt_->stream));
   }
+  const char* at = std::getenv("TDCC_AUTOTUNE");
+  if (at ? at[0] == '1' : nz_ > 1) autotune();
+}
+
+namespace {
+struct TuneChoice {
+  int group = 0, pb = -1, k_chunk = 0, splits = 1;
+  float ms = 0.f;
+};
+std::mutex g_tune_mu;
+std::map<std::string, TuneChoice> g_tune;  // launch shape -> choice (this process)
+}  // namespace
+
+void Graph::autotune() {
+  hipStream_t s = rt_->stream;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  TDCC_HIP(hipEventCreate(&e0));
+  TDCC_HIP(hipEventCreate(&e1));
+  std::string report;
+  try {
+    for (auto& L : launches_) {
+      OzGemmArgs* oz = nullptr;
+      GemmScatterArgs* sc = nullptr;
+      int* lsplits = nullptr;
+      if (L->kind == Launch::GETT && L->gp.on && L->gp.S > 0 && !L->gp.dot && L->g.M > 0 && L->g.N > 0) {
+        oz = &L->oz;
+        if (!L->gp.fused) sc = &L->sc;
+      } else if (L->kind == Launch::LADDER && L->lad.S > 0) {
+        oz = &L->lad.oz;
+        lsplits = &L->lad.splits;
+      }
+      if (!oz) continue;
+      char key[160];
+      std::snprintf(key, sizeof key, "S=%d M=%d N=%d Kp=%d chunk=%d splits=%d epi=%d", oz->S, oz->M, oz->N, oz->Kp,
+                    oz->k_chunk, oz->splits, oz->epi);
+      TuneChoice best;
+      bool have = false;
+      {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+          best = it->second;
+          have = true;
+        }
+      }
+      if (!have) {
+        std::vector<TuneChoice> cand;
+        std::vector<std::pair<int, int>> ks = {{oz->k_chunk, oz->splits}};
+        if (oz->splits > 2)  // fewer K chunks (the partial buffers hold the planned count)
+          for (int sp = oz->splits / 2; sp >= 2; sp /= 2) {
+            int64_t ch = (oz->Kp + sp - 1) / sp;
+            ch = (ch + 63) / 64 * 64;
+            if (ch > ozgemm_max_k(oz->S)) break;
+            ks.push_back({(int)ch, (int)((oz->Kp + ch - 1) / ch)});
+          }
+        for (auto [ch, sp] : ks)
+          for (int grp : {0, 1, 4, 8})
+            for (int pb : {-1, 0, 256, 512}) {
+              if (sp > 1 && pb > 0) continue;  // persistent blocks only without K splits
+              TuneChoice c;
+              c.group = grp;
+              c.pb = pb;
+              c.k_chunk = ch;
+              c.splits = sp;
+              cand.push_back(c);
+            }
+        OzGemmArgs base = *oz;
+        for (auto& c : cand) {
+          OzGemmArgs t = base;
+          t.tune_group = c.group;
+          t.tune_pb = c.pb;
+          t.k_chunk = c.k_chunk;
+          t.splits = c.splits;
+          launch_ozgemm(t, s);  // warm
+          TDCC_HIP(hipEventRecord(e0, s));
+          for (int r = 0; r < 3; ++r) launch_ozgemm(t, s);
+          TDCC_HIP(hipEventRecord(e1, s));
+          TDCC_HIP(hipEventSynchronize(e1));
+          float ms = 0.f;
+          TDCC_HIP(hipEventElapsedTime(&ms, e0, e1));
+          c.ms = ms / 3.f;
+          if (!have || c.ms < best.ms) {
+            best = c;
+            have = true;
+          }
+        }
+        TDCC_HIP(hipGetLastError());
+        {
+          std::lock_guard<std::mutex> lk(g_tune_mu);
+          g_tune[key] = best;
+        }
+        char line[320];
+        std::snprintf(line, sizeof line, "%s\tgroup=%d\tpersistent=%d\tk_chunk=%d\tsplits=%d\tms=%.4f\tdefault_ms=%.4f\n",
+                      key, best.group, best.pb, best.k_chunk, best.splits, best.ms, cand.front().ms);
+        report += line;
+      }
+      oz->tune_group = best.group;
+      oz->tune_pb = best.pb;
+      oz->k_chunk = best.k_chunk;
+      oz->splits = best.splits;
+      if (sc) sc->splits = best.splits;
+      if (lsplits) *lsplits = best.splits;
+    }
+  } catch (...) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
+  }
+  TDCC_HIP(hipEventDestroy(e0));
+  TDCC_HIP(hipEventDestroy(e1));
+  if (const char* f = std::getenv("TDCC_AUTOTUNE_FILE"); f && !report.empty()) {
+    if (FILE* fp = std::fopen(f, "a")) {
+      std::fputs(report.c_str(), fp);
+      std::fclose(fp);
+    }
+  }
 }
 
 static DgemmArgs ladder_gemm(const LadderArgs& a) {
@@ -2061,7 +2293,7 @@ static DgemmArgs ladder_gemm(const LadderArgs& a) {
   g.A = a.up;
   g.B = a.vp;
   g.C = a.rp;
-  g.M = (int)(2 * a.rows);
+  g.M = (int)(2 * a.rows * a.nz);
   g.N = (int)a.P;
   g.K = (int)a.P;
   g.lda = g.ldb = g.ldc = a.P;
@@ -2069,20 +2301,33 @@ static DgemmArgs ladder_gemm(const LadderArgs& a) {
   g.beta = 0.0;
   g.k_chunk = (int)a.k_chunk;
   g.splits = a.S > 0 ? 1 : a.splits;
-  g.c_split_stride = 2 * a.rows * a.P;
+  g.c_split_stride = 2 * a.rows * a.nz * a.P;
   return g;
 }
 
+// Packing and unpacking of each trajectory's rows (one trajectory unless stacked).
+static void ladder_pack(const LadderArgs& a, hipStream_t s) {
+  for (int z = 0; z < a.nz; ++z)
+    launch_pack_pairs(a.u_re + z * a.u_zs, a.u_im + z * a.u_zs, a.up + z * 2 * a.rows * a.P, a.o, a.v, a.ij_pairs, s);
+}
+
+static void ladder_unpack(const LadderArgs& a, hipStream_t s) {
+  for (int z = 0; z < a.nz; ++z)
+    launch_unpack_pairs(a.rp + z * 2 * a.rows * a.P, a.out_re + z * a.out_zs, a.out_im + z * a.out_zs, a.o, a.v, 1.0,
+                        a.ij_pairs, s);
+}
+
 static void run_ladder(const LadderArgs& a, hipStream_t s) {
-  launch_pack_pairs(a.u_re, a.u_im, a.up, a.o, a.v, a.ij_pairs, s);
+  ladder_pack(a, s);
   if (a.S > 0) {
     launch_ozaki_split(a.su, false, s);
     launch_ozgemm(a.oz, s);
   } else {
     launch_dgemm(ladder_gemm(a), true, s);
   }
-  launch_sum_splits(a.rp, a.splits, 2 * a.rows * a.P, 2 * a.rows * a.P, s);
-  launch_unpack_pairs(a.rp, a.out_re, a.out_im, a.o, a.v, 1.0, a.ij_pairs, s);
+  const int64_t packed = 2 * a.rows * a.nz * a.P;
+  launch_sum_splits(a.rp, a.splits, packed, packed, s);
+  ladder_unpack(a, s);
 }
 
 void Graph::run(hipStream_t s) const {
@@ -2261,14 +2506,12 @@ void Graph::run_profiled(hipStream_t s, std::vector<LaunchRecord>& out) const {
         }
         case Launch::LADDER: {
           const LadderArgs& a = L->lad;
-          const double elems = 2.0 * a.o * a.o * a.v * a.v, packed = 2.0 * a.rows * a.P;
-          timed("pack_pairs_kernel", 0.0, 8.0 * (2.0 * a.rows * a.v * a.v + packed),
-                [&] { launch_pack_pairs(a.u_re, a.u_im, a.up, a.o, a.v, a.ij_pairs, s); });
+          const double elems = 2.0 * a.o * a.o * a.v * a.v * a.nz, packed = 2.0 * a.rows * a.nz * a.P;
+          timed("pack_pairs_kernel", 0.0, 8.0 * (2.0 * a.rows * a.nz * a.v * a.v + packed), [&] { ladder_pack(a, s); });
           const DgemmArgs g = ladder_gemm(a);
           char dt[96];
           if (a.S > 0) {
-            timed("ozaki_split_rows_kernel", 0.0, (8.0 + a.S) * 2.0 * a.rows * a.P,
-                  [&] { launch_ozaki_split(a.su, false, s); });
+            timed("ozaki_split_rows_kernel", 0.0, (8.0 + a.S) * packed, [&] { launch_ozaki_split(a.su, false, s); });
             std::snprintf(dt, sizeof dt, "packed ladder M=%d N=%d K=%d digits=%d", g.M, g.N, g.K, a.S);
             detail = dt;
             timed(ozgemm_kernel_name(a.oz.S), 2.0 * g.M * (double)g.N * g.K, 0.0, [&] { launch_ozgemm(a.oz, s); });
@@ -2279,9 +2522,8 @@ void Graph::run_profiled(hipStream_t s, std::vector<LaunchRecord>& out) const {
           }
           if (a.splits > 1)
             timed("sum_splits_kernel", 0.0, 8.0 * packed * (a.splits + 1),
-                  [&] { launch_sum_splits(a.rp, a.splits, 2 * a.rows * a.P, 2 * a.rows * a.P, s); });
-          timed("unpack_pairs_kernel", 0.0, 8.0 * (elems + packed),
-                [&] { launch_unpack_pairs(a.rp, a.out_re, a.out_im, a.o, a.v, 1.0, a.ij_pairs, s); });
+                  [&] { launch_sum_splits(a.rp, a.splits, (int64_t)packed, (int64_t)packed, s); });
+          timed("unpack_pairs_kernel", 0.0, 8.0 * (elems + packed), [&] { ladder_unpack(a, s); });
           break;
         }
         case Launch::FIELD:
@@ -2325,7 +2567,7 @@ int64_t Graph::contract_flops() const {
   const int64_t P = (int64_t)v_ * (v_ - 1) / 2;
   for (size_t i = 0; i < nodes_.size(); ++i)
     if (nodes_[i].op == Op::Ladder && i < live_.size() && live_[i])
-      f += 2 * ladder_rows2(o_, nodes_[i].ij_pairs) * P * P;
+      f += 2 * ladder_rows2(o_, nodes_[i].ij_pairs) * (nodes_[i].zdep ? nz_ : 1) * P * P;
   return f;
 }
 

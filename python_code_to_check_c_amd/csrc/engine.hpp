@@ -100,6 +100,14 @@ struct Node {
   // unchanged under exchanging axes 0, 1 and axes 2, 3 together (the outer
   // product 'ia,jb->ijab' of a tensor with itself)
   bool xsym = false;
+  // stacked trajectories (Graph::set_stack): the node holds one value per
+  // trajectory z, trajectory z at element offset z * zs from trajectory 0
+  // (zdep false, zs 0: one value shared by all trajectories)
+  bool zdep = false;
+  int64_t zs = 0;
+  // Contract: the launches run once per trajectory (both operands per
+  // trajectory on a dense-GEMM path) instead of once over the stacked rows
+  bool zloop = false;
 };
 
 struct Handle {
@@ -185,6 +193,17 @@ class Graph {
   // Field and Trace nodes created from now on belong to trajectory b of a
   // batched program (-1: not batched).
   void set_traj(int b) { traj_ = b; }
+  // Stacked trajectories: the graph describes one trajectory and runs it for
+  // nz of them at once.  Externals declared per trajectory (set_zstride) and
+  // Field nodes (one field per trajectory, from Runtime::batch) carry a
+  // trajectory axis, and so does every node computed from them; nodes of
+  // integrals alone are shared.  Contractions with one per-trajectory operand
+  // stack the trajectories into that operand's free rows (one GEMM against
+  // the shared operand); linear combinations, GETT batches and the rest take
+  // the trajectory axis in their launch, or run once per trajectory.
+  void set_stack(int nz) { nz_ = nz < 1 ? 1 : nz; }
+  int stack() const { return nz_; }
+  void set_zstride(int ext_id, int64_t zs) { nodes_[ext_id].zs = zs; }
   // Particle-particle ladder over the antisymmetric integral block:
   //   ket == 0: out_ijab = sum_ef <ab||ef> u_ijef   ('abef,ijef->ijab')
   //   ket == 1: out_ijab = sum_ef <ef||ab> u_ijef   ('efab,ijef->ijab')
@@ -229,6 +248,12 @@ class Graph {
   void plan();                  // liveness + arena layout + offset tables
   int64_t arena_bytes() const { return arena_bytes_; }
   void bind(double* arena);     // resolve device pointers, build launch records
+  // Autotuner of the digit GEMM launches (tile order, persistent grid, K
+  // splits up to the planned count), run at bind on the bound operands; on by
+  // default for stacked programs, TDCC_AUTOTUNE=1 / 0 forces it on / off.
+  // Choices are kept per launch shape for the process and appended to the
+  // file TDCC_AUTOTUNE_FILE names.
+  void autotune();
   void run(hipStream_t s) const;
 
   // One eager pass with a HIP event pair around every kernel launch.
@@ -246,7 +271,9 @@ class Graph {
 
  private:
   int add(Node n);
-  Dev resolve(int id) const;
+  Dev resolve(int id, int z = 0) const;
+  void mark_stack();               // zdep / zs of every live node (plan)
+  int64_t zbytes(const Node& n) const;  // bytes of one trajectory's buffer
   void dump_nodes() const;  // TDCC_DUMP_NODES=1 at plan
   void lower_contract(int id);
   void lower_lincomb(int id);
@@ -266,6 +293,7 @@ class Graph {
   int ones_id_ = -1;
   int fp64_depth_ = 0;
   int traj_ = -1;
+  int nz_ = 1;
   bool ij_pairs_ = false;
   bool k_pairs_ = false;
   // digit planes of static operands (split precision), filled at bind

@@ -325,7 +325,8 @@ __global__ void lincomb_kernel(LincombArgs a) {
         rem = q;
       }
     }
-    int64_t oo = 0;
+    const int64_t zo = blockIdx.y;
+    int64_t oo = zo * a.ozs;
     for (int d = 0; d < a.rank; ++d) oo += ix[d] * a.ost[d];
     double sr = 0.0, si = 0.0;
     if (a.accumulate) {
@@ -334,7 +335,7 @@ __global__ void lincomb_kernel(LincombArgs a) {
     }
     for (int t = 0; t < a.nterms; ++t) {
       const LincombTerm& T = a.t[t];
-      int64_t so = 0, so2 = 0;
+      int64_t so = zo * T.zs, so2 = zo * T.zs2;
       for (int d = 0; d < a.rank; ++d) {
         so += ix[d] * T.st[d];
         so2 += ix[d] * T.st2[d];
@@ -368,35 +369,37 @@ __global__ void lincomb_kernel(LincombArgs a) {
 // as lincomb_kernel.
 __global__ __launch_bounds__(256) void lincomb_flat_kernel(LincombArgs a) {
   const int64_t n2 = a.total >> 1;
+  const int64_t zo = blockIdx.y, oz = zo * a.ozs;
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q <= n2; q += (int64_t)gridDim.x * blockDim.x) {
     const int64_t e = 2 * q;
     if (e >= a.total) break;
     const bool pair = e + 1 < a.total;
     double sr[2] = {0.0, 0.0}, si[2] = {0.0, 0.0};
     if (a.accumulate) {
-      sr[0] = a.out_re[e];
-      if (pair) sr[1] = a.out_re[e + 1];
+      sr[0] = a.out_re[oz + e];
+      if (pair) sr[1] = a.out_re[oz + e + 1];
       if (a.out_im) {
-        si[0] = a.out_im[e];
-        if (pair) si[1] = a.out_im[e + 1];
+        si[0] = a.out_im[oz + e];
+        if (pair) si[1] = a.out_im[oz + e + 1];
       }
     }
     for (int t = 0; t < a.nterms; ++t) {
       const LincombTerm& T = a.t[t];
+      const int64_t te = zo * T.zs + e;
       double xr[2], xi[2] = {0.0, 0.0};
       if (pair) {
-        const double2 v = *reinterpret_cast<const double2*>(T.re + e);
+        const double2 v = *reinterpret_cast<const double2*>(T.re + te);
         xr[0] = v.x;
         xr[1] = v.y;
         if (T.im) {
-          const double2 w = *reinterpret_cast<const double2*>(T.im + e);
+          const double2 w = *reinterpret_cast<const double2*>(T.im + te);
           xi[0] = w.x;
           xi[1] = w.y;
         }
       } else {
-        xr[0] = T.re[e];
+        xr[0] = T.re[te];
         xr[1] = 0.0;
-        if (T.im) xi[0] = T.im[e];
+        if (T.im) xi[0] = T.im[te];
       }
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
@@ -418,11 +421,11 @@ __global__ __launch_bounds__(256) void lincomb_flat_kernel(LincombArgs a) {
       }
     }
     if (pair) {
-      *reinterpret_cast<double2*>(a.out_re + e) = make_double2(sr[0], sr[1]);
-      if (a.out_im) *reinterpret_cast<double2*>(a.out_im + e) = make_double2(si[0], si[1]);
+      *reinterpret_cast<double2*>(a.out_re + oz + e) = make_double2(sr[0], sr[1]);
+      if (a.out_im) *reinterpret_cast<double2*>(a.out_im + oz + e) = make_double2(si[0], si[1]);
     } else {
-      a.out_re[e] = sr[0];
-      if (a.out_im) a.out_im[e] = si[0];
+      a.out_re[oz + e] = sr[0];
+      if (a.out_im) a.out_im[oz + e] = si[0];
     }
   }
 }
@@ -437,9 +440,10 @@ static bool lincomb_is_flat(const LincombArgs& a) {
     if (a.ost[d] != st && a.dims[d] != 1) return false;
     st *= a.dims[d];
   }
+  if (a.nz > 1 && (a.ozs & 1)) return false;
   for (int t = 0; t < a.nterms; ++t) {
     const LincombTerm& T = a.t[t];
-    if (T.re2 || !al16(T.re) || !al16(T.im)) return false;
+    if (T.re2 || !al16(T.re) || !al16(T.im) || (a.nz > 1 && (T.zs & 1))) return false;
     for (int d = 0; d < a.rank; ++d)
       if (T.st[d] != a.ost[d] && a.dims[d] != 1) return false;
   }
@@ -477,9 +481,10 @@ __device__ __forceinline__ void lincomb_tile_terms(const LincombArgs& a, const i
                                                    double* sr, double* si) {
   const int R = a.rank;
   const int lx = threadIdx.x & 31, ly = threadIdx.x >> 5;
+  const int64_t zo = blockIdx.y;
   for (int t = 0; t < a.nterms; ++t) {
     const LincombTerm& T = a.t[t];
-    int64_t sbase = 0, sbase2 = 0;
+    int64_t sbase = zo * T.zs, sbase2 = zo * T.zs2;
     for (int d = 0; d < nlead; ++d) {
       sbase += lead[d] * T.st[d];
       sbase2 += lead[d] * T.st2[d];
@@ -553,7 +558,7 @@ __global__ __launch_bounds__(256) void lincomb_tiled_kernel(LincombArgs a) {
     lead[0] = outer;
   }
   const int lx = threadIdx.x & 31, ly = threadIdx.x >> 5;
-  int64_t obase = 0;
+  int64_t obase = (int64_t)blockIdx.y * a.ozs;
   for (int d = 0; d < R - 2; ++d) obase += lead[d] * a.ost[d];
   double sr[4], si[4];
 #pragma unroll
@@ -619,9 +624,10 @@ __global__ __launch_bounds__(256) void lincomb_anti_kernel(LincombArgs a, int64_
   const int lx = threadIdx.x & 31, ly = threadIdx.x >> 5;
   const bool diag_tile = aab && tA == tB;
   const int64_t os2 = a.ost[2];
+  const int64_t oz = (int64_t)blockIdx.y * a.ozs;
   if (pidx >= npairs) {  // zero slice x[i][i]
     const int64_t i = pidx - npairs;
-    const int64_t ob = i * (a.ost[0] + a.ost[1]);
+    const int64_t ob = oz + i * (a.ost[0] + a.ost[1]);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int64_t r = r0 + ly + 8 * k, c = c0 + lx;
@@ -644,8 +650,8 @@ __global__ __launch_bounds__(256) void lincomb_anti_kernel(LincombArgs a, int64_
     lead[0] = pidx / D1;
     lead[1] = pidx - lead[0] * D1;
   }
-  const int64_t ob = lead[0] * a.ost[0] + lead[1] * a.ost[1];
-  const int64_t obm = lead[1] * a.ost[0] + lead[0] * a.ost[1];  // x[j][i] (aij)
+  const int64_t ob = oz + lead[0] * a.ost[0] + lead[1] * a.ost[1];
+  const int64_t obm = oz + lead[1] * a.ost[0] + lead[0] * a.ost[1];  // x[j][i] (aij)
   double sr[4], si[4];
   lincomb_tile_terms(a, lead, 2, r0, c0, D2, D3, tr, ti, sr, si);
   if (diag_tile) {
@@ -721,18 +727,19 @@ static bool lincomb_use_tiled(const LincombArgs& a) {
 
 void launch_lincomb(const LincombArgs& a, hipStream_t s) {
   if (a.total <= 0) return;
+  const unsigned nz = (unsigned)(a.nz < 1 ? 1 : a.nz);
   if (lincomb_use_anti(a)) {
     const int64_t n2 = (a.dims[2] + kLinTile - 1) / kLinTile, n3 = (a.dims[3] + kLinTile - 1) / kLinTile;
     const int64_t npairs = (a.anti & 1) ? a.dims[0] * (a.dims[0] - 1) / 2 : a.dims[0] * a.dims[1];
     const int64_t ntiles = (a.anti & 2) ? n3 * (n3 + 1) / 2 : n2 * n3;
     const int64_t blocks = (npairs + ((a.anti & 1) ? a.dims[0] : 0)) * ntiles;
     if (blocks > 0 && blocks < (int64_t)1 << 31) {
-      hipLaunchKernelGGL(lincomb_anti_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a, npairs, ntiles);
+      hipLaunchKernelGGL(lincomb_anti_kernel, dim3((unsigned)blocks, nz), dim3(256), 0, s, a, npairs, ntiles);
       return;
     }
   }
   if (lincomb_is_flat(a)) {
-    hipLaunchKernelGGL(lincomb_flat_kernel, dim3(grid_for((a.total + 1) / 2)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(lincomb_flat_kernel, dim3(grid_for((a.total + 1) / 2), nz), dim3(256), 0, s, a);
     return;
   }
   if (lincomb_use_tiled(a)) {
@@ -742,12 +749,12 @@ void launch_lincomb(const LincombArgs& a, hipStream_t s) {
     for (int d = 0; d < R - 2; ++d) outer *= a.dims[d];
     const int64_t blocks = outer * n2 * n3;
     if (blocks < (int64_t)1 << 31) {
-      hipLaunchKernelGGL(lincomb_tiled_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a);
+      hipLaunchKernelGGL(lincomb_tiled_kernel, dim3((unsigned)blocks, nz), dim3(256), 0, s, a);
       return;
     }
   }
-  if (a.total < ((int64_t)1 << 31)) hipLaunchKernelGGL(lincomb_kernel<true>, dim3(grid_for(a.total)), dim3(256), 0, s, a);
-  else hipLaunchKernelGGL(lincomb_kernel<false>, dim3(grid_for(a.total)), dim3(256), 0, s, a);
+  if (a.total < ((int64_t)1 << 31)) hipLaunchKernelGGL(lincomb_kernel<true>, dim3(grid_for(a.total), nz), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL(lincomb_kernel<false>, dim3(grid_for(a.total), nz), dim3(256), 0, s, a);
 }
 
 const char* lincomb_kernel_name(const LincombArgs& a) {

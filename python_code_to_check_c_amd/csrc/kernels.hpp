@@ -68,6 +68,9 @@ struct LincombTerm {
   // product term: coef * x * y with y = (re2, im2) read at st2 (re2 null: none)
   const double* re2 = nullptr; const double* im2 = nullptr;
   int64_t st2[4] = {0, 0, 0, 0};
+  // stacked trajectories: element offset of trajectory z is z * zs (zs2 for
+  // the product factor); 0 for an operand shared by all trajectories
+  int64_t zs = 0, zs2 = 0;
 };
 struct LincombArgs {
   double* out_re; double* out_im;
@@ -82,6 +85,10 @@ struct LincombArgs {
   // and 1, bit 1 for axes 2 and 3 (every term is then read at the elements
   // with i < j / a <= b tiles only and the rest is stored as mirror images)
   int anti = 0;
+  // stacked trajectories: nz trajectories over grid.y, the output of
+  // trajectory z at element offset z * ozs
+  int nz = 1;
+  int64_t ozs = 0;
 };
 void launch_lincomb(const LincombArgs& a, hipStream_t s);
 // Name of the kernel launch_lincomb runs for these arguments (profiles).
@@ -331,11 +338,19 @@ struct OzGemmArgs {
   double* out_re = nullptr; double* out_im = nullptr;
   double alpha_re = 1.0, alpha_im = 0.0;
   int tile_group = 1;                 // bands of tile rows (grouped_tile); 1: row index fastest
+  // launch choices of an autotuner (bitwise neutral: every output is summed by
+  // one block in the same order either way): 0 / -1 keep launch_ozgemm's
+  // defaults; tune_group > 0 bands of tile rows; tune_pb 0 one block per
+  // tile, > 0 that many persistent blocks
+  int tune_group = 0;
+  int tune_pb = -1;
 };
 void launch_ozgemm(const OzGemmArgs& a, hipStream_t s);
 
 // Name of the kernel launch_ozgemm runs for S digits (profiles).
-inline const char* ozgemm_kernel_name(int S) { return S <= 4 ? "ozgemm8_kernel" : "ozgemm_kernel"; }
+inline const char* ozgemm_kernel_name(int S) {
+  return S <= 4 ? "ozgemm8_kernel" : (S == 5 ? "ozgemm8n_kernel" : "ozgemm_kernel");
+}
 // Largest K range one int32 accumulator may sum exactly for S digits.
 int64_t ozgemm_max_k(int S);
 

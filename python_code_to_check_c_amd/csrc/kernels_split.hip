@@ -711,6 +711,168 @@ __global__ __launch_bounds__(512, 1) void ozgemm8_kernel(OzGemmArgs g) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Digit-pair GEMM for 5 and 6 digits on v_mfma_i32_32x32x32_i8: the 8-wave
+// 128 x 128 tile of the kernel above with K steps of 32 (one 32-wide slice),
+// so that the double-buffered digit tiles of A and B fit the 160 KB of LDS
+// (2 x S x 8 KB per operand).  Threads 0..255 load the A tile, 256..511 the B
+// tile (one 16-byte chunk per digit each), and a wave reads its A digits for
+// one 32-row block at a time and one B digit at a time, which keeps the
+// S * 32 accumulator registers, operands and prefetch within 256 registers
+// (two waves per SIMD).  LDS rows are 32 bytes with the chunk index XORed by
+// (row >> 3) & 1: 16 lanes reading rows r..r+15 at one chunk then cover the 64
+// banks once.  Epilogue, tile order and persistence as in ozgemm8_kernel.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int oz_swz2(int row, int kc) { return row * 2 + (kc ^ ((row >> 3) & 1)); }
+
+template <int S>
+__global__ __launch_bounds__(512, 1) void ozgemm8n_kernel(OzGemmArgs g) {
+  constexpr int BN = 128, BK = 32;
+  __shared__ v4i As[2][S][OZ_BM * 2];
+  __shared__ v4i Bs[2][S][BN * 2];
+  __shared__ double sA[OZ_BM], sB[BN];
+  const int tiles_m = (g.M + OZ_BM - 1) / OZ_BM, tiles_n = (g.N + BN - 1) / BN;
+  const int tiles = tiles_m * tiles_n;
+  const int tid = threadIdx.x;
+  const int kb = g.k_chunk > 0 ? (int)blockIdx.y * g.k_chunk : 0;
+  const int ke = g.k_chunk > 0 ? min(g.Kp, kb + g.k_chunk) : g.Kp;
+  double* const Cout = g.C + (int64_t)blockIdx.y * g.c_split_stride;
+  const bool on_b = tid >= 256;
+  const int lt = tid & 255, lrow = lt >> 1, lchunk = lt & 1;
+  const __amdgpu_buffer_rsrc_t rX = on_b
+      ? __builtin_amdgcn_make_buffer_rsrc((void*)g.B, (short)0, (int)(S * g.b_plane), 0x00020000)
+      : __builtin_amdgcn_make_buffer_rsrc((void*)g.A, (short)0, (int)(S * g.a_plane), 0x00020000);
+  const int xplane = on_b ? (int)g.b_plane : (int)g.a_plane;
+  int m0 = 0, n0 = 0, vx = 0;
+  auto setup = [&](int t) {
+    const int L = remap_tile(t, tiles);
+    int tm, tn;
+    grouped_tile(L, tiles_m, tiles_n, g.tile_group, tm, tn);
+    m0 = tm * OZ_BM;
+    n0 = tn * BN;
+    vx = on_b ? min(n0 + lrow, g.N - 1) * g.Kp + lchunk * 16 : min(m0 + lrow, g.M - 1) * g.Kp + lchunk * 16;
+  };
+  v4i rx[S];
+  auto load_global = [&](int k0) {
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+      rx[s] = __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(rX, vx, s * xplane + k0, 0));
+  };
+  auto store_lds = [&](int buf) {
+    if (on_b) {
+#pragma unroll
+      for (int s = 0; s < S; ++s) Bs[buf][s][oz_swz2(lrow, lchunk)] = rx[s];
+    } else {
+#pragma unroll
+      for (int s = 0; s < S; ++s) As[buf][s][oz_swz2(lrow, lchunk)] = rx[s];
+    }
+  };
+  const int lane = tid & 63, w = tid >> 6;
+  const int wm = (w & 1) * 64, wn = (w >> 1) * 32;
+  const int lr = lane & 31, lh = lane >> 5;
+  const double ar = g.alpha_re, ai = g.alpha_im;
+
+  int t = blockIdx.x;
+  setup(t);
+  load_global(kb);
+  for (;;) {
+    v16i acc[S][2];
+#pragma unroll
+    for (int q = 0; q < S; ++q)
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[q][i][e] = 0;
+    if (tid < OZ_BM) sA[tid] = m0 + tid < g.M ? g.sa[m0 + tid] : 0.0;
+    else if (tid - OZ_BM < BN) sB[tid - OZ_BM] = n0 + tid - OZ_BM < g.N ? g.sb[n0 + tid - OZ_BM] : 0.0;
+    store_lds(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = kb; k0 < ke; k0 += BK) {
+      const bool more = k0 + BK < ke;
+      if (more) load_global(k0 + BK);
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        v4i a[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) a[s] = As[buf][s][oz_swz2(wm + i * 32 + lr, lh)];
+#pragma unroll
+        for (int tt = 0; tt < S; ++tt) {
+          const v4i b = Bs[buf][tt][oz_swz2(wn + lr, lh)];
+#pragma unroll
+          for (int s = 0; s + tt < S; ++s)
+            acc[s + tt][i] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[s], b, acc[s + tt][i], 0, 0, 0);
+        }
+      }
+      if (more) store_lds(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+    }
+    const int em0 = m0, en0 = n0;
+    const int tnext = t + (int)gridDim.x;
+    if (tnext < tiles) {
+      setup(tnext);
+      load_global(kb);
+    }
+    // digit group q (pairs with s + t = q + 2, 1-based digits) weighs 254^-(q+2)
+    const int col = en0 + wn + lr;
+    const bool colok = col < g.N;
+    const double sbc = sB[wn + lr];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int gq = 0; gq < 4; ++gq) {
+        const int row0 = em0 + wm + i * 32 + 8 * gq + 4 * lh;
+        double z[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          double v = 0.0;
+#pragma unroll
+          for (int q = S - 1; q >= 0; --q) v += (double)acc[q][i][4 * gq + r] * oz_group_weight(q);
+          z[r] = (colok && row0 + r < g.M) ? v * (sA[row0 + r - em0] * sbc) : 0.0;
+        }
+        if (g.epi < 0) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (colok && row0 + r < g.M) Cout[(int64_t)(row0 + r) * g.ldc + col] = z[r];
+        } else if (g.epi == 0) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (colok && row0 + r < g.M)
+              store_mirrored(g.out_re, nullptr, g.cm[row0 + r], g.cn[col], g.cm2 ? g.cm2[row0 + r] : -1,
+                             g.cn2 ? g.cn2[col] : -1, ar * z[r], 0.0);
+        } else if (g.epi == 1) {
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            if (!colok || row0 + 2 * h >= g.M) continue;
+            const double re = z[2 * h], im = z[2 * h + 1];
+            const int mr = (row0 >> 1) + h;
+            store_mirrored(g.out_re, g.out_im, g.cm[mr], g.cn[col], g.cm2 ? g.cm2[mr] : -1, g.cn2 ? g.cn2[col] : -1,
+                           ar * re - ai * im, ar * im + ai * re);
+          }
+        } else {
+          double x[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) x[r] = __shfl_xor(z[r], 1);
+          if ((lr & 1) == 0 && colok) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+              if (row0 + 2 * h >= g.M) continue;
+              const double re = z[2 * h] - x[2 * h + 1];  // rr - ii
+              const double im = x[2 * h] + z[2 * h + 1];  // ri + ir
+              const int mr = (row0 >> 1) + h, nc = col >> 1;
+              store_mirrored(g.out_re, g.out_im, g.cm[mr], g.cn[nc], g.cm2 ? g.cm2[mr] : -1, g.cn2 ? g.cn2[nc] : -1,
+                             ar * re - ai * im, ar * im + ai * re);
+            }
+          }
+        }
+      }
+    if (tnext >= tiles) break;
+    __syncthreads();
+    t = tnext;
+  }
+}
+
 int64_t ozgemm_max_k(int S) {
   // S pairs share the largest group; |d| <= 127 for every digit
   const int64_t lim = ((int64_t)1 << 31) - 1;
@@ -750,20 +912,21 @@ static int oz_persistent_blocks() {
 void launch_ozgemm(const OzGemmArgs& args, hipStream_t s) {
   if (args.M <= 0 || args.N <= 0) return;
   OzGemmArgs g = args;
-  g.tile_group = oz_tile_group((g.M + OZ_BM - 1) / OZ_BM, (g.N + 127) / 128);
-  const int bn = g.S <= 4 ? 128 : 64;
+  g.tile_group = g.tune_group > 0 ? g.tune_group : oz_tile_group((g.M + OZ_BM - 1) / OZ_BM, (g.N + 127) / 128);
+  const int bn = g.S <= 5 ? 128 : 64;
   const int tiles = ((g.M + OZ_BM - 1) / OZ_BM) * ((g.N + bn - 1) / bn);
   const int splits = g.k_chunk > 0 ? g.splits : 1;
   const int pb = oz_persistent_blocks();
   // persistent only with few K steps per tile (at 50 K steps one block per
   // tile measured 12 % faster: the dispatcher balances long tiles better)
-  const int gx8 = (g.S <= 4 && splits == 1 && pb > 0 && tiles > pb && g.Kp <= 8 * OZ_BK) ? pb : tiles;
+  int gx8 = (g.S <= 5 && splits == 1 && pb > 0 && tiles > pb && g.Kp <= 8 * OZ_BK) ? pb : tiles;
+  if (g.tune_pb >= 0 && g.S <= 5) gx8 = (g.tune_pb > 0 && splits == 1 && g.tune_pb < tiles) ? g.tune_pb : tiles;
   const dim3 grid(tiles, splits), grid8(gx8, splits), block(256), block8(512);
   switch (g.S) {
     case 2: hipLaunchKernelGGL((ozgemm8_kernel<2>), grid8, block8, 0, s, g); break;
     case 3: hipLaunchKernelGGL((ozgemm8_kernel<3>), grid8, block8, 0, s, g); break;
     case 4: hipLaunchKernelGGL((ozgemm8_kernel<4>), grid8, block8, 0, s, g); break;
-    case 5: hipLaunchKernelGGL((ozgemm_kernel<5, 2>), grid, block, 0, s, g); break;
+    case 5: hipLaunchKernelGGL((ozgemm8n_kernel<5>), grid8, block8, 0, s, g); break;
     case 6: hipLaunchKernelGGL((ozgemm_kernel<6, 2>), grid, block, 0, s, g); break;
     default: break;
   }
