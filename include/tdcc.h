@@ -61,13 +61,21 @@ int tdcc_synth_tei(tdcc_ctx* ctx, uint32_t config_id, double sigma);
 /* Arithmetic of the large contractions (the o^2 v^4 ladders and every
  * contraction above 2 GFLOP, i.e. the o^3 v^3 rings and o^2 v^3 terms).
  * digits = 0: FP64 (the default).  digits = 2..6: split precision -- both
- * operands are cut into `digits` int8 digit planes with one power-of-two
- * scale per row (Ozaki scheme), the digit pairs are multiplied on int8 MFMA
- * with exact int32 accumulation, and the result is assembled in FP64; the
- * relative truncation per operand element is <= 0.5 * 128^-digits of its row
- * maximum.  The RK4 state, the field and Fock build, the small contractions
- * and the dipole stay FP64.  Replaces the reference's complex128 einsum
- * arithmetic (opt_einsum contract, CCSD_Helper.py:20) for those terms. */
+ * operands are cut into `digits` int8 digit planes of radix 254 (digits in
+ * [-127, 127]) with one power-of-two scale per row (Ozaki scheme), the digit
+ * pairs are multiplied on int8 MFMA with exact int32 accumulation, and the
+ * result is assembled in FP64; the truncation per operand element is at most
+ * 0.5 * 254^-digits of its row scale (< 2 * 254^-digits of the row maximum).
+ * The RK4 state, the field and Fock build, the small contractions and the
+ * dipole stay FP64.  Replaces the reference's complex128 einsum arithmetic
+ * (opt_einsum contract, CCSD_Helper.py:20) for those terms.
+ *
+ * Launch autotuning (environment): TDCC_AUTOTUNE=1 times candidate tile
+ * orders, persistent grids and K splits (up to the planned count) for every
+ * digit-GEMM launch shape when a program is bound, and keeps the fastest for
+ * the process (on by default for batched programs); TDCC_AUTOTUNE_FILE
+ * appends each choice to that file.  Tile order and persistence do not change
+ * results; a different K split changes the order of the FP64 partial sums. */
 int tdcc_set_precision(tdcc_ctx* ctx, int digits);
 
 /* Propagated state t1, t2, lam1, lam2 (complex, interleaved). */
@@ -135,8 +143,11 @@ int tdcc_get_state_slot(tdcc_ctx* ctx, int slot, int which, double* t1, double* 
 
 /* The loop body of tdcc_rk4_run for every slot at once: slot b uses the field
  * -A[b] mu[pol[b]] exp(i 2 pi w0[b] t); all slots share t0, dt and precs.  One
- * captured graph advances all slots by one step.  mu_out: nb x nsteps x 3
- * complex (slot-major); t_out: nsteps times. */
+ * captured graph advances all slots by one step: one step program with a
+ * trajectory axis over the slots, in which contractions of shared integral
+ * blocks with the slots' amplitudes run as one GEMM over the slots' stacked
+ * rows (TDCC_STACK=0: the slots' step programs side by side, one launch per
+ * slot).  mu_out: nb x nsteps x 3 complex (slot-major); t_out: nsteps times. */
 int tdcc_rk4_run_batch(tdcc_ctx* ctx, int method, int nsteps, double t0, double dt, const int* pol,
                        const double* w0, const double* A, int precs, double* mu_out, double* t_out);
 
