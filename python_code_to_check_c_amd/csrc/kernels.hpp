@@ -348,9 +348,7 @@ struct OzGemmArgs {
 void launch_ozgemm(const OzGemmArgs& a, hipStream_t s);
 
 // Name of the kernel launch_ozgemm runs for S digits (profiles).
-inline const char* ozgemm_kernel_name(int S) {
-  return S <= 4 ? "ozgemm8_kernel" : (S == 5 ? "ozgemm8n_kernel" : "ozgemm_kernel");
-}
+const char* ozgemm_kernel_name(int S);
 // Largest K range one int32 accumulator may sum exactly for S digits.
 int64_t ozgemm_max_k(int S);
 

@@ -873,6 +873,184 @@ __global__ __launch_bounds__(512, 1) void ozgemm8n_kernel(OzGemmArgs g) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// The 8-wave digit-pair GEMM with asynchronous global-to-LDS copies
+// (global_load_lds_dwordx4: lane l of a wave writes its 16 bytes at the wave's
+// LDS base + 16 l) into three LDS buffers, loads issued two K steps ahead: a K
+// step of 32 (S 15- or 10-MFMA groups per 32-row block) is shorter than a
+// global round trip, which a one-step register prefetch left exposed.  No
+// staging registers and no LDS store instructions; one barrier per K step.
+// Tile 128 x 128, 8 waves (2 x 4), each wave 64 x 32 outputs; LDS rows of 32
+// bytes as in ozgemm8n_kernel (chunk XORed with (row >> 3) & 1): lane l of
+// wave w (A side w < 4, B side w >= 4) fills physical chunk c = 64 (w & 3) + l
+// of a digit plane, i.e. row c >> 1, logical chunk (c & 1) ^ ((row >> 3) & 1).
+// Epilogue, tile order and persistence as in ozgemm8_kernel.
+// ---------------------------------------------------------------------------
+// One 16-byte global -> LDS copy per lane, LDS address = dst (wave-uniform)
+// + 16 * lane.  Written as one asm block so that the compiler does not track
+// it: its waits are explicit s_waitcnt vmcnt(n) in the caller, which lets two
+// K steps stay in flight across the step's barrier (the compiler's own
+// tracking of these copies waits for all of them at every barrier).  The block
+// has no register outputs; the compiler sets m0 (an input operand), and the
+// s_nop covers the m0 write -> LDS copy hazard.
+__device__ __forceinline__ void lds_copy16(const int8_t* src, v4i* dst) {
+  const uint32_t lds = __builtin_amdgcn_readfirstlane(
+      (uint32_t)(uintptr_t)(__attribute__((address_space(3))) v4i*)dst);
+  asm volatile("s_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(src), "{m0}"(lds) : "memory");
+}
+
+template <int S>
+__global__ __launch_bounds__(512, 1) void ozgemm8d_kernel(OzGemmArgs g) {
+  constexpr int BN = 128, BK = 32, NBUF = 3;
+  __shared__ v4i As[NBUF][S][OZ_BM * 2];
+  __shared__ v4i Bs[NBUF][S][BN * 2];
+  __shared__ double sA[OZ_BM], sB[BN];
+  const int tiles_m = (g.M + OZ_BM - 1) / OZ_BM, tiles_n = (g.N + BN - 1) / BN;
+  const int tiles = tiles_m * tiles_n;
+  const int tid = threadIdx.x;
+  const int kb = g.k_chunk > 0 ? (int)blockIdx.y * g.k_chunk : 0;
+  const int ke = g.k_chunk > 0 ? min(g.Kp, kb + g.k_chunk) : g.Kp;
+  const int nk = (ke - kb + BK - 1) / BK;
+  double* const Cout = g.C + (int64_t)blockIdx.y * g.c_split_stride;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const bool on_b = wave >= 4;
+  const int lane = tid & 63;
+  const int pc = 64 * (wave & 3) + lane;                 // physical chunk this lane fills
+  const int lrow = pc >> 1, lchunk = (pc & 1) ^ ((lrow >> 3) & 1);
+  const int8_t* const xbase = on_b ? g.B : g.A;
+  const int64_t xplane = on_b ? g.b_plane : g.a_plane;
+  int m0 = 0, n0 = 0;
+  int64_t xrow = 0;
+  auto setup = [&](int t) {
+    const int L = remap_tile(t, tiles);
+    int tm, tn;
+    grouped_tile(L, tiles_m, tiles_n, g.tile_group, tm, tn);
+    m0 = tm * OZ_BM;
+    n0 = tn * BN;
+    xrow = on_b ? (int64_t)min(n0 + lrow, g.N - 1) * g.Kp : (int64_t)min(m0 + lrow, g.M - 1) * g.Kp;
+  };
+  // copies K step j of the current tile into buffer j % 3: one 16-byte chunk
+  // per lane per digit plane, written by the copy engine to LDS at the wave's
+  // base + 16 * lane
+  auto issue = [&](int j) {
+    const int buf = j % NBUF, k0 = kb + j * BK;
+    const int8_t* src = xbase + xrow + k0 + lchunk * 16;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      v4i* dst = on_b ? &Bs[buf][s][64 * (wave & 3)] : &As[buf][s][64 * (wave & 3)];
+      lds_copy16(src + s * xplane, dst);
+    }
+  };
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 32;
+  const int lr = lane & 31, lh = lane >> 5;
+  const double ar = g.alpha_re, ai = g.alpha_im;
+
+  int t = blockIdx.x;
+  setup(t);
+  for (;;) {
+    // the tile's scales first (their loads complete before the copies start),
+    // then the copies of its first two K steps
+    if (tid < OZ_BM) sA[tid] = m0 + tid < g.M ? g.sa[m0 + tid] : 0.0;
+    else if (tid - OZ_BM < BN) sB[tid - OZ_BM] = n0 + tid - OZ_BM < g.N ? g.sb[n0 + tid - OZ_BM] : 0.0;
+    if (nk > 0) issue(0);
+    if (nk > 1) issue(1);
+    v16i acc[S][2];
+#pragma unroll
+    for (int q = 0; q < S; ++q)
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[q][i][e] = 0;
+    for (int j = 0; j < nk; ++j) {
+      // step j's copies are the oldest outstanding: wait until only step
+      // j + 1's S copies (if issued) remain in flight
+      if (j + 1 < nk) {
+        if constexpr (S == 5) __builtin_amdgcn_s_waitcnt(0xF75);  // vmcnt(5)
+        else if constexpr (S == 4) __builtin_amdgcn_s_waitcnt(0xF74);  // vmcnt(4)
+        else __builtin_amdgcn_s_waitcnt(0);
+      } else {
+        __builtin_amdgcn_s_waitcnt(0xF70);  // vmcnt(0)
+      }
+      __syncthreads();
+      if (j + 2 < nk) issue(j + 2);
+      const int buf = j % NBUF;
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        v4i a[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) a[s] = As[buf][s][oz_swz2(wm + i * 32 + lr, lh)];
+#pragma unroll
+        for (int tt = 0; tt < S; ++tt) {
+          const v4i b = Bs[buf][tt][oz_swz2(wn + lr, lh)];
+#pragma unroll
+          for (int s = 0; s + tt < S; ++s)
+            acc[s + tt][i] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[s], b, acc[s + tt][i], 0, 0, 0);
+        }
+      }
+    }
+    const int em0 = m0, en0 = n0;
+    const int tnext = t + (int)gridDim.x;
+    const double sbc = sB[wn + lr];
+    const int col = en0 + wn + lr;
+    const bool colok = col < g.N;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int gq = 0; gq < 4; ++gq) {
+        const int row0 = em0 + wm + i * 32 + 8 * gq + 4 * lh;
+        double z[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          double v = 0.0;
+#pragma unroll
+          for (int q = S - 1; q >= 0; --q) v += (double)acc[q][i][4 * gq + r] * oz_group_weight(q);
+          z[r] = (colok && row0 + r < g.M) ? v * (sA[row0 + r - em0] * sbc) : 0.0;
+        }
+        if (g.epi < 0) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (colok && row0 + r < g.M) Cout[(int64_t)(row0 + r) * g.ldc + col] = z[r];
+        } else if (g.epi == 0) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (colok && row0 + r < g.M)
+              store_mirrored(g.out_re, nullptr, g.cm[row0 + r], g.cn[col], g.cm2 ? g.cm2[row0 + r] : -1,
+                             g.cn2 ? g.cn2[col] : -1, ar * z[r], 0.0);
+        } else if (g.epi == 1) {
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            if (!colok || row0 + 2 * h >= g.M) continue;
+            const double re = z[2 * h], im = z[2 * h + 1];
+            const int mr = (row0 >> 1) + h;
+            store_mirrored(g.out_re, g.out_im, g.cm[mr], g.cn[col], g.cm2 ? g.cm2[mr] : -1, g.cn2 ? g.cn2[col] : -1,
+                           ar * re - ai * im, ar * im + ai * re);
+          }
+        } else {
+          double x[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) x[r] = __shfl_xor(z[r], 1);
+          if ((lr & 1) == 0 && colok) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+              if (row0 + 2 * h >= g.M) continue;
+              const double re = z[2 * h] - x[2 * h + 1];  // rr - ii
+              const double im = x[2 * h] + z[2 * h + 1];  // ri + ir
+              const int mr = (row0 >> 1) + h, nc = col >> 1;
+              store_mirrored(g.out_re, g.out_im, g.cm[mr], g.cn[nc], g.cm2 ? g.cm2[mr] : -1, g.cn2 ? g.cn2[nc] : -1,
+                             ar * re - ai * im, ar * im + ai * re);
+            }
+          }
+        }
+      }
+    if (tnext >= tiles) break;
+    // every wave is done with the LDS buffers and the scales before the next
+    // tile's scales and copies replace them
+    __syncthreads();
+    t = tnext;
+    setup(t);
+  }
+}
+
 int64_t ozgemm_max_k(int S) {
   // S pairs share the largest group; |d| <= 127 for every digit
   const int64_t lim = ((int64_t)1 << 31) - 1;
@@ -909,6 +1087,21 @@ static int oz_persistent_blocks() {
   return cus;
 }
 
+// The LDS-direct kernel (global -> LDS copies, two K steps in flight) for 4
+// and 5 digits; TDCC_OZ_DIRECT=0 selects the register-staged kernels.
+bool ozgemm_direct() {
+  static const bool on = [] {
+    const char* e = std::getenv("TDCC_OZ_DIRECT");
+    return e ? std::atoi(e) != 0 : true;
+  }();
+  return on;
+}
+
+const char* ozgemm_kernel_name(int S) {
+  if ((S == 4 || S == 5) && ozgemm_direct()) return "ozgemm8d_kernel";
+  return S <= 4 ? "ozgemm8_kernel" : (S == 5 ? "ozgemm8n_kernel" : "ozgemm_kernel");
+}
+
 void launch_ozgemm(const OzGemmArgs& args, hipStream_t s) {
   if (args.M <= 0 || args.N <= 0) return;
   OzGemmArgs g = args;
@@ -922,6 +1115,11 @@ void launch_ozgemm(const OzGemmArgs& args, hipStream_t s) {
   int gx8 = (g.S <= 5 && splits == 1 && pb > 0 && tiles > pb && g.Kp <= 8 * OZ_BK) ? pb : tiles;
   if (g.tune_pb >= 0 && g.S <= 5) gx8 = (g.tune_pb > 0 && splits == 1 && g.tune_pb < tiles) ? g.tune_pb : tiles;
   const dim3 grid(tiles, splits), grid8(gx8, splits), block(256), block8(512);
+  if (ozgemm_direct() && (g.S == 4 || g.S == 5)) {
+    if (g.S == 4) hipLaunchKernelGGL((ozgemm8d_kernel<4>), grid8, block8, 0, s, g);
+    else hipLaunchKernelGGL((ozgemm8d_kernel<5>), grid8, block8, 0, s, g);
+    return;
+  }
   switch (g.S) {
     case 2: hipLaunchKernelGGL((ozgemm8_kernel<2>), grid8, block8, 0, s, g); break;
     case 3: hipLaunchKernelGGL((ozgemm8_kernel<3>), grid8, block8, 0, s, g); break;
