@@ -169,6 +169,8 @@ def main():
                     help="int8 digits per operand of the split-precision path (0: FP64 only)")
     ap.add_argument("--error-steps", type=int, default=0,
                     help="also propagate this many steps in both precisions and report the dipole difference")
+    ap.add_argument("--error-digits", default=None,
+                    help="comma-separated digit counts for the --error-steps comparison (default: --digits)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--dump-profile", default=None, help="write every profiled launch (tsv) to this file")
@@ -180,6 +182,13 @@ def main():
                     metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV", "COPY_MIB"),
                     help="only turn rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE counter files (workload and copy "
                          "calibration) into the per-kernel HBM traffic table on stdout, and exit")
+    ap.add_argument("--counter-summary", default=None, metavar="COUNTER_CSV",
+                    help="only sum a rocprofv3 --pmc counter csv per kernel (with the MFMA-busy fraction "
+                         "SQ_VALU_MFMA_BUSY_CYCLES / (4 SIMDs x SQ_BUSY_CU_CYCLES) and LDS bank-conflict rate "
+                         "where those counters are present) and exit")
+    ap.add_argument("--trace-summary", nargs=2, default=None, metavar=("KERNEL_TRACE_CSV", "STEPS"),
+                    help="only summarise a rocprofv3 --kernel-trace csv of a bench.py leg per step (STEPS = the timed "
+                         "steps, the dispatches after the last state upload) and exit")
     ap.add_argument("--batched", action="store_true",
                     help="the batched workload instead: 4 molecule shapes x 3 polarisations x 8 frequencies, "
                          "one context and one captured step per molecule (prints its own JSON line)")
@@ -206,6 +215,12 @@ def main():
         return
     if args.pmc_summary:
         pmc_summary(*args.pmc_summary)
+        return
+    if args.counter_summary:
+        counter_summary(args.counter_summary)
+        return
+    if args.trace_summary:
+        trace_summary(args.trace_summary[0], int(args.trace_summary[1]))
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding of this "
@@ -466,6 +481,63 @@ def _counter_rows(path):
                 for r in csv.DictReader(f)]
 
 
+def counter_summary(path):
+    """Per-kernel sums of every counter in a rocprofv3 --pmc csv, kernels by
+    SQ_BUSY_CU_CYCLES (or by their first counter)."""
+    import csv
+    acc, launches = {}, {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            n = r["Kernel_Name"].split("(")[0].replace("void ", "")
+            acc.setdefault(n, {}).setdefault(r["Counter_Name"], 0.0)
+            acc[n][r["Counter_Name"]] += float(r["Counter_Value"])
+            launches.setdefault(n, set()).add(r["Dispatch_Id"])
+    names = sorted({c for v in acc.values() for c in v})
+    key = "SQ_BUSY_CU_CYCLES" if "SQ_BUSY_CU_CYCLES" in names else names[0]
+    extra = []
+    if {"SQ_VALU_MFMA_BUSY_CYCLES", "SQ_BUSY_CU_CYCLES"} <= set(names):
+        extra.append(("mfma_busy_frac", lambda v: v["SQ_VALU_MFMA_BUSY_CYCLES"] / (4 * v["SQ_BUSY_CU_CYCLES"])
+                      if v["SQ_BUSY_CU_CYCLES"] else 0.0))
+    if {"SQ_LDS_BANK_CONFLICT", "SQ_LDS_IDX_ACTIVE"} <= set(names):
+        extra.append(("lds_conflict_rate", lambda v: v["SQ_LDS_BANK_CONFLICT"] / v["SQ_LDS_IDX_ACTIVE"]
+                      if v["SQ_LDS_IDX_ACTIVE"] else 0.0))
+    print("\t".join(["kernel", "launches"] + names + [e[0] for e in extra]))
+    for n, v in sorted(acc.items(), key=lambda kv: -kv[1].get(key, 0.0)):
+        print("\t".join([n, str(len(launches[n]))] + [f"{v.get(c, 0.0):.4e}" for c in names] +
+                        [f"{e[1](v):.4f}" for e in extra]))
+
+
+def trace_summary(path, steps):
+    """Per-kernel statistics per RK4 step of a rocprofv3 kernel trace of a
+    bench.py leg: the dispatches after the last runtime copy kernel that
+    precedes the trailing ones (the state upload before the timed run) are the
+    timed steps, `steps` graph replays; set-up, autotuner trials and the
+    warm-up come before it.  The replays run a multi-stream graph, so the order of
+    dispatches differs between steps; the counts per step do not."""
+    import csv
+    with open(path) as f:
+        rows = sorted(csv.DictReader(f), key=lambda r: int(r["Dispatch_Id"]))
+    rt = [r["Kernel_Name"].startswith("__amd_rocclr") for r in rows]
+    end = len(rows)
+    while end > 0 and rt[end - 1]:
+        end -= 1
+    beg = end
+    while beg > 0 and not rt[beg - 1]:
+        beg -= 1
+    sel = rows[beg:end]
+    acc = {}
+    for r in sel:
+        acc.setdefault(r["Kernel_Name"], []).append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3)
+    total = sum(sum(v) for v in acc.values())
+    print(f"# {len(sel) // steps} dispatches per step over {steps} steps ({total * 1e-3:.2f} ms of kernel time, "
+          f"{total * 1e-3 / steps:.2f} ms per step)")
+    print("| kernel | calls per step | ms per step | avg us | min us | max us | % |")
+    print("|---|---|---|---|---|---|---|")
+    for n, v in sorted(acc.items(), key=lambda kv: -sum(kv[1])):
+        print(f"| `{n}` | {len(v) / steps:g} | {sum(v) * 1e-3 / steps:.3f} | {np.mean(v):.1f} | {min(v):.1f} | "
+              f"{max(v):.1f} | {100 * sum(v) / total:.1f} |")
+
+
 def pmc_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, copy_mib):
     """Per-kernel HBM bytes per launch from FETCH_SIZE / WRITE_SIZE (KiB per
     dispatch), each scaled so that the copy kernel's counter equals the bytes
@@ -538,45 +610,59 @@ def residual_errors(ctx, method, digits, sysm, o, v, F):
     return out
 
 
+ERROR_WINDOWS = (1, 10, 20, 40, 60, 100, 250, 500, 1000, 1500, 2000)
+
+
 def long_error(ctx, args, dt, w0, A, pol, precs, chunk=250):
-    """Propagate error_steps steps from X0 in FP64 and in split precision and
-    return the max |mu_z| difference (re and im).  Where the FP64 trajectory
-    leaves the finite numbers the comparison ends at its last finite step;
+    """Propagate error_steps steps from X0 in FP64 and in split precision (the
+    bench's digit count, or each of --error-digits) and return the max |mu_z|
+    difference (re and im), also over the first n steps for n in
+    ERROR_WINDOWS.  Where the FP64 trajectory leaves the finite numbers the
+    comparison ends at its last finite step (the split runs stop there too);
     `reference_stop_step` is the first step at which the reference's loop
     would stop on divergence (|Re mu_z| > 100, CCSD_Helper.py:1248), and the
     difference is also reported over the steps before it."""
-    traj = {}
-    for digits in (0, args.digits):
+    def run(digits, nmax):
         ctx.set_precision(digits)
         ctx.checkpoint(1)
-        t, mus = 0.0, []
-        done = 0
-        while done < args.error_steps:
-            n = min(chunk, args.error_steps - done)
+        t, mus, done = 0.0, [], 0
+        while done < nmax:
+            n = min(chunk, nmax - done)
             mu, ts = ctx.rk4_run(args.method, n, t, dt, w0, A, pol, precs)
             mus.append(mu)
             t = float(ts[-1])
             done += n
-            log(f"error run digits={digits}: {done}/{args.error_steps} steps")
+            log(f"error run digits={digits}: {done}/{nmax} steps")
             if not np.all(np.isfinite(mu[:, 2])):
                 break
-        traj[digits] = np.concatenate(mus)
-    ref = traj[0][:, 2]
+        return np.concatenate(mus)[:, 2]
+
+    ref = run(0, args.error_steps)
     fin = np.isfinite(ref)
     n_fin = int(np.argmin(fin)) if not fin.all() else len(ref)
     big = np.abs(ref.real) > 100
     n_stop = int(np.argmax(big)) + 1 if big.any() else None
-    n = min(n_fin, len(traj[args.digits]))
-    d = traj[args.digits][:n, 2] - ref[:n]
-    res = {"steps": args.error_steps, "compared_steps": n, "re": float(np.abs(d.real).max()),
-           "im": float(np.abs(d.imag).max()),
-           "mu_z_excursion_fp64": float(np.abs(ref[:n] - ref[0]).max()),
+    res = {"steps": args.error_steps, "compared_steps": n_fin,
+           "mu_z_excursion_fp64": float(np.abs(ref[:n_fin] - ref[0]).max()) if n_fin else 0.0,
            "fp64_first_nonfinite_step": None if fin.all() else n_fin + 1,
-           "reference_stop_step": n_stop}
-    if n_stop:
-        w = min(n_stop - 1, n)
-        res["before_reference_stop"] = ({"steps": w, "re": float(np.abs(d[:w].real).max()),
-                                         "im": float(np.abs(d[:w].imag).max())} if w > 0 else {"steps": 0})
+           "reference_stop_step": n_stop, "by_digits": {}}
+    digit_list = [int(x) for x in args.error_digits.split(",")] if args.error_digits else [args.digits]
+    for digits in digit_list:
+        m = run(digits, n_fin)
+        n = min(n_fin, len(m))
+        d = m[:n] - ref[:n]
+        r = {"compared_steps": n, "re": float(np.abs(d.real).max()), "im": float(np.abs(d.imag).max()),
+             "first_n_steps": {str(w): [float(np.abs(d[:w].real).max()), float(np.abs(d[:w].imag).max())]
+                               for w in ERROR_WINDOWS if w <= n}}
+        if n_stop:
+            w = min(n_stop - 1, n)
+            r["before_reference_stop"] = ({"steps": w, "re": float(np.abs(d[:w].real).max()),
+                                           "im": float(np.abs(d[:w].imag).max())} if w > 0 else {"steps": 0})
+        res["by_digits"][str(digits)] = r
+        if digits == args.digits:
+            res.update({k: r[k] for k in ("re", "im")})
+            if "before_reference_stop" in r:
+                res["before_reference_stop"] = r["before_reference_stop"]
     return res
 
 
