@@ -292,6 +292,15 @@ static double skinny_min_work() {
   return v < 0 ? 1e300 : v;
 }
 
+// Elements per block (run_rows x run) from which a k-major skinny contraction
+// with short runs takes the short-run kernels; TDCC_SKINNY_RUNS_MIN
+// overrides it (0: every such shape, a negative value: none).
+static int64_t skinny_runs_min() {
+  const char* e = std::getenv("TDCC_SKINNY_RUNS_MIN");
+  const int64_t v = e ? std::atoll(e) : 512;
+  return v < 0 ? INT64_MAX : v;
+}
+
 int Graph::contract(const std::string& spec, int a, int b, double ar, double ai) {
   std::vector<std::string> ins;
   std::string lc;
@@ -1397,9 +1406,12 @@ void Graph::lower_contract(int id) {
   }
   // summed letters in the memory order of an operand whose contiguous index
   // is summed; when both are, the per-step digit split of the larger one
-  // follows its own memory order (integral blocks are split once per bind)
+  // follows its own memory order (integral blocks are split once per bind),
+  // and a skinny contraction streams its big operand in its memory order
   const double a_cut = is_static(a) ? 0.0 : (double)gsize(gm), b_cut = is_static(b) ? 0.0 : (double)gsize(gn);
-  const bool k_by_b = bkc && (!akc || (rt_->split_digits > 0 && !n.fp64 && use_gemm && !use_dot && b_cut > a_cut));
+  const bool skinny_shape = !use_gemm && !use_dot && std::min(gsize(gm), gsize(gn)) <= kSkinnyMax;
+  const bool k_by_b = bkc && (!akc || (rt_->split_digits > 0 && !n.fp64 && use_gemm && !use_dot && b_cut > a_cut) ||
+                              (skinny_shape && gsize(gn) > gsize(gm)));
   gk = k_by_b ? by_stride(gk, sb) : by_stride(gk, sa);
 
   auto push = [&](const std::vector<int64_t>& t) {
@@ -1503,6 +1515,29 @@ void Graph::lower_contract(int id) {
     k.k_chunk = ch;
     k.alpha_re = n.ar * n.kfac;
     k.alpha_im = n.ai * n.kfac;
+    // k-major big operand whose summed index runs only a few elements: the
+    // short-run kernel pair when its blocks of run_rows x run elements are
+    // contiguous (checked on the offset tables)
+    if (k.g_kmajor && k.I <= 4) {
+      const int64_t* gr = host_tables_.data() + (small_a ? L->tab[2] : L->tab[0]);
+      const int64_t* gkt = host_tables_.data() + (small_a ? L->tab[3] : L->tab[1]);
+      int64_t R = 1;
+      while (R < K && gkt[R] == gkt[0] + R) ++R;
+      bool ok = R >= 2 && R <= 32 && K % R == 0 && K / R >= 2;
+      for (int64_t q = 0; ok && q < K; ++q) ok = gkt[q] == gkt[q - q % R] + q % R;
+      int64_t Jl = 1;
+      while (ok && Jl < k.J && gr[Jl] == gr[0] + Jl * R) ++Jl;
+      ok = ok && k.J % Jl == 0 && Jl * (R + 1) <= kSkinnyRunsMax && Jl * R >= skinny_runs_min() &&
+           k.J / Jl <= 65535;
+      for (int64_t j = 0; ok && j < k.J; ++j) ok = gr[j] == gr[j - j % Jl] + (j % Jl) * R;
+      if (ok) {
+        k.run = (int)R;
+        k.run_rows = (int)Jl;
+        k.splits = (int)(K / R);
+        k.k_chunk = R;
+        sp = K / R;
+      }
+    }
     n.scratch_len = sp > 1 ? sp * (L->ac || L->bc ? 2 : 1) * (int64_t)k.I * k.J * (int64_t)sizeof(double) : 0;
   }
   if (use_dot) {
@@ -1678,6 +1713,10 @@ void Graph::bind(double* arena) {
           k.c_im = n.cplx ? C.im : nullptr;
           k.partial = g.partial;
           if ((k.s_im || k.g_im) && !k.c_im) throw Error("internal: complex product into a real output");
+          if ((k.splits > 1 || k.run > 0) && !k.partial) throw Error("internal: skinny partial sums without scratch");
+          if (k.run > 0 && (k.splits != k.K / k.run || k.splits < 2 || k.run > 32 || k.I > 4 ||
+                            (int64_t)k.run_rows * (k.run + 1) > kSkinnyRunsMax || k.J % k.run_rows != 0))
+            throw Error("internal: inconsistent short-run skinny launch");
         }
         if (L->gp.dot) {
           const GemmPath& gp = L->gp;
@@ -2462,7 +2501,8 @@ void Graph::run_profiled(hipStream_t s, std::vector<LaunchRecord>& out) const {
             const double bytes = 8.0 * ((L->small_is_a ? (L->ac ? 2 : 1) : (L->bc ? 2 : 1)) * (double)k.I * k.K +
                                         (L->small_is_a ? (L->bc ? 2 : 1) : (L->ac ? 2 : 1)) * (double)k.J * k.K +
                                         pl * k.I * (double)k.J);
-            timed(k.g_kmajor ? "skinny_k_kernel" : "skinny_rows_kernel", fl, bytes, [&] { launch_skinny(k, s); });
+            timed(k.run > 0 ? "skinny_runs_kernel" : (k.g_kmajor ? "skinny_k_kernel" : "skinny_rows_kernel"), fl, bytes,
+                  [&] { launch_skinny(k, s); });
             break;
           }
           std::snprintf(dt, sizeof dt, "%s M=%d N=%d K=%d B=%d S=%d", nodes_[L->node].spec.c_str(), L->g.M, L->g.N,

@@ -372,7 +372,16 @@ struct SkinnyArgs {
   double* partial;
   double alpha_re, alpha_im;
   bool g_kmajor;
+  // Short runs (run > 0, big operand k-major): k = kh * run + r with
+  // gk[k] = gk[kh * run] + r, and big rows j = jh * run_rows + jl with
+  // gr[j] = gr[jh * run_rows] + jl * run, so each (jh, kh) block of the big
+  // operand is run_rows x run contiguous elements.  One block per (jh, kh)
+  // stages it through LDS and writes a partial per kh (splits = K / run,
+  // k_chunk = run); a second kernel sums the partials in kh order.
+  int run = 0, run_rows = 0;
 };
+// Largest run_rows * (run + 1) of the short-run kernel (its LDS block).
+constexpr int kSkinnyRunsMax = 3584;
 void launch_skinny(const SkinnyArgs& a, hipStream_t s);
 
 // ---------------------------------------------------------------------------

@@ -221,6 +221,89 @@ __global__ void skinny_reduce_kernel(SkinnyArgs a, int cplx) {
   }
 }
 
+// Short runs: the big operand's contiguous index is summed but runs only
+// `run` elements (e.g. 20 consecutive m of ieam summed over (e, m)), so a wave
+// over consecutive k touches a few separate short runs per load.  Instead one
+// block takes one (jh, kh) block of run_rows x run contiguous elements: loads
+// it coalesced into LDS (row jl at jl * (run + 1), the padding spreads the
+// rows' banks), and thread jl sums its row against the small operand's run in
+// r order.  Partial kh goes to partial[kh] in the layout skinny_reduce_kernel
+// reads; skinny_runs_reduce_kernel sums the partials in kh order.
+template <bool SC, bool GC>
+__global__ __launch_bounds__(256) void skinny_runs_kernel(SkinnyArgs a) {
+  constexpr bool CC = SC || GC;
+  __shared__ double gre[kSkinnyRunsMax], gim[GC ? kSkinnyRunsMax : 1];
+  __shared__ double sre[4][64], sim[SC ? 4 : 1][64];
+  const int R = a.run, Jl = a.run_rows, n = R * Jl;
+  const int64_t kh = blockIdx.x, jh = blockIdx.y;
+  const int64_t base = a.gr[jh * Jl] + a.gk[kh * R];
+  for (int e = threadIdx.x; e < n; e += blockDim.x) {
+    const int jl = e / R, r = e - jl * R;
+    gre[jl * (R + 1) + r] = a.g_re[base + e];
+    if (GC) gim[jl * (R + 1) + r] = a.g_im[base + e];
+  }
+  for (int e = threadIdx.x; e < a.I * R; e += blockDim.x) {
+    const int i = e / R, r = e - i * R;
+    const int64_t off = a.sr[i] + a.sk[kh * R + r];
+    sre[i][r] = a.s_re[off];
+    if (SC) sim[i][r] = a.s_im[off];
+  }
+  __syncthreads();
+  const int64_t total = (int64_t)a.I * a.J;
+  double* const p = a.partial + kh * (CC ? 2 : 1) * total;
+  for (int jl = threadIdx.x; jl < Jl; jl += blockDim.x) {
+    double acc_re[4] = {0.0, 0.0, 0.0, 0.0}, acc_im[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int r = 0; r < R; ++r) {
+      const double br = gre[jl * (R + 1) + r], bi = GC ? gim[jl * (R + 1) + r] : 0.0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if (i >= a.I) break;
+        const double s_r = sre[i][r], s_i = SC ? sim[i][r] : 0.0;
+        acc_re[i] = fma(s_r, br, acc_re[i]);
+        if (SC && GC) acc_re[i] = fma(-s_i, bi, acc_re[i]);
+        if (GC) acc_im[i] = fma(s_r, bi, acc_im[i]);
+        if (SC) acc_im[i] = fma(s_i, br, acc_im[i]);
+      }
+    }
+    const int64_t j = jh * Jl + jl;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (i >= a.I) break;
+      p[(int64_t)i * a.J + j] = acc_re[i];
+      if (CC) p[total + (int64_t)i * a.J + j] = acc_im[i];
+    }
+  }
+}
+
+// Sum of the short-run partials: 64 outputs per block, wave w adds partials
+// w, w + 4, ... in order, then the four wave sums are added in wave order.
+__global__ __launch_bounds__(256) void skinny_runs_reduce_kernel(SkinnyArgs a, int cplx) {
+  __shared__ double wr[4][64], wi[4][64];
+  const int64_t total = (int64_t)a.I * a.J;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t e = (int64_t)blockIdx.x * 64 + lane;
+  double vr = 0.0, vi = 0.0;
+  if (e < total) {
+#pragma unroll 8
+    for (int s = w; s < a.splits; s += 4) {
+      const double* p = a.partial + (int64_t)s * (cplx ? 2 : 1) * total;
+      vr += p[e];
+      if (cplx) vi += p[total + e];
+    }
+  }
+  wr[w][lane] = vr;
+  wi[w][lane] = vi;
+  __syncthreads();
+  if (w == 0 && e < total) {
+    vr = ((wr[0][lane] + wr[1][lane]) + wr[2][lane]) + wr[3][lane];
+    vi = ((wi[0][lane] + wi[1][lane]) + wi[2][lane]) + wi[3][lane];
+    const int64_t i = e / a.J, j = e - i * a.J;
+    const int64_t o = a.cs[i] + a.cb[j];
+    a.c_re[o] = a.alpha_re * vr - a.alpha_im * vi;
+    if (cplx) a.c_im[o] = a.alpha_re * vi + a.alpha_im * vr;
+  }
+}
+
 template <bool SC, bool GC, int IT>
 static void skinny_launch(const SkinnyArgs& a, hipStream_t s) {
   // small rows split over the block's 4 waves for 8 or more of them, except
@@ -245,6 +328,18 @@ static void skinny_dispatch(const SkinnyArgs& a, hipStream_t s) {
 void launch_skinny(const SkinnyArgs& a, hipStream_t s) {
   if (a.I <= 0 || a.J <= 0) return;
   const bool sc = a.s_im != nullptr, gc = a.g_im != nullptr;
+  if (a.run > 0) {
+    // K / run blocks along x, J / run_rows along y (checked when lowered)
+    const dim3 grid((unsigned)(a.K / a.run), (unsigned)(a.J / a.run_rows)), block(256);
+    if (sc && gc) hipLaunchKernelGGL((skinny_runs_kernel<true, true>), grid, block, 0, s, a);
+    else if (sc) hipLaunchKernelGGL((skinny_runs_kernel<true, false>), grid, block, 0, s, a);
+    else if (gc) hipLaunchKernelGGL((skinny_runs_kernel<false, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((skinny_runs_kernel<false, false>), grid, block, 0, s, a);
+    const int64_t total = (int64_t)a.I * a.J;
+    hipLaunchKernelGGL(skinny_runs_reduce_kernel, dim3((unsigned)((total + 63) / 64)), dim3(256), 0, s, a,
+                       (int)(sc || gc));
+    return;
+  }
   if (sc && gc) skinny_dispatch<true, true>(a, s);
   else if (sc) skinny_dispatch<true, false>(a, s);
   else if (gc) skinny_dispatch<false, true>(a, s);

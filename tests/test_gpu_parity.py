@@ -24,7 +24,9 @@ RHS = [("T1eq_rhs", _lib.RHS_T1), ("T2eq_rhs", _lib.RHS_T2), ("lam_1eq_rhs", _li
 
 
 # Engine modes: the default lowering; every skinny contraction (one side with
-# at most 32 rows) on the streaming VALU kernel; no skinny path (everything
+# at most 32 rows) on the streaming VALU kernel; the same with every k-major
+# one whose summed index runs a few contiguous elements on the short-run
+# kernels (by default only from 512 elements per block on); no skinny path (everything
 # small on the strided MFMA kernel); every batch-free contraction through the
 # dense-GEMM path, outputs antisymmetric in two column letters computed for
 # their ordered pairs at any summed length (by default only from K = 1024 on);
@@ -38,12 +40,14 @@ RHS = [("T1eq_rhs", _lib.RHS_T1), ("T2eq_rhs", _lib.RHS_T2), ("lam_1eq_rhs", _li
 # another is folded into it, and outer products of two matrices are formed
 # where a linear combination adds them).
 MODES = {"default": {}, "skinny": {"TDCC_SKINNY_MIN": "0"}, "gett": {"TDCC_SKINNY_MIN": "-1"},
+         "skinny_runs": {"TDCC_SKINNY_MIN": "0", "TDCC_SKINNY_RUNS_MIN": "0"},
          "gemm": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_NPAIR_MIN_K": "0"},
          "gemm_splitk": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_GEMM_SPLITS": "3"},
          "full_rows": {"TDCC_IJ_PAIRS": "0"}, "plain_lincomb": {"TDCC_LINCOMB_ANTI": "0"},
          "dot": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_DOT_MIN_K": "0"},
          "no_inline": {"TDCC_INLINE_LINCOMB": "0", "TDCC_FOLD_OUTER": "0"}}
-ENV_KEYS = ("TDCC_GEMM_MIN_FLOPS", "TDCC_GEMM_SPLITS", "TDCC_SKINNY_MIN", "TDCC_IJ_PAIRS", "TDCC_NPAIR_MIN_K",
+ENV_KEYS = ("TDCC_GEMM_MIN_FLOPS", "TDCC_GEMM_SPLITS", "TDCC_SKINNY_MIN", "TDCC_SKINNY_RUNS_MIN", "TDCC_IJ_PAIRS",
+            "TDCC_NPAIR_MIN_K",
             "TDCC_LINCOMB_ANTI", "TDCC_DOT_MIN_K", "TDCC_INLINE_LINCOMB", "TDCC_FOLD_OUTER")
 
 
