@@ -745,6 +745,36 @@ static int64_t ladder_scratch_bytes(int64_t R2, int v, int S) {
   return ladder_pmax_off(R2, v, S) + al256(ozaki_split_workspace(R2, ladder_Kp(v), false));
 }
 
+// A split's offset tables with a constant step (rows r * ldx + c, or k + c)
+// are replaced by that step, their constant folded into the source planes, so
+// the split kernels read rows without table lookups (and contiguous rows with
+// 16-byte loads).
+void Graph::simplify_split_tables(SplitArgs& a, int64_t rtab_off, int64_t ktab_off) const {
+  const int64_t* rt = host_tables_.data() + rtab_off;
+  const int64_t* kt = host_tables_.data() + ktab_off;
+  const int64_t R = a.Rb, K = a.K;
+  if (R < 1 || K < 1) return;
+  bool kc = true;
+  for (int64_t k = 0; kc && k < K; ++k) kc = kt[k] == kt[0] + k;
+  const int64_t step = R > 1 ? rt[1] - rt[0] : 0;
+  bool ra = true;
+  for (int64_t r = 0; ra && r < R; ++r) ra = rt[r] == rt[0] + r * step;
+  int64_t shift = 0;
+  if (kc) {
+    shift += kt[0];
+    a.ktab = nullptr;
+  }
+  if (ra) {
+    shift += rt[0];
+    a.rtab = nullptr;
+    a.ldx = step;
+  }
+  if (shift != 0) {
+    a.src_re += shift;
+    if (a.src_im) a.src_im += shift;
+  }
+}
+
 bool Graph::is_static(int id) const {
   const Node& n = nodes_[id];
   if (n.op == Op::View) return is_static(n.base);
@@ -1790,6 +1820,8 @@ void Graph::bind(double* arena) {
           sb.pmax = reinterpret_cast<double*>(scr + gp.pb_off);
           sb.multipass = split_multipass();
           sb.interleave = gp.fused;
+          simplify_split_tables(sa, L->tab[0], L->tab[1]);
+          simplify_split_tables(sb, L->tab[2], L->tab[3]);
           // static operands: digits live in the cache, split here once per bind
           // (the node's scratch serves as the split workspace)
           // shared digits of computed operands live in their arena block

@@ -298,6 +298,7 @@ class Graph {
   bool k_pairs_ = false;
   // digit planes of static operands (split precision), filled at bind
   bool is_static(int id) const;
+  void simplify_split_tables(SplitArgs& a, int64_t rtab_off, int64_t ktab_off) const;
   struct DigitCache {
     int64_t digits_off = 0, scale_off = 0;
     bool filled = false;

@@ -43,6 +43,23 @@ __device__ __forceinline__ int64_t split_koff(const SplitArgs& a, int64_t k) { r
 
 __device__ __forceinline__ double abs_or_inf(double v) { return std::isfinite(v) ? fabs(v) : INFINITY; }
 
+// x[k .. k+3] of a row (0 past K): two 16-byte loads when the row's k are
+// contiguous (no k table) and the address is 16-byte aligned, else one load
+// per element through the k table.
+__device__ __forceinline__ void load_row4(const SplitArgs& a, const double* base, int64_t k, double v[4]) {
+  if (!a.ktab && k + 3 < a.K && (reinterpret_cast<uintptr_t>(base + k) & 15) == 0) {
+    const double2 x = *reinterpret_cast<const double2*>(base + k);
+    const double2 y = *reinterpret_cast<const double2*>(base + k + 2);
+    v[0] = x.x;
+    v[1] = x.y;
+    v[2] = y.x;
+    v[3] = y.y;
+  } else {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = k + q < a.K ? base[split_koff(a, k + q)] : 0.0;
+  }
+}
+
 
 // next digit of 4 values, packed little-endian into 4 bytes
 __device__ __forceinline__ uint32_t next_digit4(double v[4]) {
@@ -70,9 +87,16 @@ __global__ __launch_bounds__(256) void split_max_rows_kernel(SplitArgs a) {
     double m = 0.0;
     if (r < R) {
       const double* base = split_row_base(a, r);
+      if (!a.ktab) {
+        double v[4];
+        load_row4(a, base, k0, v);
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (ko[j] >= 0) m = fmax(m, abs_or_inf(base[ko[j]]));
+        for (int j = 0; j < 4; ++j) m = fmax(m, abs_or_inf(v[j]));
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (ko[j] >= 0) m = fmax(m, abs_or_inf(base[ko[j]]));
+      }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
@@ -137,12 +161,9 @@ __global__ __launch_bounds__(256) void split_rows_wave_kernel(SplitArgs a) {
   const double* base = split_row_base(a, r);
   double v[4];
   double m = 0.0;
+  load_row4(a, base, 4 * lane, v);
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int64_t k = 4 * lane + q;
-    v[q] = k < a.K ? base[split_koff(a, k)] : 0.0;
-    m = fmax(m, abs_or_inf(v[q]));
-  }
+  for (int q = 0; q < 4; ++q) m = fmax(m, abs_or_inf(v[q]));
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
   const bool finite = std::isfinite(m);
@@ -175,8 +196,14 @@ __global__ __launch_bounds__(256) void split_digits_rows_kernel(SplitArgs a) {
     const int e = finite ? ilogb(sc) : 0;
     const double* base = split_row_base(a, r);
     double v[4];
+    if (!a.ktab) {
+      load_row4(a, base, k0, v);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) v[j] = (finite && ko[j] >= 0) ? ldexp(base[ko[j]], -e) : 0.0;
+      for (int j = 0; j < 4; ++j) v[j] = finite ? ldexp(v[j], -e) : 0.0;
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v[j] = (finite && ko[j] >= 0) ? ldexp(base[ko[j]], -e) : 0.0;
+    }
     int8_t* d = a.D + r * a.Kp + k0;
 #pragma unroll
     for (int s = 0; s < S; ++s) *reinterpret_cast<uint32_t*>(d + s * a.plane) = next_digit4(v);
@@ -226,13 +253,11 @@ __global__ __launch_bounds__(256) void split_rows_resident_kernel(SplitArgs a) {
   double v[NJ][4];
   double m = 0.0;
 #pragma unroll
-  for (int j = 0; j < NJ; ++j)
+  for (int j = 0; j < NJ; ++j) {
+    load_row4(a, base, 1024 * (int64_t)j + 4 * tid, v[j]);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int64_t k = 1024 * (int64_t)j + 4 * tid + q;
-      v[j][q] = k < a.K ? base[split_koff(a, k)] : 0.0;
-      m = fmax(m, abs_or_inf(v[j][q]));
-    }
+    for (int q = 0; q < 4; ++q) m = fmax(m, abs_or_inf(v[j][q]));
+  }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
   if (lane == 0) red[w] = m;
