@@ -71,11 +71,12 @@ int tdcc_synth_tei(tdcc_ctx* ctx, uint32_t config_id, double sigma);
  * (opt_einsum contract, CCSD_Helper.py:20) for those terms.
  *
  * Launch autotuning (environment): TDCC_AUTOTUNE=1 times candidate tile
- * orders, persistent grids and K splits (up to the planned count) for every
- * digit-GEMM launch shape when a program is bound, and keeps the fastest for
- * the process (on by default for batched programs); TDCC_AUTOTUNE_FILE
- * appends each choice to that file.  Tile order and persistence do not change
- * results; a different K split changes the order of the FP64 partial sums. */
+ * orders and persistent grids for every digit-GEMM launch shape when a
+ * program is bound, and keeps the fastest for the process (on by default for
+ * batched programs); these choices leave every result bit for bit unchanged.
+ * TDCC_AUTOTUNE=2 also tries fewer K splits (down to 2), which changes the
+ * order of the FP64 partial sums.  TDCC_AUTOTUNE_FILE appends each choice to
+ * that file. */
 int tdcc_set_precision(tdcc_ctx* ctx, int digits);
 
 /* Propagated state t1, t2, lam1, lam2 (complex, interleaved). */

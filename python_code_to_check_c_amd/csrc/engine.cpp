@@ -2241,8 +2241,12 @@ void Graph::bind(double* arena) {
     TDCC_HIP(hipGetLastError());
     TDCC_HIP(hipStreamSynchronize(rt_->stream));
   }
+  // 1: tile orders and persistent grids, which leave every output bit for bit
+  // unchanged; 2: also fewer K splits (they reorder the FP64 sum of the
+  // partials).  Default 1 for batched programs, off otherwise.
   const char* at = std::getenv("TDCC_AUTOTUNE");
-  if (at ? at[0] == '1' : nz_ > 1) autotune();
+  const int mode = at ? std::atoi(at) : (nz_ > 1 ? 1 : 0);
+  if (mode > 0) autotune(mode >= 2);
 }
 
 namespace {
@@ -2254,7 +2258,7 @@ std::mutex g_tune_mu;
 std::map<std::string, TuneChoice> g_tune;  // launch shape -> choice (this process)
 }  // namespace
 
-void Graph::autotune() {
+void Graph::autotune(bool k_splits) {
   hipStream_t s = rt_->stream;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   TDCC_HIP(hipEventCreate(&e0));
@@ -2274,8 +2278,8 @@ void Graph::autotune() {
       }
       if (!oz) continue;
       char key[160];
-      std::snprintf(key, sizeof key, "S=%d M=%d N=%d Kp=%d chunk=%d splits=%d epi=%d", oz->S, oz->M, oz->N, oz->Kp,
-                    oz->k_chunk, oz->splits, oz->epi);
+      std::snprintf(key, sizeof key, "S=%d M=%d N=%d Kp=%d chunk=%d splits=%d epi=%d%s", oz->S, oz->M, oz->N, oz->Kp,
+                    oz->k_chunk, oz->splits, oz->epi, k_splits ? " ksplits" : "");
       TuneChoice best;
       bool have = false;
       {
@@ -2289,7 +2293,7 @@ void Graph::autotune() {
       if (!have) {
         std::vector<TuneChoice> cand;
         std::vector<std::pair<int, int>> ks = {{oz->k_chunk, oz->splits}};
-        if (oz->splits > 2)  // fewer K chunks (the partial buffers hold the planned count)
+        if (k_splits && oz->splits > 2)  // fewer K chunks (the partial buffers hold the planned count)
           for (int sp = oz->splits / 2; sp >= 2; sp /= 2) {
             int64_t ch = (oz->Kp + sp - 1) / sp;
             ch = (ch + 63) / 64 * 64;

@@ -253,7 +253,7 @@ class Graph {
   // default for stacked programs, TDCC_AUTOTUNE=1 / 0 forces it on / off.
   // Choices are kept per launch shape for the process and appended to the
   // file TDCC_AUTOTUNE_FILE names.
-  void autotune();
+  void autotune(bool k_splits);
   void run(hipStream_t s) const;
 
   // One eager pass with a HIP event pair around every kernel launch.
