@@ -73,6 +73,7 @@ struct GemmPath {
   int S = 0;
   bool a_cols = false, b_cols = false;
   int64_t Kp = 0, a_plane = 0, b_plane = 0;
+  bool dil = false;  // digit-interleaved rows for both operands (short K)
   int64_t a8_off = 0, b8_off = 0, sa_off = 0, sb_off = 0, pa_off = 0, pb_off = 0, zb_off = 0;
   // operands that never change while the program is bound (integral blocks)
   // are split once per bind into the graph's digit cache: entry index or -1
@@ -1623,6 +1624,11 @@ void Graph::lower_contract(int id) {
       sp = (gp.Kp + chunk - 1) / chunk;
       gp.splits = (int)sp;
       gp.k_chunk = sp > 1 ? chunk : 0;
+      // short K (at most 8 K steps of 32): a digit row's K block keeps its S
+      // digits side by side, so a K step reads whole lines of each row (the
+      // planar layout reads a quarter line per row and step, and at short K
+      // the lines leave L2 before the next steps use the rest)
+      gp.dil = gp.Kp <= 256 && ozgemm_interleaved_ok(S);
       gp.a8_off = 0;
       gp.b8_off = S * gp.a_plane;
       gp.sa_off = gp.b8_off + S * gp.b_plane;
@@ -1639,7 +1645,7 @@ void Graph::lower_contract(int id) {
                             bool cols, int64_t nrows) {
         return nodes_[x].key + "|" + labels + "|" + rows + "|" + rpair + "|" + gk + "|" + n.kpair + "|" +
                (cols ? "c" : "r") + "|" + std::to_string(nrows) + "|" + std::to_string(S) + "|" +
-               (gp.fused ? "i" : "s");
+               (gp.fused ? "i" : "s") + (gp.dil ? "|d" : "");
       };
       auto cache_entry = [&](int x, const std::string& key, int64_t nrows, int64_t plane) {
         if (!is_static(x)) return -1;
@@ -1804,6 +1810,7 @@ void Graph::bind(double* arena) {
           sa.pmax = reinterpret_cast<double*>(scr + gp.pa_off);
           sa.multipass = split_multipass();
           sa.interleave = gp.fused;
+          sa.dil = gp.dil;
           SplitArgs& sb = L->sb;  // B' = [B_re; B_im] (pb*N) x K
           sb.src_re = B.re;
           sb.src_im = L->bc ? B.im : nullptr;
@@ -1820,6 +1827,7 @@ void Graph::bind(double* arena) {
           sb.pmax = reinterpret_cast<double*>(scr + gp.pb_off);
           sb.multipass = split_multipass();
           sb.interleave = gp.fused;
+          sb.dil = gp.dil;
           simplify_split_tables(sa, L->tab[0], L->tab[1]);
           simplify_split_tables(sb, L->tab[2], L->tab[3]);
           // static operands: digits live in the cache, split here once per bind
@@ -1849,6 +1857,7 @@ void Graph::bind(double* arena) {
           oz.B = sb.D;
           oz.a_plane = gp.a_plane;
           oz.b_plane = gp.b_plane;
+          oz.dil = gp.dil;
           oz.sa = sa.scale;
           oz.sb = sb.scale;
           oz.C = reinterpret_cast<double*>(scr + gp.zb_off);

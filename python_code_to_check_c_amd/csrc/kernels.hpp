@@ -310,6 +310,10 @@ struct SplitArgs {
   double* pmax;         // workspace: ozaki_split_workspace(R, Kp, by_columns) bytes
   bool multipass;       // always use the three-pass kernels (else single-pass when rows fit)
   bool interleave;      // with 2 planes: row r is plane r & 1 of base row r >> 1 (else plane r / Rb)
+  // Digit-interleaved rows: digit s of (r, k) at D + r Kp S + (k / 32) 32 S +
+  // 32 s + k % 32 (the S digits of a 32-wide K block of a row side by side),
+  // else at D + s plane + r Kp + k.
+  bool dil = false;
 };
 // Three launches: partial row maxima over k tiles, row scales, digits.
 // by_columns: threads run along r (for sources whose row index is the
@@ -344,8 +348,13 @@ struct OzGemmArgs {
   // tile, > 0 that many persistent blocks
   int tune_group = 0;
   int tune_pb = -1;
+  // both operands' digits in the digit-interleaved row layout (SplitArgs::dil);
+  // only with ozgemm_interleaved_ok(S)
+  bool dil = false;
 };
 void launch_ozgemm(const OzGemmArgs& a, hipStream_t s);
+// Whether the digit GEMM for S digits reads digit-interleaved operands.
+bool ozgemm_interleaved_ok(int S);
 
 // Name of the kernel launch_ozgemm runs for S digits (profiles).
 const char* ozgemm_kernel_name(int S);

@@ -41,6 +41,13 @@ __device__ __forceinline__ const double* split_row_base(const SplitArgs& a, int6
 
 __device__ __forceinline__ int64_t split_koff(const SplitArgs& a, int64_t k) { return a.ktab ? a.ktab[k] : k; }
 
+// Address of digit s of element (r, k) (k a multiple of 4; a group of 4 or
+// 16 digits starting at k stays inside one 32-wide K block).
+template <int S>
+__device__ __forceinline__ int8_t* digit_ptr(const SplitArgs& a, int64_t r, int64_t k, int s) {
+  return a.dil ? a.D + r * a.Kp * S + (k >> 5) * (32 * S) + 32 * s + (k & 31) : a.D + s * a.plane + r * a.Kp + k;
+}
+
 __device__ __forceinline__ double abs_or_inf(double v) { return std::isfinite(v) ? fabs(v) : INFINITY; }
 
 // x[k .. k+3] of a row (0 past K): two 16-byte loads when the row's k are
@@ -173,9 +180,8 @@ __global__ __launch_bounds__(256) void split_rows_wave_kernel(SplitArgs a) {
   if (k >= a.Kp) return;
 #pragma unroll
   for (int q = 0; q < 4; ++q) v[q] = finite ? ldexp(v[q], -e) : 0.0;
-  int8_t* d = a.D + r * a.Kp + k;
 #pragma unroll
-  for (int s = 0; s < S; ++s) *reinterpret_cast<uint32_t*>(d + s * a.plane) = next_digit4(v);
+  for (int s = 0; s < S; ++s) *reinterpret_cast<uint32_t*>(digit_ptr<S>(a, r, k, s)) = next_digit4(v);
 }
 
 template <int S>
@@ -204,9 +210,8 @@ __global__ __launch_bounds__(256) void split_digits_rows_kernel(SplitArgs a) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) v[j] = (finite && ko[j] >= 0) ? ldexp(base[ko[j]], -e) : 0.0;
     }
-    int8_t* d = a.D + r * a.Kp + k0;
 #pragma unroll
-    for (int s = 0; s < S; ++s) *reinterpret_cast<uint32_t*>(d + s * a.plane) = next_digit4(v);
+    for (int s = 0; s < S; ++s) *reinterpret_cast<uint32_t*>(digit_ptr<S>(a, r, k0, s)) = next_digit4(v);
   }
 }
 
@@ -233,10 +238,9 @@ __global__ __launch_bounds__(256) void split_digits_cols_kernel(SplitArgs a) {
 #pragma unroll
     for (int s = 0; s < S; ++s) out[q][s] = next_digit4(v);
   }
-  int8_t* d = a.D + r * a.Kp + k0;
 #pragma unroll
   for (int s = 0; s < S; ++s)
-    *reinterpret_cast<uint4*>(d + s * a.plane) = make_uint4(out[0][s], out[1][s], out[2][s], out[3][s]);
+    *reinterpret_cast<uint4*>(digit_ptr<S>(a, r, k0, s)) = make_uint4(out[0][s], out[1][s], out[2][s], out[3][s]);
 }
 
 // Single-pass splits for rows that fit in registers: the block reads its
@@ -266,7 +270,6 @@ __global__ __launch_bounds__(256) void split_rows_resident_kernel(SplitArgs a) {
   const bool finite = std::isfinite(m);
   const int e = finite ? (m > 0.0 ? ilogb(m) + 2 : 0) : 0;
   if (tid == 0) a.scale[r] = finite ? ldexp(1.0, e) : NAN;
-  int8_t* drow = a.D + r * a.Kp;
 #pragma unroll
   for (int j = 0; j < NJ; ++j) {
     const int64_t k = 1024 * (int64_t)j + 4 * tid;
@@ -274,7 +277,7 @@ __global__ __launch_bounds__(256) void split_rows_resident_kernel(SplitArgs a) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) v[j][q] = finite ? ldexp(v[j][q], -e) : 0.0;
 #pragma unroll
-    for (int s = 0; s < S; ++s) *reinterpret_cast<uint32_t*>(drow + s * a.plane + k) = next_digit4(v[j]);
+    for (int s = 0; s < S; ++s) *reinterpret_cast<uint32_t*>(digit_ptr<S>(a, r, k, s)) = next_digit4(v[j]);
   }
 }
 
@@ -303,7 +306,6 @@ __global__ __launch_bounds__(256) void split_cols_resident_kernel(SplitArgs a) {
   const bool finite = std::isfinite(m);
   const int e = finite ? (m > 0.0 ? ilogb(m) + 2 : 0) : 0;
   if (g == 0) a.scale[r] = finite ? ldexp(1.0, e) : NAN;
-  int8_t* drow = a.D + r * a.Kp;
 #pragma unroll
   for (int j = 0; j < NJ; ++j) {
     const int64_t k0 = 64 * (int64_t)j + 16 * g;
@@ -319,7 +321,7 @@ __global__ __launch_bounds__(256) void split_cols_resident_kernel(SplitArgs a) {
     }
 #pragma unroll
     for (int s = 0; s < S; ++s)
-      *reinterpret_cast<uint4*>(drow + s * a.plane + k0) = make_uint4(out[0][s], out[1][s], out[2][s], out[3][s]);
+      *reinterpret_cast<uint4*>(digit_ptr<S>(a, r, k0, s)) = make_uint4(out[0][s], out[1][s], out[2][s], out[3][s]);
   }
 }
 
@@ -943,7 +945,12 @@ __global__ __launch_bounds__(512, 1) void ozgemm8d_kernel(OzGemmArgs g) {
   const int pc = 64 * (wave & 3) + lane;                 // physical chunk this lane fills
   const int lrow = pc >> 1, lchunk = (pc & 1) ^ ((lrow >> 3) & 1);
   const int8_t* const xbase = on_b ? g.B : g.A;
-  const int64_t xplane = on_b ? g.b_plane : g.a_plane;
+  // planar digits: row stride Kp, K offset k, digit stride plane; digit-
+  // interleaved rows: row stride Kp S, K offset k S (k a multiple of 32),
+  // digit stride 32
+  const int64_t xplane = g.dil ? 32 : (on_b ? g.b_plane : g.a_plane);
+  const int64_t xrstride = g.dil ? (int64_t)g.Kp * S : g.Kp;
+  const int kmul = g.dil ? S : 1;
   int m0 = 0, n0 = 0;
   int64_t xrow = 0;
   auto setup = [&](int t) {
@@ -952,14 +959,14 @@ __global__ __launch_bounds__(512, 1) void ozgemm8d_kernel(OzGemmArgs g) {
     grouped_tile(L, tiles_m, tiles_n, g.tile_group, tm, tn);
     m0 = tm * OZ_BM;
     n0 = tn * BN;
-    xrow = on_b ? (int64_t)min(n0 + lrow, g.N - 1) * g.Kp : (int64_t)min(m0 + lrow, g.M - 1) * g.Kp;
+    xrow = on_b ? (int64_t)min(n0 + lrow, g.N - 1) * xrstride : (int64_t)min(m0 + lrow, g.M - 1) * xrstride;
   };
   // copies K step j of the current tile into buffer j % 3: one 16-byte chunk
   // per lane per digit plane, written by the copy engine to LDS at the wave's
   // base + 16 * lane
   auto issue = [&](int j) {
     const int buf = j % NBUF, k0 = kb + j * BK;
-    const int8_t* src = xbase + xrow + k0 + lchunk * 16;
+    const int8_t* src = xbase + xrow + (int64_t)k0 * kmul + lchunk * 16;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       v4i* dst = on_b ? &Bs[buf][s][64 * (wave & 3)] : &As[buf][s][64 * (wave & 3)];
@@ -1121,6 +1128,8 @@ bool ozgemm_direct() {
   }();
   return on;
 }
+
+bool ozgemm_interleaved_ok(int S) { return (S == 4 || S == 5) && ozgemm_direct(); }
 
 const char* ozgemm_kernel_name(int S) {
   if ((S == 4 || S == 5) && ozgemm_direct()) return "ozgemm8d_kernel";
