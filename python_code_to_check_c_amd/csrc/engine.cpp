@@ -74,6 +74,8 @@ struct GemmPath {
   bool a_cols = false, b_cols = false;
   int64_t Kp = 0, a_plane = 0, b_plane = 0;
   bool dil = false;  // digit-interleaved rows for both operands (short K)
+  // scales per K split of A' / B' (rows too long for a single-pass split)
+  bool a_chunked = false, b_chunked = false;
   int64_t a8_off = 0, b8_off = 0, sa_off = 0, sb_off = 0, pa_off = 0, pb_off = 0, zb_off = 0;
   // operands that never change while the program is bound (integral blocks)
   // are split once per bind into the graph's digit cache: entry index or -1
@@ -291,6 +293,13 @@ static double skinny_min_work() {
   const char* e = std::getenv("TDCC_SKINNY_MIN");
   const double v = e ? std::atof(e) : 65536.0;
   return v < 0 ? 1e300 : v;
+}
+
+// TDCC_SPLIT_CHUNKED=1: scales per K split for every row-oriented operand of a
+// split-K digit GEMM with K > 256 (by default only for rows over 16384).
+static bool split_chunked_always() {
+  const char* e = std::getenv("TDCC_SPLIT_CHUNKED");
+  return e && std::atoi(e) != 0;
 }
 
 // Elements per block (run_rows x run) from which a k-major skinny contraction
@@ -1629,11 +1638,20 @@ void Graph::lower_contract(int id) {
       // planar layout reads a quarter line per row and step, and at short K
       // the lines leave L2 before the next steps use the rest)
       gp.dil = gp.Kp <= 256 && ozgemm_interleaved_ok(S);
+      // a row-oriented operand whose rows are too long for the single-pass
+      // split (over 16384) gets one scale per row and K split, so that each
+      // chunk is split in one pass instead of three (max, scale, digits)
+      auto chunked = [&](bool cols) {
+        return sp > 1 && !cols && chunk <= 16384 && gp.Kp > 256 && (gp.Kp > 16384 || split_chunked_always());
+      };
+      gp.a_chunked = chunked(gp.a_cols);
+      gp.b_chunked = chunked(gp.b_cols);
+      const int64_t a_scales = R * (gp.a_chunked ? sp : 1), b_scales = NB * (gp.b_chunked ? sp : 1);
       gp.a8_off = 0;
       gp.b8_off = S * gp.a_plane;
       gp.sa_off = gp.b8_off + S * gp.b_plane;
-      gp.sb_off = gp.sa_off + al256(8 * R);
-      gp.pa_off = gp.sb_off + al256(8 * NB);
+      gp.sb_off = gp.sa_off + al256(8 * a_scales);
+      gp.pa_off = gp.sb_off + al256(8 * b_scales);
       gp.pb_off = gp.pa_off + al256(ozaki_split_workspace(R, gp.Kp, gp.a_cols));
       gp.zb_off = gp.pb_off + al256(ozaki_split_workspace(NB, gp.Kp, gp.b_cols));
       gp.fused = sp == 1;
@@ -1642,31 +1660,31 @@ void Graph::lower_contract(int id) {
       // row and summed letters in order, pair restrictions, orientation, rows
       // (planes included), digits, plane interleaving
       auto layout_key = [&](int x, const std::string& labels, const std::string& rows, const std::string& rpair,
-                            bool cols, int64_t nrows) {
+                            bool cols, int64_t nrows, bool chk) {
         return nodes_[x].key + "|" + labels + "|" + rows + "|" + rpair + "|" + gk + "|" + n.kpair + "|" +
                (cols ? "c" : "r") + "|" + std::to_string(nrows) + "|" + std::to_string(S) + "|" +
-               (gp.fused ? "i" : "s") + (gp.dil ? "|d" : "");
+               (gp.fused ? "i" : "s") + (gp.dil ? "|d" : "") + (chk ? "|k" + std::to_string(chunk) : "");
       };
-      auto cache_entry = [&](int x, const std::string& key, int64_t nrows, int64_t plane) {
+      auto cache_entry = [&](int x, const std::string& key, int64_t nscales, int64_t plane) {
         if (!is_static(x)) return -1;
         auto it = cache_index_.find(key);
         if (it != cache_index_.end()) return it->second;
         DigitCache e;
         e.digits_off = cache_bytes_;
         e.scale_off = cache_bytes_ + S * plane;
-        cache_bytes_ = e.scale_off + al256(8 * nrows);
+        cache_bytes_ = e.scale_off + al256(8 * nscales);
         cache_.push_back(e);
         const int idx = (int)cache_.size() - 1;
         cache_index_[key] = idx;
         return idx;
       };
-      const std::string akey = layout_key(a, la, gm, mpair, gp.a_cols, R);
-      const std::string bkey = layout_key(b, lb, gn, npair, gp.b_cols, NB);
-      gp.a_cache = cache_entry(a, akey, R, gp.a_plane);
-      gp.b_cache = cache_entry(b, bkey, NB, gp.b_plane);
-      if (gp.a_cache < 0 && !n.zloop) dyn_uses_[akey].push_back({id, 0, a, S * gp.a_plane + al256(8 * R)});
+      const std::string akey = layout_key(a, la, gm, mpair, gp.a_cols, R, gp.a_chunked);
+      const std::string bkey = layout_key(b, lb, gn, npair, gp.b_cols, NB, gp.b_chunked);
+      gp.a_cache = cache_entry(a, akey, a_scales, gp.a_plane);
+      gp.b_cache = cache_entry(b, bkey, b_scales, gp.b_plane);
+      if (gp.a_cache < 0 && !n.zloop) dyn_uses_[akey].push_back({id, 0, a, S * gp.a_plane + al256(8 * a_scales)});
       if (gp.b_cache < 0 && bkey != akey && !n.zloop)
-        dyn_uses_[bkey].push_back({id, 1, b, S * gp.b_plane + al256(8 * NB)});
+        dyn_uses_[bkey].push_back({id, 1, b, S * gp.b_plane + al256(8 * b_scales)});
       L->node = id;
       launches_.push_back(std::move(L));
       return;
@@ -1811,6 +1829,7 @@ void Graph::bind(double* arena) {
           sa.multipass = split_multipass();
           sa.interleave = gp.fused;
           sa.dil = gp.dil;
+          sa.sk_chunk = gp.a_chunked ? gp.k_chunk : 0;
           SplitArgs& sb = L->sb;  // B' = [B_re; B_im] (pb*N) x K
           sb.src_re = B.re;
           sb.src_im = L->bc ? B.im : nullptr;
@@ -1828,6 +1847,7 @@ void Graph::bind(double* arena) {
           sb.multipass = split_multipass();
           sb.interleave = gp.fused;
           sb.dil = gp.dil;
+          sb.sk_chunk = gp.b_chunked ? gp.k_chunk : 0;
           simplify_split_tables(sa, L->tab[0], L->tab[1]);
           simplify_split_tables(sb, L->tab[2], L->tab[3]);
           // static operands: digits live in the cache, split here once per bind
@@ -1860,6 +1880,8 @@ void Graph::bind(double* arena) {
           oz.dil = gp.dil;
           oz.sa = sa.scale;
           oz.sb = sb.scale;
+          oz.sa_split_stride = gp.a_chunked ? (int64_t)gp.pa * g.M : 0;
+          oz.sb_split_stride = gp.b_chunked ? (int64_t)gp.pb * g.N : 0;
           oz.C = reinterpret_cast<double*>(scr + gp.zb_off);
           oz.M = (int)(gp.pa * g.M);
           oz.N = (int)(gp.pb * g.N);
@@ -2302,7 +2324,9 @@ void Graph::autotune(bool k_splits) {
       if (!have) {
         std::vector<TuneChoice> cand;
         std::vector<std::pair<int, int>> ks = {{oz->k_chunk, oz->splits}};
-        if (k_splits && oz->splits > 2)  // fewer K chunks (the partial buffers hold the planned count)
+        // fewer K chunks (the partial buffers hold the planned count); not with
+        // scales per K split, which are cut at the planned chunks
+        if (k_splits && oz->splits > 2 && !oz->sa_split_stride && !oz->sb_split_stride)
           for (int sp = oz->splits / 2; sp >= 2; sp /= 2) {
             int64_t ch = (oz->Kp + sp - 1) / sp;
             ch = (ch + 63) / 64 * 64;

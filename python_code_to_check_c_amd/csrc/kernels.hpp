@@ -314,6 +314,9 @@ struct SplitArgs {
   // 32 s + k % 32 (the S digits of a 32-wide K block of a row side by side),
   // else at D + s plane + r Kp + k.
   bool dil = false;
+  // K chunks of their own scale (> 0, rows orientation, single pass): row r's
+  // k in [c sk_chunk, (c+1) sk_chunk) gets scale[c R + r]; 0: one scale per row
+  int64_t sk_chunk = 0;
 };
 // Three launches: partial row maxima over k tiles, row scales, digits.
 // by_columns: threads run along r (for sources whose row index is the
@@ -351,6 +354,9 @@ struct OzGemmArgs {
   // both operands' digits in the digit-interleaved row layout (SplitArgs::dil);
   // only with ozgemm_interleaved_ok(S)
   bool dil = false;
+  // scales per K split (SplitArgs::sk_chunk == k_chunk): split y reads
+  // sa + y * sa_split_stride (sb likewise); 0: one scale per row for all splits
+  int64_t sa_split_stride = 0, sb_split_stride = 0;
 };
 void launch_ozgemm(const OzGemmArgs& a, hipStream_t s);
 // Whether the digit GEMM for S digits reads digit-interleaved operands.

@@ -53,8 +53,10 @@ __device__ __forceinline__ double abs_or_inf(double v) { return std::isfinite(v)
 // x[k .. k+3] of a row (0 past K): two 16-byte loads when the row's k are
 // contiguous (no k table) and the address is 16-byte aligned, else one load
 // per element through the k table.
-__device__ __forceinline__ void load_row4(const SplitArgs& a, const double* base, int64_t k, double v[4]) {
-  if (!a.ktab && k + 3 < a.K && (reinterpret_cast<uintptr_t>(base + k) & 15) == 0) {
+__device__ __forceinline__ void load_row4(const SplitArgs& a, const double* base, int64_t k, double v[4],
+                                          int64_t kend = -1) {
+  if (kend < 0) kend = a.K;
+  if (!a.ktab && k + 3 < kend && (reinterpret_cast<uintptr_t>(base + k) & 15) == 0) {
     const double2 x = *reinterpret_cast<const double2*>(base + k);
     const double2 y = *reinterpret_cast<const double2*>(base + k + 2);
     v[0] = x.x;
@@ -63,7 +65,7 @@ __device__ __forceinline__ void load_row4(const SplitArgs& a, const double* base
     v[3] = y.y;
   } else {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) v[q] = k + q < a.K ? base[split_koff(a, k + q)] : 0.0;
+    for (int q = 0; q < 4; ++q) v[q] = k + q < kend ? base[split_koff(a, k + q)] : 0.0;
   }
 }
 
@@ -253,12 +255,16 @@ __global__ __launch_bounds__(256) void split_rows_resident_kernel(SplitArgs a) {
   __shared__ double red[4];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int64_t r = blockIdx.x;
+  // K chunk blockIdx.y of its own scale (sk_chunk > 0), else the whole row
+  const int64_t klo = (int64_t)blockIdx.y * a.sk_chunk;
+  const int64_t khi = a.sk_chunk > 0 ? min(a.K, klo + a.sk_chunk) : a.K;
+  const int64_t kphi = a.sk_chunk > 0 ? min(a.Kp, klo + a.sk_chunk) : a.Kp;
   const double* base = split_row_base(a, r);
   double v[NJ][4];
   double m = 0.0;
 #pragma unroll
   for (int j = 0; j < NJ; ++j) {
-    load_row4(a, base, 1024 * (int64_t)j + 4 * tid, v[j]);
+    load_row4(a, base, klo + 1024 * (int64_t)j + 4 * tid, v[j], khi);
 #pragma unroll
     for (int q = 0; q < 4; ++q) m = fmax(m, abs_or_inf(v[j][q]));
   }
@@ -269,11 +275,11 @@ __global__ __launch_bounds__(256) void split_rows_resident_kernel(SplitArgs a) {
   m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
   const bool finite = std::isfinite(m);
   const int e = finite ? (m > 0.0 ? ilogb(m) + 2 : 0) : 0;
-  if (tid == 0) a.scale[r] = finite ? ldexp(1.0, e) : NAN;
+  if (tid == 0) a.scale[(int64_t)blockIdx.y * a.planes * a.Rb + r] = finite ? ldexp(1.0, e) : NAN;
 #pragma unroll
   for (int j = 0; j < NJ; ++j) {
-    const int64_t k = 1024 * (int64_t)j + 4 * tid;
-    if (k >= a.Kp) break;
+    const int64_t k = klo + 1024 * (int64_t)j + 4 * tid;
+    if (k >= kphi) break;
 #pragma unroll
     for (int q = 0; q < 4; ++q) v[j][q] = finite ? ldexp(v[j][q], -e) : 0.0;
 #pragma unroll
@@ -339,16 +345,19 @@ static void split_launch(const SplitArgs& a, bool by_columns, hipStream_t s) {
   // serial loads dominate, while the whole operand is small enough that
   // uncoalesced loads cost little.
   constexpr int64_t kWaveColsMaxRows = 2048;
-  if (!a.multipass && a.Kp <= 256 && (!by_columns || R <= kWaveColsMaxRows)) {
+  if (a.sk_chunk == 0 && !a.multipass && a.Kp <= 256 && (!by_columns || R <= kWaveColsMaxRows)) {
     hipLaunchKernelGGL(split_rows_wave_kernel<S>, dim3((unsigned)((R + 3) / 4)), block, 0, s, a);
     return;
   }
-  if (!a.multipass && !by_columns && a.Kp <= 16384) {
-    const dim3 grid((unsigned)R);
-    if (a.Kp <= 1024) hipLaunchKernelGGL((split_rows_resident_kernel<S, 1>), grid, block, 0, s, a);
-    else if (a.Kp <= 2048) hipLaunchKernelGGL((split_rows_resident_kernel<S, 2>), grid, block, 0, s, a);
-    else if (a.Kp <= 4096) hipLaunchKernelGGL((split_rows_resident_kernel<S, 4>), grid, block, 0, s, a);
-    else if (a.Kp <= 8192) hipLaunchKernelGGL((split_rows_resident_kernel<S, 8>), grid, block, 0, s, a);
+  // rows in one pass: whole rows up to 16384, or K chunks with their own
+  // scales (always single pass; the chunk is at most 16384)
+  if (a.sk_chunk > 0 || (!a.multipass && !by_columns && a.Kp <= 16384)) {
+    const int64_t len = a.sk_chunk > 0 ? a.sk_chunk : a.Kp;
+    const dim3 grid((unsigned)R, (unsigned)(a.sk_chunk > 0 ? (a.Kp + a.sk_chunk - 1) / a.sk_chunk : 1));
+    if (len <= 1024) hipLaunchKernelGGL((split_rows_resident_kernel<S, 1>), grid, block, 0, s, a);
+    else if (len <= 2048) hipLaunchKernelGGL((split_rows_resident_kernel<S, 2>), grid, block, 0, s, a);
+    else if (len <= 4096) hipLaunchKernelGGL((split_rows_resident_kernel<S, 4>), grid, block, 0, s, a);
+    else if (len <= 8192) hipLaunchKernelGGL((split_rows_resident_kernel<S, 8>), grid, block, 0, s, a);
     else hipLaunchKernelGGL((split_rows_resident_kernel<S, 16>), grid, block, 0, s, a);
     return;
   }
@@ -461,8 +470,8 @@ __global__ __launch_bounds__(256) void ozgemm_kernel(OzGemmArgs g) {
   const int lr = lane & 15, lk = lane >> 4;
 
   load_global(kb);
-  if (tid < OZ_BM) sA[tid] = m0 + tid < g.M ? g.sa[m0 + tid] : 0.0;
-  else if (tid - OZ_BM < BN) sB[tid - OZ_BM] = n0 + tid - OZ_BM < g.N ? g.sb[n0 + tid - OZ_BM] : 0.0;
+  if (tid < OZ_BM) sA[tid] = m0 + tid < g.M ? g.sa[(int64_t)blockIdx.y * g.sa_split_stride + m0 + tid] : 0.0;
+  else if (tid - OZ_BM < BN) sB[tid - OZ_BM] = n0 + tid - OZ_BM < g.N ? g.sb[(int64_t)blockIdx.y * g.sb_split_stride + n0 + tid - OZ_BM] : 0.0;
   store_lds(0);
   __syncthreads();
   int buf = 0;
@@ -641,8 +650,8 @@ __global__ __launch_bounds__(512, 1) void ozgemm8_kernel(OzGemmArgs g) {
       for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int e = 0; e < 16; ++e) acc[q][i][e] = 0;
-    if (tid < OZ_BM) sA[tid] = m0 + tid < g.M ? g.sa[m0 + tid] : 0.0;
-    else if (tid - OZ_BM < BN) sB[tid - OZ_BM] = n0 + tid - OZ_BM < g.N ? g.sb[n0 + tid - OZ_BM] : 0.0;
+    if (tid < OZ_BM) sA[tid] = m0 + tid < g.M ? g.sa[(int64_t)blockIdx.y * g.sa_split_stride + m0 + tid] : 0.0;
+    else if (tid - OZ_BM < BN) sB[tid - OZ_BM] = n0 + tid - OZ_BM < g.N ? g.sb[(int64_t)blockIdx.y * g.sb_split_stride + n0 + tid - OZ_BM] : 0.0;
     store_lds(0);
     __syncthreads();
     int buf = 0;
@@ -810,8 +819,8 @@ __global__ __launch_bounds__(512, 1) void ozgemm8n_kernel(OzGemmArgs g) {
       for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int e = 0; e < 16; ++e) acc[q][i][e] = 0;
-    if (tid < OZ_BM) sA[tid] = m0 + tid < g.M ? g.sa[m0 + tid] : 0.0;
-    else if (tid - OZ_BM < BN) sB[tid - OZ_BM] = n0 + tid - OZ_BM < g.N ? g.sb[n0 + tid - OZ_BM] : 0.0;
+    if (tid < OZ_BM) sA[tid] = m0 + tid < g.M ? g.sa[(int64_t)blockIdx.y * g.sa_split_stride + m0 + tid] : 0.0;
+    else if (tid - OZ_BM < BN) sB[tid - OZ_BM] = n0 + tid - OZ_BM < g.N ? g.sb[(int64_t)blockIdx.y * g.sb_split_stride + n0 + tid - OZ_BM] : 0.0;
     store_lds(0);
     __syncthreads();
     int buf = 0;
@@ -982,8 +991,8 @@ __global__ __launch_bounds__(512, 1) void ozgemm8d_kernel(OzGemmArgs g) {
   for (;;) {
     // the tile's scales first (their loads complete before the copies start),
     // then the copies of its first two K steps
-    if (tid < OZ_BM) sA[tid] = m0 + tid < g.M ? g.sa[m0 + tid] : 0.0;
-    else if (tid - OZ_BM < BN) sB[tid - OZ_BM] = n0 + tid - OZ_BM < g.N ? g.sb[n0 + tid - OZ_BM] : 0.0;
+    if (tid < OZ_BM) sA[tid] = m0 + tid < g.M ? g.sa[(int64_t)blockIdx.y * g.sa_split_stride + m0 + tid] : 0.0;
+    else if (tid - OZ_BM < BN) sB[tid - OZ_BM] = n0 + tid - OZ_BM < g.N ? g.sb[(int64_t)blockIdx.y * g.sb_split_stride + n0 + tid - OZ_BM] : 0.0;
     if (nk > 0) issue(0);
     if (nk > 1) issue(1);
     v16i acc[S][2];

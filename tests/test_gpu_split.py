@@ -39,10 +39,14 @@ def bound(S):
 # at every summed length (at large sizes only from K = 1024 on).  "dot" sends
 # the dot-product shapes (at most 64 A rows, 512 B rows) to the FP64 dot
 # kernel at every summed length (at large sizes only from K = 8192 on).
+# "chunked" cuts every dense-GEMM contraction into 3 K splits and gives each
+# row-oriented operand with K > 256 one scale per row and split, split in one
+# pass per chunk (at large sizes only for rows longer than 16384).
 MODES = {"default": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_NPAIR_MIN_K": "0"},
          "multipass": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_SPLIT_MULTIPASS": "1"},
          "stream": {"TDCC_STATIC_STREAM_MIN": "0"},
-         "dot": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_DOT_MIN_K": "0"}}
+         "dot": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_DOT_MIN_K": "0"},
+         "chunked": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_GEMM_SPLITS": "3", "TDCC_SPLIT_CHUNKED": "1"}}
 
 
 @pytest.fixture(scope="module", params=[(n, m) for m in MODES for n in ("o4v6", "c1_o10v4")],
@@ -52,7 +56,7 @@ def case(request):
     g = dict(np.load(os.path.join(HERE, "golden", f"golden_{name}.npz")))
     print(f"\n[split case {name} {mode}]", flush=True)
     keys = ("TDCC_GEMM_MIN_FLOPS", "TDCC_SPLIT_MULTIPASS", "TDCC_STATIC_STREAM_MIN", "TDCC_NPAIR_MIN_K",
-            "TDCC_DOT_MIN_K")
+            "TDCC_DOT_MIN_K", "TDCC_GEMM_SPLITS", "TDCC_SPLIT_CHUNKED")
     saved = {k: os.environ.get(k) for k in keys}
     for k in keys:
         os.environ.pop(k, None)
