@@ -17,7 +17,9 @@ v^2 = 25,600 terms and the residuals at this state are small differences of
 much larger terms (T2eq_rhs_CC2 at S = 4: 1.5e-7 of its largest element, 1.8e-8
 absolute); and one more digit must shrink every residual's error by at least
 32 (the truncation is 128^-S per operand element), which a dropped or misplaced
-digit product would not do.
+digit product would not do.  (The digits are radix 254, so the truncation is
+0.5 * 254^-S of the row scale and these bars, set for radix 128, hold with
+room to spare.)
 """
 import os
 

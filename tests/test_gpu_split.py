@@ -4,10 +4,11 @@ Every batch-free contraction is forced through the dense-GEMM path
 (TDCC_GEMM_MIN_FLOPS=0), so at these small shapes all rings, o^2 v^3 terms and
 the ladders run on the digit GEMM.  The reference's outputs (golden vectors)
 are the yardstick:
-  * each operand element is truncated to 0.5 * 128^-S of its row maximum, so
-    residuals, the dipole and RK4 trajectories must agree with the reference to
-    a relative error of at most 30 * 128^-S (the constant covers the sums of
-    up to a few hundred terms at these shapes; measured ratios are 2-8);
+  * each operand element is truncated to 0.5 * 254^-S of its row scale
+    (radix-254 digits), so residuals, the dipole and RK4 trajectories must
+    agree with the reference to a relative error of at most 30 * 128^-S (a
+    bar set for radix-128 digits, looser by (254/128)^S than radix 254 needs;
+    the constant covers the sums of up to a few hundred terms at these shapes);
   * int32 accumulation is exact and the FP64 assembly has a fixed order, so a
     repeated evaluation is bitwise identical;
   * switching back to FP64 gives the FP64 results (1e-11).
