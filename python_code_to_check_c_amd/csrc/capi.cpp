@@ -736,6 +736,8 @@ Program* prepare_run(tdcc_ctx* c, int method, int nsteps, double t0, double dt, 
   std::vector<double> tab;
   field_table(nsteps, t0, dt, w0, tab, times);
   TDCC_HIP(hipMemcpyAsync(c->table, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  // the pageable source must outlive the copy
+  TDCC_HIP(hipStreamSynchronize(c->stream));
   c->rt.negA = -A;
   c->rt.pol = pol;
   c->rt.dt6 = dt / 6.0;
@@ -1027,17 +1029,21 @@ int tdcc_rk4_run_batch(tdcc_ctx* c, int method, int nsteps, double t0, double dt
       TDCC_HIP(hipMalloc(&c->bmu, (size_t)c->nb * cap * 6 * sizeof(double)));
       c->bcap = cap;
     }
-    std::vector<double> tab, times;
+    // one host table per trajectory, all alive until the copies are done
+    std::vector<std::vector<double>> tabs(c->nb);
+    std::vector<double> times;
     for (int b = 0; b < c->nb; ++b) {
-      field_table(nsteps, t0, dt, w0[b], tab, times);
+      field_table(nsteps, t0, dt, w0[b], tabs[b], times);
       double* dst = c->btable + (size_t)b * c->bcap * 6;
-      TDCC_HIP(hipMemcpyAsync(dst, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      TDCC_HIP(hipMemcpyAsync(dst, tabs[b].data(), tabs[b].size() * sizeof(double), hipMemcpyHostToDevice,
+                              c->stream));
       Runtime::Traj& tr = c->rt.batch[b];
       tr.pol = pol[b];
       tr.negA = -A[b];
       tr.field_table = dst;
       tr.mu_out = c->bmu + (size_t)b * c->bcap * 6;
     }
+    TDCC_HIP(hipStreamSynchronize(c->stream));
     c->rt.dt6 = dt / 6.0;
     double scale = 1.0;
     for (int k = 0; k < precs; ++k) scale *= 10.0;
