@@ -311,6 +311,29 @@ static int64_t skinny_runs_min() {
   return v < 0 ? INT64_MAX : v;
 }
 
+// Digits a digit-GEMM launch multiplies with (at most the split's S):
+// TDCC_DIGITS_LOW lists launches (einsum specs, "ladder", "all") that use
+// TDCC_DIGITS_LOW_S digits (default S - 1) of their operands' S-digit planes.
+static int launch_digits(const std::string& spec, bool ladder, int S) {
+  const char* e = std::getenv("TDCC_DIGITS_LOW");
+  if (!e || !*e || S <= 2) return S;
+  bool hit = false;
+  std::string cur;
+  for (const char* c = e;; ++c) {
+    if (*c == ';' || *c == 0) {
+      hit = hit || cur == "all" || (ladder ? cur == "ladder" : cur == spec);
+      cur.clear();
+      if (*c == 0) break;
+    } else {
+      cur.push_back(*c);
+    }
+  }
+  if (!hit) return S;
+  const char* es = std::getenv("TDCC_DIGITS_LOW_S");
+  const int s = es ? std::atoi(es) : S - 1;
+  return std::max(2, std::min(S, s));
+}
+
 int Graph::contract(const std::string& spec, int a, int b, double ar, double ai) {
   std::vector<std::string> ins;
   std::string lc;
@@ -1424,8 +1447,14 @@ void Graph::lower_contract(int id) {
     std::stable_sort(g.begin(), g.end(), [&](char x, char y) { return st.at(x) > st.at(y); });
     return g;
   };
-  gm = akc ? gm : by_stride(gm, sa);
-  gn = bkc ? gn : by_stride(gn, sb);
+  // Output-bound strided contractions (a short summed group, an output much
+  // larger than either operand) keep the output's letter order on both free
+  // sides, so that their stores run along the output's memory; otherwise a
+  // side not contiguous along K follows its operand's memory order.
+  const bool out_bound = !use_gemm && gb.empty() && gsize(gk) <= 32 &&
+                         (double)gsize(gm) * gsize(gn) >= 4.0 * (double)gsize(gk) * (gsize(gm) + gsize(gn));
+  gm = (akc || out_bound) ? gm : by_stride(gm, sa);
+  gn = (bkc || out_bound) ? gn : by_stride(gn, sb);
   // On the GEMM path an output antisymmetric in two letters of the M (or N)
   // group is computed for their ordered combinations (diagonal included) and
   // the epilogue stores each element's mirror images as well.
@@ -1887,7 +1916,9 @@ void Graph::bind(double* arena) {
           oz.N = (int)(gp.pb * g.N);
           oz.ldc = oz.N;
           oz.Kp = (int)gp.Kp;
-          oz.S = gp.S;
+          oz.S = launch_digits(n.spec, false, gp.S);
+          if (gp.dil && !ozgemm_interleaved_ok(oz.S)) oz.S = gp.S;
+          oz.Sl = gp.S;
           oz.k_chunk = (int)gp.k_chunk;
           oz.splits = gp.splits;
           oz.c_split_stride = (int64_t)oz.M * oz.N;
@@ -2109,7 +2140,8 @@ void Graph::bind(double* arena) {
           oz.M = (int)R2;
           oz.N = (int)a.P;
           oz.Kp = (int)Kp;
-          oz.S = a.S;
+          oz.S = launch_digits("", true, a.S);
+          oz.Sl = a.S;
           const int64_t chunk = ladder_k_chunk(R2, v_, a.S);
           if (chunk == 0 && Kp > ozgemm_max_k(a.S)) throw Error("ladder K beyond the exact int32 range of the digit GEMM");
           oz.k_chunk = (int)chunk;

@@ -330,7 +330,9 @@ struct OzGemmArgs {
   const double* sa; const double* sb; // row scales of A (M) and B (N)
   double* C;                          // C[m * ldc + n] = sum_k A[m][k] * B[n][k]
   int64_t ldc;
-  int M, N, Kp, S;
+  int M, N, Kp, S;                    // S: digits multiplied (at most the planes' Sl)
+  int Sl = 0;                         // digits per element in the planes (0: S); the
+                                      // digit-interleaved layout's strides follow it
   int k_chunk, splits;                // split-K over grid.y (k_chunk multiple of 64), partials
   int64_t c_split_stride;             // c_split_stride apart
   // Fused scatter epilogue (epi >= 0, one split only): instead of C, write

@@ -958,8 +958,9 @@ __global__ __launch_bounds__(512, 1) void ozgemm8d_kernel(OzGemmArgs g) {
   // interleaved rows: row stride Kp S, K offset k S (k a multiple of 32),
   // digit stride 32
   const int64_t xplane = g.dil ? 32 : (on_b ? g.b_plane : g.a_plane);
-  const int64_t xrstride = g.dil ? (int64_t)g.Kp * S : g.Kp;
-  const int kmul = g.dil ? S : 1;
+  const int sl = g.Sl > 0 ? g.Sl : S;
+  const int64_t xrstride = g.dil ? (int64_t)g.Kp * sl : g.Kp;
+  const int kmul = g.dil ? sl : 1;
   int m0 = 0, n0 = 0;
   int64_t xrow = 0;
   auto setup = [&](int t) {
