@@ -311,6 +311,40 @@ static int64_t skinny_runs_min() {
   return v < 0 ? INT64_MAX : v;
 }
 
+// K splits of a digit GEMM whose tile grid fills few rounds of the CUs: the
+// split count (1 .. 64, chunks a multiple of 64) that minimizes a cost model
+// in units of one tile's 32-wide K step: rounds x (K steps per split + 3 for
+// a tile's prologue and epilogue), plus, with splits, writing and summing the
+// partial products (about one K step per 1.1 us at 4 TB/s).  The tail of a
+// grid just over a whole number of rounds is what it avoids (e.g. 175 tiles
+// x 3 splits = 2.05 rounds).  TDCC_SPLIT_MODEL=0 keeps the fixed rule.
+constexpr int64_t kModelCUs = 256;
+static bool split_model_on() {
+  const char* e = std::getenv("TDCC_SPLIT_MODEL");
+  return !e || std::atoi(e) != 0;
+}
+static int64_t model_splits(int64_t tiles, int64_t Kp, int64_t R, int64_t NB, int64_t planned) {
+  auto cost = [&](int64_t s) {
+    int64_t chunk = (Kp + s - 1) / s;
+    chunk = (chunk + 63) / 64 * 64;
+    const int64_t sp = (Kp + chunk - 1) / chunk;
+    const double rounds = (double)((tiles * sp + kModelCUs - 1) / kModelCUs);
+    double t = rounds * ((double)chunk / 32.0 + 3.0);
+    if (sp > 1) t += (double)(sp + 1) * R * NB * 8.0 / 4e12 / 1.1e-6;
+    return t;
+  };
+  int64_t best = planned;
+  double bc = cost(planned);
+  for (int64_t s = 1; s <= 64 && s * 64 <= Kp; ++s) {
+    const double c = cost(s);
+    if (c < bc * 0.97) {  // a clear gain only
+      bc = c;
+      best = s;
+    }
+  }
+  return best;
+}
+
 // Digits a digit-GEMM launch multiplies with (at most the split's S):
 // TDCC_DIGITS_LOW lists launches (einsum specs, "ladder", "all") that use
 // TDCC_DIGITS_LOW_S digits (default S - 1) of their operands' S-digit planes.
@@ -1654,6 +1688,7 @@ void Graph::lower_contract(int id) {
       const int64_t tiles = ((R + 127) / 128) * ((NB + bn - 1) / bn);
       int64_t sp = 1;
       if (tiles < 240 && K >= 2048) sp = std::min<int64_t>((K + 1023) / 1024, (480 + tiles - 1) / tiles);
+      if (split_model_on() && tiles < 4 * kModelCUs && K >= 2048) sp = model_splits(tiles, gp.Kp, R, NB, sp);
       if (const char* fs = std::getenv("TDCC_GEMM_SPLITS")) sp = std::max<int64_t>(1, std::min<int64_t>(std::atol(fs), gp.Kp / 64));
       const int64_t maxk = ozgemm_max_k(S);
       sp = std::max<int64_t>(sp, (gp.Kp + maxk - 1) / maxk);
