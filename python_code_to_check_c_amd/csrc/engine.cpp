@@ -86,6 +86,12 @@ struct GemmPath {
   bool a_split = true, b_split = true;
   // one split: the digit GEMM scatters into C itself (planes interleaved by row)
   bool fused = false;
+  // complex x complex from three real products (3M): digit rows [re; im;
+  // re + im] of each operand (3 M and 3 N rows, planes not interleaved); one
+  // launch computes P1 = A_re B_re^T and P2 = A_im B_im^T into the node
+  // scratch at p12_off, a second P3 = A_s B_s^T with the combining epilogue
+  bool m3 = false;
+  int64_t p12_off = 0;
   // dot-product kernel (FP64) instead of the GEMM: partials in the node scratch
   bool dot = false;
 };
@@ -98,7 +104,7 @@ struct Graph::Launch {
   Gather2dArgs ga{}, gb{};
   DgemmArgs dg{};
   SplitArgs sa{}, sb{};
-  OzGemmArgs oz{};
+  OzGemmArgs oz{}, oz2{};  // oz2: the 3M path's second launch
   GemmScatterArgs sc{};
   DotArgs dt{};
   // skinny path: one operand with at most kSkinnyMax rows (small_is_a: the A side)
@@ -343,6 +349,14 @@ static int64_t model_splits(int64_t tiles, int64_t Kp, int64_t R, int64_t NB, in
     }
   }
   return best;
+}
+
+// Complex x complex digit GEMMs from three real products: large ones by
+// default; TDCC_M3=0 keeps the four-product GEMM on interleaved re/im rows,
+// 2 takes every complex x complex single-split digit GEMM (a test mode).
+static int m3_mode() {
+  const char* e = std::getenv("TDCC_M3");
+  return e ? std::atoi(e) : 1;
 }
 
 // Digits a digit-GEMM launch multiplies with (at most the split's S):
@@ -1682,8 +1696,13 @@ void Graph::lower_contract(int id) {
       gp.a_cols = !akc;
       gp.b_cols = !bkc;
       gp.Kp = (K + 63) / 64 * 64;
-      gp.a_plane = al256(R * gp.Kp);
-      gp.b_plane = al256(NB * gp.Kp);
+      // 3M: complex x complex, long enough K for the fused single-split path
+      // of the LDS-direct kernel, and enough work for the two launches to pay
+      gp.m3 = L->ac && L->bc && m3_mode() > 0 && ozgemm_interleaved_ok(S) &&
+              (m3_mode() == 2 || (gp.Kp > 256 && M >= 512 && Nn >= 512 && (double)M * Nn * K >= 1e9));
+      const int64_t Rx = gp.m3 ? 3 * M : R, NBx = gp.m3 ? 3 * Nn : NB;  // digit rows
+      gp.a_plane = al256(Rx * gp.Kp);
+      gp.b_plane = al256(NBx * gp.Kp);
       const int64_t bn = S <= 5 ? 128 : 64;
       const int64_t tiles = ((R + 127) / 128) * ((NB + bn - 1) / bn);
       int64_t sp = 1;
@@ -1701,7 +1720,12 @@ void Graph::lower_contract(int id) {
       // digits side by side, so a K step reads whole lines of each row (the
       // planar layout reads a quarter line per row and step, and at short K
       // the lines leave L2 before the next steps use the rest)
-      gp.dil = gp.Kp <= 256 && ozgemm_interleaved_ok(S);
+      if (sp > 1) gp.m3 = false;  // partial products go through the scatter kernel
+      gp.dil = gp.Kp <= 256 && ozgemm_interleaved_ok(S) && !gp.m3;
+      if (!gp.m3) {
+        gp.a_plane = al256(R * gp.Kp);
+        gp.b_plane = al256(NB * gp.Kp);
+      }
       // a row-oriented operand whose rows are too long for the single-pass
       // split (over 16384) gets one scale per row and K split, so that each
       // chunk is split in one pass instead of three (max, scale, digits)
@@ -1710,16 +1734,18 @@ void Graph::lower_contract(int id) {
       };
       gp.a_chunked = chunked(gp.a_cols);
       gp.b_chunked = chunked(gp.b_cols);
-      const int64_t a_scales = R * (gp.a_chunked ? sp : 1), b_scales = NB * (gp.b_chunked ? sp : 1);
+      const int64_t a_scales = (gp.m3 ? 3 * M : R) * (gp.a_chunked ? sp : 1),
+                    b_scales = (gp.m3 ? 3 * Nn : NB) * (gp.b_chunked ? sp : 1);
       gp.a8_off = 0;
       gp.b8_off = S * gp.a_plane;
       gp.sa_off = gp.b8_off + S * gp.b_plane;
       gp.sb_off = gp.sa_off + al256(8 * a_scales);
       gp.pa_off = gp.sb_off + al256(8 * b_scales);
-      gp.pb_off = gp.pa_off + al256(ozaki_split_workspace(R, gp.Kp, gp.a_cols));
-      gp.zb_off = gp.pb_off + al256(ozaki_split_workspace(NB, gp.Kp, gp.b_cols));
+      gp.pb_off = gp.pa_off + al256(ozaki_split_workspace(gp.m3 ? 3 * M : R, gp.Kp, gp.a_cols));
+      gp.zb_off = gp.pb_off + al256(ozaki_split_workspace(gp.m3 ? 3 * Nn : NB, gp.Kp, gp.b_cols));
       gp.fused = sp == 1;
-      n.scratch_len = gp.zb_off + (gp.fused ? 0 : al256(8 * sp * R * NB));
+      gp.p12_off = gp.zb_off;
+      n.scratch_len = gp.zb_off + (gp.m3 ? al256(8 * 2 * M * Nn) : (gp.fused ? 0 : al256(8 * sp * R * NB)));
       // layout of an operand's digit planes: its value, its index labels, the
       // row and summed letters in order, pair restrictions, orientation, rows
       // (planes included), digits, plane interleaving
@@ -1727,7 +1753,8 @@ void Graph::lower_contract(int id) {
                             bool cols, int64_t nrows, bool chk) {
         return nodes_[x].key + "|" + labels + "|" + rows + "|" + rpair + "|" + gk + "|" + n.kpair + "|" +
                (cols ? "c" : "r") + "|" + std::to_string(nrows) + "|" + std::to_string(S) + "|" +
-               (gp.fused ? "i" : "s") + (gp.dil ? "|d" : "") + (chk ? "|k" + std::to_string(chunk) : "");
+               (gp.m3 ? "3" : (gp.fused ? "i" : "s")) + (gp.dil ? "|d" : "") +
+               (chk ? "|k" + std::to_string(chunk) : "");
       };
       auto cache_entry = [&](int x, const std::string& key, int64_t nscales, int64_t plane) {
         if (!is_static(x)) return -1;
@@ -1742,8 +1769,8 @@ void Graph::lower_contract(int id) {
         cache_index_[key] = idx;
         return idx;
       };
-      const std::string akey = layout_key(a, la, gm, mpair, gp.a_cols, R, gp.a_chunked);
-      const std::string bkey = layout_key(b, lb, gn, npair, gp.b_cols, NB, gp.b_chunked);
+      const std::string akey = layout_key(a, la, gm, mpair, gp.a_cols, gp.m3 ? 3 * M : R, gp.a_chunked);
+      const std::string bkey = layout_key(b, lb, gn, npair, gp.b_cols, gp.m3 ? 3 * Nn : NB, gp.b_chunked);
       gp.a_cache = cache_entry(a, akey, a_scales, gp.a_plane);
       gp.b_cache = cache_entry(b, bkey, b_scales, gp.b_plane);
       if (gp.a_cache < 0 && !n.zloop) dyn_uses_[akey].push_back({id, 0, a, S * gp.a_plane + al256(8 * a_scales)});
@@ -1882,7 +1909,7 @@ void Graph::bind(double* arena) {
           sa.rtab = g.am;
           sa.ktab = g.ak;
           sa.Rb = g.M;
-          sa.planes = gp.pa;
+          sa.planes = gp.m3 ? 3 : gp.pa;
           sa.K = g.K;
           sa.Kp = gp.Kp;
           sa.S = gp.S;
@@ -1891,7 +1918,7 @@ void Graph::bind(double* arena) {
           sa.scale = reinterpret_cast<double*>(scr + gp.sa_off);
           sa.pmax = reinterpret_cast<double*>(scr + gp.pa_off);
           sa.multipass = split_multipass();
-          sa.interleave = gp.fused;
+          sa.interleave = gp.fused && !gp.m3;
           sa.dil = gp.dil;
           sa.sk_chunk = gp.a_chunked ? gp.k_chunk : 0;
           SplitArgs& sb = L->sb;  // B' = [B_re; B_im] (pb*N) x K
@@ -1900,7 +1927,7 @@ void Graph::bind(double* arena) {
           sb.rtab = g.bn;
           sb.ktab = g.bk;
           sb.Rb = g.N;
-          sb.planes = gp.pb;
+          sb.planes = gp.m3 ? 3 : gp.pb;
           sb.K = g.K;
           sb.Kp = gp.Kp;
           sb.S = gp.S;
@@ -1909,7 +1936,7 @@ void Graph::bind(double* arena) {
           sb.scale = reinterpret_cast<double*>(scr + gp.sb_off);
           sb.pmax = reinterpret_cast<double*>(scr + gp.pb_off);
           sb.multipass = split_multipass();
-          sb.interleave = gp.fused;
+          sb.interleave = gp.fused && !gp.m3;
           sb.dil = gp.dil;
           sb.sk_chunk = gp.b_chunked ? gp.k_chunk : 0;
           simplify_split_tables(sa, L->tab[0], L->tab[1]);
@@ -1957,7 +1984,42 @@ void Graph::bind(double* arena) {
           oz.k_chunk = (int)gp.k_chunk;
           oz.splits = gp.splits;
           oz.c_split_stride = (int64_t)oz.M * oz.N;
-          if (gp.fused) {
+          if (gp.m3) {
+            // launch 1: P1, P2 (digit rows [0, M) and [M, 2M) of A against the
+            // same of B) into the node scratch; launch 2: P3 (rows [2M, 3M))
+            // and the complex result through the output tables
+            oz.M = g.M;
+            oz.N = g.N;
+            oz.ldc = g.N;
+            oz.C = reinterpret_cast<double*>(scr + gp.p12_off);
+            oz.epi = -1;
+            oz.nb = 2;
+            oz.a_zstride = (int64_t)g.M * gp.Kp;
+            oz.b_zstride = (int64_t)g.N * gp.Kp;
+            oz.sa_zstride = g.M;
+            oz.sb_zstride = g.N;
+            oz.c_zstride = (int64_t)g.M * g.N;
+            OzGemmArgs& o2 = L->oz2;
+            o2 = oz;
+            o2.nb = 1;
+            o2.A = oz.A + 2 * oz.a_zstride;
+            o2.B = oz.B + 2 * oz.b_zstride;
+            o2.sa = oz.sa + 2 * oz.sa_zstride;
+            o2.sb = oz.sb + 2 * oz.sb_zstride;
+            o2.C = nullptr;
+            o2.epi = 3;
+            o2.p12 = oz.C;
+            o2.p12_ld = g.N;
+            o2.p12_stride = (int64_t)g.M * g.N;
+            o2.cm = g.cm;
+            o2.cn = g.cn;
+            o2.cm2 = L->tab[9] >= 0 ? dev_tables_ + L->tab[9] : nullptr;
+            o2.cn2 = L->tab[10] >= 0 ? dev_tables_ + L->tab[10] : nullptr;
+            o2.out_re = C.re;
+            o2.out_im = C.im;
+            o2.alpha_re = n.ar * n.kfac;
+            o2.alpha_im = n.ai * n.kfac;
+          } else if (gp.fused) {
             oz.C = nullptr;
             oz.epi = (L->ac && L->bc) ? 2 : (L->ac ? 1 : 0);
             oz.cm = g.cm;
@@ -2363,21 +2425,28 @@ void Graph::autotune(bool k_splits) {
   TDCC_HIP(hipEventCreate(&e1));
   std::string report;
   try {
+    // the digit-GEMM launches (two for a 3M contraction)
+    struct Target {
+      OzGemmArgs* oz;
+      GemmScatterArgs* sc;
+      int* lsplits;
+    };
+    std::vector<Target> targets;
     for (auto& L : launches_) {
-      OzGemmArgs* oz = nullptr;
-      GemmScatterArgs* sc = nullptr;
-      int* lsplits = nullptr;
       if (L->kind == Launch::GETT && L->gp.on && L->gp.S > 0 && !L->gp.dot && L->g.M > 0 && L->g.N > 0) {
-        oz = &L->oz;
-        if (!L->gp.fused) sc = &L->sc;
+        targets.push_back({&L->oz, L->gp.fused ? nullptr : &L->sc, nullptr});
+        if (L->gp.m3) targets.push_back({&L->oz2, nullptr, nullptr});
       } else if (L->kind == Launch::LADDER && L->lad.S > 0) {
-        oz = &L->lad.oz;
-        lsplits = &L->lad.splits;
+        targets.push_back({&L->lad.oz, nullptr, &L->lad.splits});
       }
-      if (!oz) continue;
+    }
+    for (const Target& tg : targets) {
+      OzGemmArgs* oz = tg.oz;
+      GemmScatterArgs* sc = tg.sc;
+      int* lsplits = tg.lsplits;
       char key[160];
-      std::snprintf(key, sizeof key, "S=%d M=%d N=%d Kp=%d chunk=%d splits=%d epi=%d%s", oz->S, oz->M, oz->N, oz->Kp,
-                    oz->k_chunk, oz->splits, oz->epi, k_splits ? " ksplits" : "");
+      std::snprintf(key, sizeof key, "S=%d M=%d N=%d Kp=%d chunk=%d splits=%d epi=%d%s%s", oz->S, oz->M, oz->N,
+                    oz->Kp, oz->k_chunk, oz->splits, oz->epi, oz->nb > 1 ? " nb=2" : "", k_splits ? " ksplits" : "");
       TuneChoice best;
       bool have = false;
       {
@@ -2518,7 +2587,8 @@ void Graph::run(hipStream_t s) const {
           if (L->gp.a_cache < 0 && L->gp.a_split) launch_ozaki_split(L->sa, L->gp.a_cols, s);
           if (L->gp.b_cache < 0 && L->gp.b_split) launch_ozaki_split(L->sb, L->gp.b_cols, s);
           launch_ozgemm(L->oz, s);
-          if (!L->gp.fused) launch_gemm_scatter(L->sc, s);
+          if (L->gp.m3) launch_ozgemm(L->oz2, s);
+          else if (!L->gp.fused) launch_gemm_scatter(L->sc, s);
           break;
         }
         if (L->gp.on) {
@@ -2605,6 +2675,18 @@ void Graph::run_profiled(hipStream_t s, std::vector<LaunchRecord>& out) const {
               split_detail(L->sb, "B");
               timed(gp.b_cols ? "ozaki_split_cols_kernel" : "ozaki_split_rows_kernel", 0.0, bbytes,
                     [&] { launch_ozaki_split(L->sb, gp.b_cols, s); });
+            }
+            if (gp.m3) {
+              // 3M: two thirds of the FP64-equivalent flops in the P1, P2 launch
+              std::snprintf(dt, sizeof dt, "%s rows=%d N=%d K=%d splits=%d digits=%d 3M P1+P2",
+                            nodes_[L->node].spec.c_str(), L->oz.M, L->oz.N, L->g.K, gp.splits, gp.S);
+              detail = dt;
+              timed(ozgemm_kernel_name(L->oz.S), fl * 2.0 / 3.0, 0.0, [&] { launch_ozgemm(L->oz, s); });
+              std::snprintf(dt, sizeof dt, "%s rows=%d N=%d K=%d splits=%d digits=%d 3M P3",
+                            nodes_[L->node].spec.c_str(), L->oz2.M, L->oz2.N, L->g.K, gp.splits, gp.S);
+              detail = dt;
+              timed(ozgemm_kernel_name(L->oz2.S), fl / 3.0, 0.0, [&] { launch_ozgemm(L->oz2, s); });
+              break;
             }
             std::snprintf(dt, sizeof dt, "%s rows=%d N=%d K=%d splits=%d digits=%d", nodes_[L->node].spec.c_str(),
                           L->oz.M, L->oz.N, L->g.K, gp.splits, gp.S);

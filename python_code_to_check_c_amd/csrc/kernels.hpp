@@ -297,7 +297,8 @@ struct SplitArgs {
   // Element (r, k) of the R x K source, R = planes * Rb, is
   //   plane(r / Rb)[rtab[r % Rb] + ktab[k]]   (offset tables), or
   //   plane(r / Rb)[(r % Rb) * ldx + k]       (rtab == ktab == nullptr);
-  // plane 0 is src_re, plane 1 src_im.
+  // plane 0 is src_re, plane 1 src_im, and with planes == 3 (not interleaved)
+  // plane 2 is src_re + src_im (the third operand of a 3-product complex GEMM).
   const double* src_re; const double* src_im;
   const int64_t* rtab; const int64_t* ktab;
   int64_t ldx;
@@ -359,6 +360,19 @@ struct OzGemmArgs {
   // scales per K split (SplitArgs::sk_chunk == k_chunk): split y reads
   // sa + y * sa_split_stride (sb likewise); 0: one scale per row for all splits
   int64_t sa_split_stride = 0, sb_split_stride = 0;
+  // nb independent products of the same shape in one launch (LDS-direct
+  // kernel, one split): product z reads A + z a_zstride, B + z b_zstride,
+  // sa + z sa_zstride, sb + z sb_zstride and writes C + z c_zstride
+  int nb = 1;
+  int64_t a_zstride = 0, b_zstride = 0, sa_zstride = 0, sb_zstride = 0, c_zstride = 0;
+  // epi 3 (complex x complex from three real products, 3M): this launch's
+  // product is P3 = (A_re + A_im)(B_re + B_im)^T, P1 = A_re B_re^T and
+  // P2 = A_im B_im^T are read from p12[m * p12_ld + n] and
+  // p12[p12_stride + m * p12_ld + n], and
+  //   value(m, n) = (P1 - P2) + i (P3 - P1 - P2)
+  // goes through the output tables as in epi 0..2 (rows and columns complex)
+  const double* p12 = nullptr;
+  int64_t p12_ld = 0, p12_stride = 0;
 };
 void launch_ozgemm(const OzGemmArgs& a, hipStream_t s);
 // Whether the digit GEMM for S digits reads digit-interleaved operands.

@@ -32,12 +32,22 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 constexpr int SR_TR = 16, SR_TK = 256;  // rows orientation tile
 constexpr int SC_TR = 64, SC_TK = 64;   // columns orientation tile
 
-__device__ __forceinline__ const double* split_row_base(const SplitArgs& a, int64_t r) {
+// Source of one row: element o is p[o], or p[o] + q[o] for the third plane
+// of a three-plane operand (re, im, re + im).
+struct RowSrc {
+  const double* p;
+  const double* q;
+};
+
+__device__ __forceinline__ RowSrc split_row_base(const SplitArgs& a, int64_t r) {
   const bool il = a.interleave && a.planes == 2;
   const int64_t p = il ? (r & 1) : r / a.Rb, rb = il ? (r >> 1) : r - p * a.Rb;
-  const double* src = p ? a.src_im : a.src_re;
-  return src + (a.rtab ? a.rtab[rb] : rb * a.ldx);
+  const int64_t off = a.rtab ? a.rtab[rb] : rb * a.ldx;
+  if (p == 2) return {a.src_re + off, a.src_im + off};
+  return {(p ? a.src_im : a.src_re) + off, nullptr};
 }
+
+__device__ __forceinline__ double row_at(const RowSrc& b, int64_t o) { return b.q ? b.p[o] + b.q[o] : b.p[o]; }
 
 __device__ __forceinline__ int64_t split_koff(const SplitArgs& a, int64_t k) { return a.ktab ? a.ktab[k] : k; }
 
@@ -53,19 +63,28 @@ __device__ __forceinline__ double abs_or_inf(double v) { return std::isfinite(v)
 // x[k .. k+3] of a row (0 past K): two 16-byte loads when the row's k are
 // contiguous (no k table) and the address is 16-byte aligned, else one load
 // per element through the k table.
-__device__ __forceinline__ void load_row4(const SplitArgs& a, const double* base, int64_t k, double v[4],
+__device__ __forceinline__ void load_row4(const SplitArgs& a, const RowSrc& base, int64_t k, double v[4],
                                           int64_t kend = -1) {
   if (kend < 0) kend = a.K;
-  if (!a.ktab && k + 3 < kend && (reinterpret_cast<uintptr_t>(base + k) & 15) == 0) {
-    const double2 x = *reinterpret_cast<const double2*>(base + k);
-    const double2 y = *reinterpret_cast<const double2*>(base + k + 2);
+  if (!a.ktab && k + 3 < kend && (reinterpret_cast<uintptr_t>(base.p + k) & 15) == 0 &&
+      (!base.q || (reinterpret_cast<uintptr_t>(base.q + k) & 15) == 0)) {
+    const double2 x = *reinterpret_cast<const double2*>(base.p + k);
+    const double2 y = *reinterpret_cast<const double2*>(base.p + k + 2);
     v[0] = x.x;
     v[1] = x.y;
     v[2] = y.x;
     v[3] = y.y;
+    if (base.q) {
+      const double2 x2 = *reinterpret_cast<const double2*>(base.q + k);
+      const double2 y2 = *reinterpret_cast<const double2*>(base.q + k + 2);
+      v[0] += x2.x;
+      v[1] += x2.y;
+      v[2] += y2.x;
+      v[3] += y2.y;
+    }
   } else {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) v[q] = k + q < kend ? base[split_koff(a, k + q)] : 0.0;
+    for (int q = 0; q < 4; ++q) v[q] = k + q < kend ? row_at(base, split_koff(a, k + q)) : 0.0;
   }
 }
 
@@ -95,7 +114,7 @@ __global__ __launch_bounds__(256) void split_max_rows_kernel(SplitArgs a) {
     const int64_t r = (int64_t)blockIdx.x * SR_TR + rg + 4 * i;
     double m = 0.0;
     if (r < R) {
-      const double* base = split_row_base(a, r);
+      const RowSrc base = split_row_base(a, r);
       if (!a.ktab) {
         double v[4];
         load_row4(a, base, k0, v);
@@ -104,7 +123,7 @@ __global__ __launch_bounds__(256) void split_max_rows_kernel(SplitArgs a) {
       } else {
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-          if (ko[j] >= 0) m = fmax(m, abs_or_inf(base[ko[j]]));
+          if (ko[j] >= 0) m = fmax(m, abs_or_inf(row_at(base, ko[j])));
       }
     }
 #pragma unroll
@@ -121,10 +140,10 @@ __global__ __launch_bounds__(256) void split_max_cols_kernel(SplitArgs a) {
   const int64_t k0 = (int64_t)blockIdx.y * SC_TK + 16 * kg;
   double m = 0.0;
   if (r < R) {
-    const double* base = split_row_base(a, r);
+    const RowSrc base = split_row_base(a, r);
 #pragma unroll
     for (int j = 0; j < 16; ++j)
-      if (k0 + j < a.K) m = fmax(m, abs_or_inf(base[split_koff(a, k0 + j)]));
+      if (k0 + j < a.K) m = fmax(m, abs_or_inf(row_at(base, split_koff(a, k0 + j))));
   }
   red[kg][rl] = m;
   __syncthreads();
@@ -167,7 +186,7 @@ __global__ __launch_bounds__(256) void split_rows_wave_kernel(SplitArgs a) {
   const int lane = threadIdx.x & 63;
   const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (r >= R) return;  // whole wave leaves together
-  const double* base = split_row_base(a, r);
+  const RowSrc base = split_row_base(a, r);
   double v[4];
   double m = 0.0;
   load_row4(a, base, 4 * lane, v);
@@ -202,7 +221,7 @@ __global__ __launch_bounds__(256) void split_digits_rows_kernel(SplitArgs a) {
     const double sc = a.scale[r];
     const bool finite = std::isfinite(sc);
     const int e = finite ? ilogb(sc) : 0;
-    const double* base = split_row_base(a, r);
+    const RowSrc base = split_row_base(a, r);
     double v[4];
     if (!a.ktab) {
       load_row4(a, base, k0, v);
@@ -210,7 +229,7 @@ __global__ __launch_bounds__(256) void split_digits_rows_kernel(SplitArgs a) {
       for (int j = 0; j < 4; ++j) v[j] = finite ? ldexp(v[j], -e) : 0.0;
     } else {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) v[j] = (finite && ko[j] >= 0) ? ldexp(base[ko[j]], -e) : 0.0;
+      for (int j = 0; j < 4; ++j) v[j] = (finite && ko[j] >= 0) ? ldexp(row_at(base, ko[j]), -e) : 0.0;
     }
 #pragma unroll
     for (int s = 0; s < S; ++s) *reinterpret_cast<uint32_t*>(digit_ptr<S>(a, r, k0, s)) = next_digit4(v);
@@ -227,7 +246,7 @@ __global__ __launch_bounds__(256) void split_digits_cols_kernel(SplitArgs a) {
   const double sc = a.scale[r];
   const bool finite = std::isfinite(sc);
   const int e = finite ? ilogb(sc) : 0;
-  const double* base = split_row_base(a, r);
+  const RowSrc base = split_row_base(a, r);
   uint32_t out[4][S];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -235,7 +254,7 @@ __global__ __launch_bounds__(256) void split_digits_cols_kernel(SplitArgs a) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int64_t k = k0 + 4 * q + j;
-      v[j] = (finite && k < a.K) ? ldexp(base[split_koff(a, k)], -e) : 0.0;
+      v[j] = (finite && k < a.K) ? ldexp(row_at(base, split_koff(a, k)), -e) : 0.0;
     }
 #pragma unroll
     for (int s = 0; s < S; ++s) out[q][s] = next_digit4(v);
@@ -259,7 +278,7 @@ __global__ __launch_bounds__(256) void split_rows_resident_kernel(SplitArgs a) {
   const int64_t klo = (int64_t)blockIdx.y * a.sk_chunk;
   const int64_t khi = a.sk_chunk > 0 ? min(a.K, klo + a.sk_chunk) : a.K;
   const int64_t kphi = a.sk_chunk > 0 ? min(a.Kp, klo + a.sk_chunk) : a.Kp;
-  const double* base = split_row_base(a, r);
+  const RowSrc base = split_row_base(a, r);
   double v[NJ][4];
   double m = 0.0;
 #pragma unroll
@@ -294,7 +313,7 @@ __global__ __launch_bounds__(256) void split_cols_resident_kernel(SplitArgs a) {
   const int rl = threadIdx.x & 63, g = threadIdx.x >> 6;
   const int64_t r = (int64_t)blockIdx.x * SC_TR + rl;
   const bool live = r < R;
-  const double* base = live ? split_row_base(a, r) : nullptr;
+  const RowSrc base = live ? split_row_base(a, r) : RowSrc{nullptr, nullptr};
   double v[NJ][16];
   double m = 0.0;
 #pragma unroll
@@ -302,7 +321,7 @@ __global__ __launch_bounds__(256) void split_cols_resident_kernel(SplitArgs a) {
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int64_t k = 64 * (int64_t)j + 16 * g + q;
-      v[j][q] = (live && k < a.K) ? base[split_koff(a, k)] : 0.0;
+      v[j][q] = (live && k < a.K) ? row_at(base, split_koff(a, k)) : 0.0;
       m = fmax(m, abs_or_inf(v[j][q]));
     }
   red[g][rl] = m;
@@ -942,18 +961,19 @@ __global__ __launch_bounds__(512, 1) void ozgemm8d_kernel(OzGemmArgs g) {
   __shared__ v4i Bs[NBUF][S][BN * 2];
   __shared__ double sA[OZ_BM], sB[BN];
   const int tiles_m = (g.M + OZ_BM - 1) / OZ_BM, tiles_n = (g.N + BN - 1) / BN;
-  const int tiles = tiles_m * tiles_n;
+  const int tiles1 = tiles_m * tiles_n, tiles = tiles1 * g.nb;  // nb products one after another
   const int tid = threadIdx.x;
   const int kb = g.k_chunk > 0 ? (int)blockIdx.y * g.k_chunk : 0;
   const int ke = g.k_chunk > 0 ? min(g.Kp, kb + g.k_chunk) : g.Kp;
   const int nk = (ke - kb + BK - 1) / BK;
-  double* const Cout = g.C + (int64_t)blockIdx.y * g.c_split_stride;
+  double* Cout = g.C + (int64_t)blockIdx.y * g.c_split_stride;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const bool on_b = wave >= 4;
   const int lane = tid & 63;
   const int pc = 64 * (wave & 3) + lane;                 // physical chunk this lane fills
   const int lrow = pc >> 1, lchunk = (pc & 1) ^ ((lrow >> 3) & 1);
-  const int8_t* const xbase = on_b ? g.B : g.A;
+  const int8_t* xbase = on_b ? g.B : g.A;
+  int zb = 0;  // product of the current tile
   // planar digits: row stride Kp, K offset k, digit stride plane; digit-
   // interleaved rows: row stride Kp S, K offset k S (k a multiple of 32),
   // digit stride 32
@@ -964,7 +984,11 @@ __global__ __launch_bounds__(512, 1) void ozgemm8d_kernel(OzGemmArgs g) {
   int m0 = 0, n0 = 0;
   int64_t xrow = 0;
   auto setup = [&](int t) {
-    const int L = remap_tile(t, tiles);
+    const int L0 = remap_tile(t, tiles);
+    zb = L0 / tiles1;
+    const int L = L0 - zb * tiles1;
+    xbase = on_b ? g.B + zb * g.b_zstride : g.A + zb * g.a_zstride;
+    Cout = g.C ? g.C + (int64_t)blockIdx.y * g.c_split_stride + zb * g.c_zstride : nullptr;
     int tm, tn;
     grouped_tile(L, tiles_m, tiles_n, g.tile_group, tm, tn);
     m0 = tm * OZ_BM;
@@ -992,8 +1016,11 @@ __global__ __launch_bounds__(512, 1) void ozgemm8d_kernel(OzGemmArgs g) {
   for (;;) {
     // the tile's scales first (their loads complete before the copies start),
     // then the copies of its first two K steps
-    if (tid < OZ_BM) sA[tid] = m0 + tid < g.M ? g.sa[(int64_t)blockIdx.y * g.sa_split_stride + m0 + tid] : 0.0;
-    else if (tid - OZ_BM < BN) sB[tid - OZ_BM] = n0 + tid - OZ_BM < g.N ? g.sb[(int64_t)blockIdx.y * g.sb_split_stride + n0 + tid - OZ_BM] : 0.0;
+    if (tid < OZ_BM)
+      sA[tid] = m0 + tid < g.M ? g.sa[(int64_t)blockIdx.y * g.sa_split_stride + zb * g.sa_zstride + m0 + tid] : 0.0;
+    else if (tid - OZ_BM < BN)
+      sB[tid - OZ_BM] = n0 + tid - OZ_BM < g.N
+          ? g.sb[(int64_t)blockIdx.y * g.sb_split_stride + zb * g.sb_zstride + n0 + tid - OZ_BM] : 0.0;
     if (nk > 0) issue(0);
     if (nk > 1) issue(1);
     v16i acc[S][2];
@@ -1052,6 +1079,16 @@ __global__ __launch_bounds__(512, 1) void ozgemm8d_kernel(OzGemmArgs g) {
 #pragma unroll
           for (int r = 0; r < 4; ++r)
             if (colok && row0 + r < g.M) Cout[(int64_t)(row0 + r) * g.ldc + col] = z[r];
+        } else if (g.epi == 3) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (colok && row0 + r < g.M) {
+              const int64_t po = (int64_t)(row0 + r) * g.p12_ld + col;
+              const double p1 = g.p12[po], p2 = g.p12[g.p12_stride + po];
+              const double re = p1 - p2, im = (z[r] - p1) - p2;
+              store_mirrored(g.out_re, g.out_im, g.cm[row0 + r], g.cn[col], g.cm2 ? g.cm2[row0 + r] : -1,
+                             g.cn2 ? g.cn2[col] : -1, ar * re - ai * im, ar * im + ai * re);
+            }
         } else if (g.epi == 0) {
 #pragma unroll
           for (int r = 0; r < 4; ++r)
@@ -1151,7 +1188,10 @@ void launch_ozgemm(const OzGemmArgs& args, hipStream_t s) {
   OzGemmArgs g = args;
   g.tile_group = g.tune_group > 0 ? g.tune_group : oz_tile_group((g.M + OZ_BM - 1) / OZ_BM, (g.N + 127) / 128);
   const int bn = g.S <= 5 ? 128 : 64;
-  const int tiles = ((g.M + OZ_BM - 1) / OZ_BM) * ((g.N + bn - 1) / bn);
+  // several products per launch (nb > 1) and the 3M epilogue exist only in the
+  // LDS-direct kernel
+  if ((g.nb > 1 || g.epi == 3) && !((g.S == 4 || g.S == 5) && ozgemm_direct())) return;
+  const int tiles = ((g.M + OZ_BM - 1) / OZ_BM) * ((g.N + bn - 1) / bn) * g.nb;
   const int splits = g.k_chunk > 0 ? g.splits : 1;
   const int pb = oz_persistent_blocks();
   // persistent only with few K steps per tile (at 50 K steps one block per

@@ -42,12 +42,16 @@ def bound(S):
 # kernel at every summed length (at large sizes only from K = 8192 on).
 # "chunked" cuts every dense-GEMM contraction into 3 K splits and gives each
 # row-oriented operand with K > 256 one scale per row and split, split in one
-# pass per chunk (at large sizes only for rows longer than 16384).
+# pass per chunk (at large sizes only for rows longer than 16384).  "m3"
+# computes every complex x complex digit GEMM from three real products
+# (re re, im im, (re + im)(re + im); at large sizes only the single-split
+# ones above 512 x 512 complex outputs and 1e9 multiply-adds).
 MODES = {"default": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_NPAIR_MIN_K": "0"},
          "multipass": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_SPLIT_MULTIPASS": "1"},
          "stream": {"TDCC_STATIC_STREAM_MIN": "0"},
          "dot": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_DOT_MIN_K": "0"},
-         "chunked": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_GEMM_SPLITS": "3", "TDCC_SPLIT_CHUNKED": "1"}}
+         "chunked": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_GEMM_SPLITS": "3", "TDCC_SPLIT_CHUNKED": "1"},
+         "m3": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_M3": "2"}}
 
 
 @pytest.fixture(scope="module", params=[(n, m) for m in MODES for n in ("o4v6", "c1_o10v4")],
@@ -57,7 +61,7 @@ def case(request):
     g = dict(np.load(os.path.join(HERE, "golden", f"golden_{name}.npz")))
     print(f"\n[split case {name} {mode}]", flush=True)
     keys = ("TDCC_GEMM_MIN_FLOPS", "TDCC_SPLIT_MULTIPASS", "TDCC_STATIC_STREAM_MIN", "TDCC_NPAIR_MIN_K",
-            "TDCC_DOT_MIN_K", "TDCC_GEMM_SPLITS", "TDCC_SPLIT_CHUNKED")
+            "TDCC_DOT_MIN_K", "TDCC_GEMM_SPLITS", "TDCC_SPLIT_CHUNKED", "TDCC_M3")
     saved = {k: os.environ.get(k) for k in keys}
     for k in keys:
         os.environ.pop(k, None)
