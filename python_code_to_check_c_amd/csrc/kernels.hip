@@ -2,6 +2,7 @@
 #include "kernels.hpp"
 
 #include <cstdio>
+#include <cstdlib>
 
 namespace tdcc {
 
@@ -363,17 +364,31 @@ __global__ void lincomb_kernel(LincombArgs a) {
   }
 }
 
-// Every term laid out like the contiguous output (no permutation, no product
-// term): element e of each operand sits at offset e, so a thread handles two
-// consecutive elements with 16-byte accesses; same arithmetic and term order
-// as lincomb_kernel.
+// Every plain term laid out like the contiguous output (no permutation):
+// element e of each such operand sits at offset e, so a thread handles two
+// consecutive elements with 16-byte accesses.  Product terms (x * y of two
+// small permuted operands, such as t1 t1 formed in place) are read element by
+// element at the offsets of e's index tuple (decomposed in 32-bit arithmetic;
+// e and e + 1 share every index but the innermost, whose extent is even).
+// Same arithmetic and term order as lincomb_kernel.
+template <bool PROD>
 __global__ __launch_bounds__(256) void lincomb_flat_kernel(LincombArgs a) {
   const int64_t n2 = a.total >> 1;
   const int64_t zo = blockIdx.y, oz = zo * a.ozs;
+  const int R = a.rank;
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q <= n2; q += (int64_t)gridDim.x * blockDim.x) {
     const int64_t e = 2 * q;
     if (e >= a.total) break;
     const bool pair = e + 1 < a.total;
+    uint32_t ix[4] = {0, 0, 0, 0};
+    if (PROD) {
+      uint32_t rem = (uint32_t)e;
+      for (int d = R - 1; d >= 0; --d) {
+        const uint32_t n = (uint32_t)a.dims[d], qd = rem / n;
+        ix[d] = rem - qd * n;
+        rem = qd;
+      }
+    }
     double sr[2] = {0.0, 0.0}, si[2] = {0.0, 0.0};
     if (a.accumulate) {
       sr[0] = a.out_re[oz + e];
@@ -385,21 +400,32 @@ __global__ __launch_bounds__(256) void lincomb_flat_kernel(LincombArgs a) {
     }
     for (int t = 0; t < a.nterms; ++t) {
       const LincombTerm& T = a.t[t];
-      const int64_t te = zo * T.zs + e;
       double xr[2], xi[2] = {0.0, 0.0};
-      if (pair) {
-        const double2 v = *reinterpret_cast<const double2*>(T.re + te);
-        xr[0] = v.x;
-        xr[1] = v.y;
-        if (T.im) {
-          const double2 w = *reinterpret_cast<const double2*>(T.im + te);
-          xi[0] = w.x;
-          xi[1] = w.y;
+      if (PROD && T.re2) {
+        int64_t so = zo * T.zs, so2 = zo * T.zs2;
+        for (int d = 0; d < R; ++d) {
+          so += (int64_t)ix[d] * T.st[d];
+          so2 += (int64_t)ix[d] * T.st2[d];
         }
+        lincomb_term_value(T, so, so2, xr[0], xi[0]);
+        if (pair) lincomb_term_value(T, so + T.st[R - 1], so2 + T.st2[R - 1], xr[1], xi[1]);
+        else xr[1] = xi[1] = 0.0;
       } else {
-        xr[0] = T.re[te];
-        xr[1] = 0.0;
-        if (T.im) xi[0] = T.im[te];
+        const int64_t te = zo * T.zs + e;
+        if (pair) {
+          const double2 v = *reinterpret_cast<const double2*>(T.re + te);
+          xr[0] = v.x;
+          xr[1] = v.y;
+          if (T.im) {
+            const double2 w = *reinterpret_cast<const double2*>(T.im + te);
+            xi[0] = w.x;
+            xi[1] = w.y;
+          }
+        } else {
+          xr[0] = T.re[te];
+          xr[1] = 0.0;
+          if (T.im) xi[0] = T.im[te];
+        }
       }
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
@@ -430,8 +456,22 @@ __global__ __launch_bounds__(256) void lincomb_flat_kernel(LincombArgs a) {
   }
 }
 
-// Whether every operand of `a` is laid out like its contiguous output, with
-// 16-byte aligned planes.
+// Whether every plain operand of `a` is laid out like its contiguous output,
+// with 16-byte aligned planes (product terms, read element by element, only
+// with an even innermost extent and fewer than 2^31 elements).
+// TDCC_LINCOMB_FLAT_PROD=0: linear combinations with product terms on the
+// element-wise kernel (a parity-test mode).
+static bool lincomb_flat_prod_on() {
+  const char* e = std::getenv("TDCC_LINCOMB_FLAT_PROD");
+  return !e || std::atoi(e) != 0;
+}
+
+static bool lincomb_has_prod(const LincombArgs& a) {
+  for (int t = 0; t < a.nterms; ++t)
+    if (a.t[t].re2) return true;
+  return false;
+}
+
 static bool lincomb_is_flat(const LincombArgs& a) {
   auto al16 = [](const double* p) { return p == nullptr || (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
   if (!al16(a.out_re) || !al16(a.out_im)) return false;
@@ -441,9 +481,12 @@ static bool lincomb_is_flat(const LincombArgs& a) {
     st *= a.dims[d];
   }
   if (a.nz > 1 && (a.ozs & 1)) return false;
+  if (lincomb_has_prod(a) && ((a.dims[a.rank - 1] & 1) || a.total >= ((int64_t)1 << 31) || !lincomb_flat_prod_on()))
+    return false;
   for (int t = 0; t < a.nterms; ++t) {
     const LincombTerm& T = a.t[t];
-    if (T.re2 || !al16(T.re) || !al16(T.im) || (a.nz > 1 && (T.zs & 1))) return false;
+    if (T.re2) continue;
+    if (!al16(T.re) || !al16(T.im) || (a.nz > 1 && (T.zs & 1))) return false;
     for (int d = 0; d < a.rank; ++d)
       if (T.st[d] != a.ost[d] && a.dims[d] != 1) return false;
   }
@@ -739,7 +782,10 @@ void launch_lincomb(const LincombArgs& a, hipStream_t s) {
     }
   }
   if (lincomb_is_flat(a)) {
-    hipLaunchKernelGGL(lincomb_flat_kernel, dim3(grid_for((a.total + 1) / 2), nz), dim3(256), 0, s, a);
+    if (lincomb_has_prod(a))
+      hipLaunchKernelGGL(lincomb_flat_kernel<true>, dim3(grid_for((a.total + 1) / 2), nz), dim3(256), 0, s, a);
+    else
+      hipLaunchKernelGGL(lincomb_flat_kernel<false>, dim3(grid_for((a.total + 1) / 2), nz), dim3(256), 0, s, a);
     return;
   }
   if (lincomb_use_tiled(a)) {
@@ -759,7 +805,7 @@ void launch_lincomb(const LincombArgs& a, hipStream_t s) {
 
 const char* lincomb_kernel_name(const LincombArgs& a) {
   if (a.total > 0 && lincomb_use_anti(a)) return "lincomb_anti_kernel";
-  if (a.total > 0 && lincomb_is_flat(a)) return "lincomb_flat_kernel";
+  if (a.total > 0 && lincomb_is_flat(a)) return lincomb_has_prod(a) ? "lincomb_flat_kernel<prod>" : "lincomb_flat_kernel";
   if (a.total > 0 && lincomb_use_tiled(a)) return "lincomb_tiled_kernel";
   return "lincomb_kernel";
 }

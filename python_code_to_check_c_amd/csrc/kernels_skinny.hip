@@ -322,6 +322,7 @@ static void skinny_dispatch(const SkinnyArgs& a, hipStream_t s) {
   if (a.I <= 4) skinny_launch<SC, GC, 4>(a, s);
   else if (a.I <= 8) skinny_launch<SC, GC, 8>(a, s);
   else if (a.I <= 16) skinny_launch<SC, GC, 16>(a, s);
+  else if (a.I <= 20) skinny_launch<SC, GC, 20>(a, s);  // o = 20: no padded rows
   else skinny_launch<SC, GC, 32>(a, s);
 }
 
