@@ -411,7 +411,13 @@ __global__ __launch_bounds__(256) void lincomb_flat_kernel(LincombArgs a) {
         if (pair) lincomb_term_value(T, so + T.st[R - 1], so2 + T.st2[R - 1], xr[1], xi[1]);
         else xr[1] = xi[1] = 0.0;
       } else {
-        const int64_t te = zo * T.zs + e;
+        // a term laid out like the output, or (with the index tuple) one whose
+        // rows are contiguous along the output's innermost axis at other strides
+        int64_t te = zo * T.zs + e;
+        if (PROD && T.rowwise) {
+          te = zo * T.zs;
+          for (int d = 0; d < R; ++d) te += (int64_t)ix[d] * T.st[d];
+        }
         if (pair) {
           const double2 v = *reinterpret_cast<const double2*>(T.re + te);
           xr[0] = v.x;
@@ -466,9 +472,26 @@ static bool lincomb_flat_prod_on() {
   return !e || std::atoi(e) != 0;
 }
 
+// A plain term not laid out like the output but contiguous along its innermost
+// axis with every other stride even (16-byte aligned pairs), read at the
+// offsets of the index tuple (e.g. an integral block viewed in the n^4 array).
+static bool lincomb_term_rowwise(const LincombArgs& a, const LincombTerm& T) {
+  if (T.re2 || a.rank < 1 || a.dims[a.rank - 1] % 2 || a.total >= ((int64_t)1 << 31) || !lincomb_flat_prod_on())
+    return false;
+  bool same = true;
+  for (int d = 0; d < a.rank; ++d) same = same && (T.st[d] == a.ost[d] || a.dims[d] == 1);
+  if (same) return false;
+  if (T.st[a.rank - 1] != 1) return false;
+  for (int d = 0; d < a.rank - 1; ++d)
+    if (T.st[d] & 1) return false;
+  return true;
+}
+
+// Product terms or row-wise terms: the flat kernel decomposes each element's
+// index tuple.
 static bool lincomb_has_prod(const LincombArgs& a) {
   for (int t = 0; t < a.nterms; ++t)
-    if (a.t[t].re2) return true;
+    if (a.t[t].re2 || lincomb_term_rowwise(a, a.t[t])) return true;
   return false;
 }
 
@@ -487,8 +510,10 @@ static bool lincomb_is_flat(const LincombArgs& a) {
     const LincombTerm& T = a.t[t];
     if (T.re2) continue;
     if (!al16(T.re) || !al16(T.im) || (a.nz > 1 && (T.zs & 1))) return false;
-    for (int d = 0; d < a.rank; ++d)
-      if (T.st[d] != a.ost[d] && a.dims[d] != 1) return false;
+    if (!lincomb_term_rowwise(a, T)) {
+      for (int d = 0; d < a.rank; ++d)
+        if (T.st[d] != a.ost[d] && a.dims[d] != 1) return false;
+    }
   }
   return true;
 }
@@ -782,10 +807,13 @@ void launch_lincomb(const LincombArgs& a, hipStream_t s) {
     }
   }
   if (lincomb_is_flat(a)) {
-    if (lincomb_has_prod(a))
-      hipLaunchKernelGGL(lincomb_flat_kernel<true>, dim3(grid_for((a.total + 1) / 2), nz), dim3(256), 0, s, a);
-    else
+    if (lincomb_has_prod(a)) {
+      LincombArgs b = a;
+      for (int t = 0; t < b.nterms; ++t) b.t[t].rowwise = lincomb_term_rowwise(a, a.t[t]);
+      hipLaunchKernelGGL(lincomb_flat_kernel<true>, dim3(grid_for((a.total + 1) / 2), nz), dim3(256), 0, s, b);
+    } else {
       hipLaunchKernelGGL(lincomb_flat_kernel<false>, dim3(grid_for((a.total + 1) / 2), nz), dim3(256), 0, s, a);
+    }
     return;
   }
   if (lincomb_use_tiled(a)) {

@@ -71,6 +71,9 @@ struct LincombTerm {
   // stacked trajectories: element offset of trajectory z is z * zs (zs2 for
   // the product factor); 0 for an operand shared by all trajectories
   int64_t zs = 0, zs2 = 0;
+  // flat kernel: read at the offsets of the index tuple (rows contiguous along
+  // the innermost axis, other strides unlike the output's); set at launch
+  bool rowwise = false;
 };
 struct LincombArgs {
   double* out_re; double* out_im;
