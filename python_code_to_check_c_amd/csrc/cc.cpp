@@ -4,6 +4,8 @@
 // contractions are shared between functions, stages and chains.
 #include "cc.hpp"
 
+#include <cstdlib>
+
 namespace tdcc {
 
 // ------------------------------------------------------------------ T side
@@ -86,16 +88,36 @@ int CCBuilder::Wmbej(int t1, int t2, int F) {  // CCSD_Helper.py:255-265
                .add(C("mbef,jf->mbej", blk("ovvv"), t1)));
 }
 
+// Wmbej (CCSD_Helper.py:255-265) stored with its axes in the order (j, b, m,
+// e): the T2 ring sums it over (m, e) against t_imae, so each (j, b) row is
+// contiguous along the summed letters in the ring's order (a row-oriented
+// digit split, and stores of the o^3 v^3 build along its contiguous axis).
+// The integral term is one permuted copy of the ovvo block, shared by the
+// stages.
+int CCBuilder::Wmbej_jbme(int t1, int t2) {
+  const int t2t1 = L(Terms().add(t2, 0.5).add(C("jf,nb->jnfb", t1, t1)));
+  const int ovvo = g_.materialize(Terms().addp(blk("ovvo"), {3, 1, 0, 2}).v);
+  return L(Terms()
+               .add(ovvo)
+               .add(C("mnej,nb->jbme", blk("oovo"), t1), -1.0)
+               .add(C("mnef,jnfb->jbme", blk("oovv"), t2t1), -1.0)
+               .add(C("mbef,jf->jbme", blk("ovvv"), t1)));
+}
+
 int CCBuilder::T2eq_rhs(int t1, int t2, int F) {  // CCSD_Helper.py:268-309
   const int fae = Fae(t1, t2, F), fmi = Fmi(t1, t2, F), fme = Fme(t1, t2, F);
-  const int wmnij = Wmnij(t1, t2, F), wmbej = Wmbej(t1, t2, F), wmnij_2 = Wmnij_2(t1, t2, F);
+  // TDCC_W_JBME=0: the ring reads Wmbej in its (m, b, e, j) layout
+  const char* wl = std::getenv("TDCC_W_JBME");
+  const bool jbme = !wl || std::atoi(wl) != 0;
+  const int wmnij = Wmnij(t1, t2, F), wmbej = jbme ? Wmbej_jbme(t1, t2) : Wmbej(t1, t2, F),
+            wmnij_2 = Wmnij_2(t1, t2, F);
   const int fvv = L(Terms().add(fae).add(C("me,mb->be", fme, t1), -0.5));
   const int foo = L(Terms().add(fmi).add(C("me,je->mj", fme, t1), 0.5));
   const int tt = t1t1(t1);
   // 'ma,nb->mnab' - 'na,mb->mnab'
   const int tau = L(Terms().add(tt).addp(tt, {1, 0, 2, 3}, -1.0));
   const int ring = L(Terms()
-                         .add(C("mbej,imae->ijab", wmbej, t2))
+                         .add(C(jbme ? "jbme,imae->ijab" : "mbej,imae->ijab", wmbej, t2))
                          .add(C3("mbej,ie,ma->ijab", blk("ovvo"), t1, t1), -1.0));
   // terms sharing an antisymmetrizer are summed first
   const int p23 = L(Terms().add(C("be,ijae->ijab", fvv, t2)).add(C("mbij,ma->ijab", blk("ovoo"), t1), -1.0));

@@ -49,6 +49,7 @@ class CCBuilder {
   int Wabef(int t1, int t2, int F);
   int Wabef_2(int t1, int t2, int F);
   int Wmbej(int t1, int t2, int F);
+  int Wmbej_jbme(int t1, int t2);  // Wmbej laid out (j, b, m, e) for the T2 ring
   int T2eq_rhs(int t1, int t2, int F);
   // CCSD Lambda side (CCSD_Helper.py:447-648)
   int LRFea(int t1, int t2, int F);

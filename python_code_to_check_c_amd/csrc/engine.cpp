@@ -513,7 +513,7 @@ int Graph::lincomb(std::vector<Term> terms) {
   std::vector<Term> flat;
   for (const Term& t : terms) {
     const Node& s = nodes_[t.src];
-    if (s.op == Op::Lincomb && flat.size() + s.terms.size() + (terms.size() - 1) <= (size_t)kInlineTerms) {
+    if (s.op == Op::Lincomb && !s.keep && flat.size() + s.terms.size() + (terms.size() - 1) <= (size_t)kInlineTerms) {
       for (const Term& u : s.terms) {
         Term c;
         c.src = u.src;
@@ -576,6 +576,16 @@ int Graph::lincomb(std::vector<Term> terms) {
     }
   n.terms = std::move(flat);
   n.key = k.str();
+  return add(std::move(n));
+}
+
+int Graph::materialize(std::vector<Term> terms) {
+  const int x = lincomb(std::move(terms));
+  if (nodes_[x].op != Op::Lincomb) return x;
+  if (nodes_[x].keep) return x;
+  Node n = nodes_[x];  // a node of its own (the plain one may be folded elsewhere)
+  n.keep = true;
+  n.key += "|keep";
   return add(std::move(n));
 }
 
@@ -1081,7 +1091,7 @@ void Graph::inline_lincombs(std::vector<char>& live) {
       for (size_t t = 0; t < L.terms.size() && !changed; ++t) {
         const int s = L.terms[t].src;
         const Node& S = nodes_[s];
-        if (S.op != Op::Lincomb || users[s] != 1 || user_of[s] != i) continue;
+        if (S.op != Op::Lincomb || S.keep || users[s] != 1 || user_of[s] != i) continue;
         // (a factor of a product term stays a tensor of its own)
         int refs = 0;
         bool factor = false;

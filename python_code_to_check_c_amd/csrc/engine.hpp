@@ -84,6 +84,8 @@ struct Node {
   int64_t scratch_len = 0;
   // stays FP64 when the runtime runs the large contractions in split precision
   bool fp64 = false;
+  // Lincomb: materialized (never folded into the linear combinations reading it)
+  bool keep = false;
   // Ladder: only the i < j rows are computed (input antisymmetric in (i, j))
   bool ij_pairs = false;
   // Contract: two summed letters p, q of which only the terms p < q are
@@ -189,6 +191,9 @@ class Graph {
   bool k_pairs() const { return k_pairs_; }
   int contract3(const std::string& spec, int a, int b, int c, double ar = 1.0);
   int lincomb(std::vector<Term> terms);
+  // a linear combination kept as its own buffer (a layout change read by several
+  // linear combinations, not folded into them)
+  int materialize(std::vector<Term> terms);
   int field(int stage);
   // Field and Trace nodes created from now on belong to trajectory b of a
   // batched program (-1: not batched).
