@@ -402,6 +402,15 @@ int CCBuilder::T2_vvvv_terms(int t1, int t2, int tau) {
   const int tau2 = L(Terms().add(tt).addp(tt, {0, 1, 3, 2}, -1.0));
   const int g2 = C("mnef,ijef->ijmn", oovv, tau);
   const int y2 = C("ijmn,mnab->ijab", g2, tau2);
+  if (g_.k_pairs()) {
+    // tau is antisymmetric in (a, b), so anti23(z, 1/4) = 1/4 g.tau, and
+    // 1/8 g.t2 + 1/4 g.tau = g.(1/8 t2 + 1/4 tau): one contraction (the
+    // summation order differs from the reference's two)
+    (void)y;
+    (void)z;
+    const int yz = C("ijmn,mnab->ijab", g, L(Terms().add(t2, 0.125).add(tau, 0.25)));
+    return L(Terms().add(ladder, 0.5).anti23(wq, -0.5).add(yz).add(y2, 0.125));
+  }
   return L(Terms().add(ladder, 0.5).anti23(wq, -0.5).add(y, 0.125).anti23(z, 0.25).add(y2, 0.125));
 }
 
