@@ -954,13 +954,16 @@ __device__ __forceinline__ void lds_copy16(const int8_t* src, v4i* dst) {
   asm volatile("s_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(src), "{m0}"(lds) : "memory");
 }
 
-template <int S>
+template <int S, int BM>
 __global__ __launch_bounds__(512, 1) void ozgemm8d_kernel(OzGemmArgs g) {
-  constexpr int BN = 128, BK = 32, NBUF = 3;
-  __shared__ v4i As[NBUF][S][OZ_BM * 2];
+  // BM = 128: waves 0-3 copy A, 4-7 copy B, wave tile 64 x 32 (two 32-row
+  // blocks); BM = 64 (A with few rows): waves 0-1 copy A, 2-5 copy B, 6-7
+  // none, wave tile 32 x 32
+  constexpr int BN = 128, BK = 32, NBUF = 3, IB = BM / 64;
+  __shared__ v4i As[NBUF][S][BM * 2];
   __shared__ v4i Bs[NBUF][S][BN * 2];
-  __shared__ double sA[OZ_BM], sB[BN];
-  const int tiles_m = (g.M + OZ_BM - 1) / OZ_BM, tiles_n = (g.N + BN - 1) / BN;
+  __shared__ double sA[BM], sB[BN];
+  const int tiles_m = (g.M + BM - 1) / BM, tiles_n = (g.N + BN - 1) / BN;
   const int tiles1 = tiles_m * tiles_n, tiles = tiles1 * g.nb;  // nb products one after another
   const int tid = threadIdx.x;
   const int kb = g.k_chunk > 0 ? (int)blockIdx.y * g.k_chunk : 0;
@@ -968,9 +971,12 @@ __global__ __launch_bounds__(512, 1) void ozgemm8d_kernel(OzGemmArgs g) {
   const int nk = (ke - kb + BK - 1) / BK;
   double* Cout = g.C + (int64_t)blockIdx.y * g.c_split_stride;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const bool on_b = wave >= 4;
+  constexpr int AW = BM / 32;  // waves copying A (2 chunks per row and plane)
+  const bool on_b = wave >= AW;
+  const bool copies = wave < AW + 4;
   const int lane = tid & 63;
-  const int pc = 64 * (wave & 3) + lane;                 // physical chunk this lane fills
+  const int cw = on_b ? wave - AW : wave;                // wave index within its operand
+  const int pc = 64 * cw + lane;                         // physical chunk this lane fills
   const int lrow = pc >> 1, lchunk = (pc & 1) ^ ((lrow >> 3) & 1);
   const int8_t* xbase = on_b ? g.B : g.A;
   int zb = 0;  // product of the current tile
@@ -991,23 +997,25 @@ __global__ __launch_bounds__(512, 1) void ozgemm8d_kernel(OzGemmArgs g) {
     Cout = g.C ? g.C + (int64_t)blockIdx.y * g.c_split_stride + zb * g.c_zstride : nullptr;
     int tm, tn;
     grouped_tile(L, tiles_m, tiles_n, g.tile_group, tm, tn);
-    m0 = tm * OZ_BM;
+    m0 = tm * BM;
     n0 = tn * BN;
-    xrow = on_b ? (int64_t)min(n0 + lrow, g.N - 1) * xrstride : (int64_t)min(m0 + lrow, g.M - 1) * xrstride;
+    xrow = on_b ? (int64_t)min(n0 + min(lrow, BN - 1), g.N - 1) * xrstride
+                : (int64_t)min(m0 + min(lrow, BM - 1), g.M - 1) * xrstride;
   };
   // copies K step j of the current tile into buffer j % 3: one 16-byte chunk
   // per lane per digit plane, written by the copy engine to LDS at the wave's
   // base + 16 * lane
   auto issue = [&](int j) {
+    if (!copies) return;
     const int buf = j % NBUF, k0 = kb + j * BK;
     const int8_t* src = xbase + xrow + (int64_t)k0 * kmul + lchunk * 16;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      v4i* dst = on_b ? &Bs[buf][s][64 * (wave & 3)] : &As[buf][s][64 * (wave & 3)];
+      v4i* dst = on_b ? &Bs[buf][s][64 * cw] : &As[buf][s][64 * cw];
       lds_copy16(src + s * xplane, dst);
     }
   };
-  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 32;
+  const int wm = (wave & 1) * (BM / 2), wn = (wave >> 1) * 32;
   const int lr = lane & 31, lh = lane >> 5;
   const double ar = g.alpha_re, ai = g.alpha_im;
 
@@ -1016,18 +1024,18 @@ __global__ __launch_bounds__(512, 1) void ozgemm8d_kernel(OzGemmArgs g) {
   for (;;) {
     // the tile's scales first (their loads complete before the copies start),
     // then the copies of its first two K steps
-    if (tid < OZ_BM)
+    if (tid < BM)
       sA[tid] = m0 + tid < g.M ? g.sa[(int64_t)blockIdx.y * g.sa_split_stride + zb * g.sa_zstride + m0 + tid] : 0.0;
-    else if (tid - OZ_BM < BN)
-      sB[tid - OZ_BM] = n0 + tid - OZ_BM < g.N
-          ? g.sb[(int64_t)blockIdx.y * g.sb_split_stride + zb * g.sb_zstride + n0 + tid - OZ_BM] : 0.0;
+    else if (tid - BM < BN)
+      sB[tid - BM] = n0 + tid - BM < g.N
+          ? g.sb[(int64_t)blockIdx.y * g.sb_split_stride + zb * g.sb_zstride + n0 + tid - BM] : 0.0;
     if (nk > 0) issue(0);
     if (nk > 1) issue(1);
-    v16i acc[S][2];
+    v16i acc[S][IB];
 #pragma unroll
     for (int q = 0; q < S; ++q)
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < IB; ++i)
 #pragma unroll
         for (int e = 0; e < 16; ++e) acc[q][i][e] = 0;
     for (int j = 0; j < nk; ++j) {
@@ -1044,7 +1052,7 @@ __global__ __launch_bounds__(512, 1) void ozgemm8d_kernel(OzGemmArgs g) {
       if (j + 2 < nk) issue(j + 2);
       const int buf = j % NBUF;
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
+      for (int i = 0; i < IB; ++i) {
         v4i a[S];
 #pragma unroll
         for (int s = 0; s < S; ++s) a[s] = As[buf][s][oz_swz2(wm + i * 32 + lr, lh)];
@@ -1063,7 +1071,7 @@ __global__ __launch_bounds__(512, 1) void ozgemm8d_kernel(OzGemmArgs g) {
     const int col = en0 + wn + lr;
     const bool colok = col < g.N;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < IB; ++i)
 #pragma unroll
       for (int gq = 0; gq < 4; ++gq) {
         const int row0 = em0 + wm + i * 32 + 8 * gq + 4 * lh;
@@ -1178,6 +1186,14 @@ bool ozgemm_direct() {
 
 bool ozgemm_interleaved_ok(int S) { return (S == 4 || S == 5) && ozgemm_direct(); }
 
+// The LDS-direct kernel's 64-row tile for an A operand of at most 64 rows
+// (TDCC_OZ_TILE64=0: always 128 rows; 2: every launch, a test mode).
+static bool oz_small_tile(const OzGemmArgs& g) {
+  const char* e = std::getenv("TDCC_OZ_TILE64");
+  const int m = e ? std::atoi(e) : 1;
+  return m == 2 || (m == 1 && g.M <= 64);
+}
+
 const char* ozgemm_kernel_name(int S) {
   if ((S == 4 || S == 5) && ozgemm_direct()) return "ozgemm8d_kernel";
   return S <= 4 ? "ozgemm8_kernel" : (S == 5 ? "ozgemm8n_kernel" : "ozgemm_kernel");
@@ -1200,8 +1216,18 @@ void launch_ozgemm(const OzGemmArgs& args, hipStream_t s) {
   if (g.tune_pb >= 0 && g.S <= 5) gx8 = (g.tune_pb > 0 && splits == 1 && g.tune_pb < tiles) ? g.tune_pb : tiles;
   const dim3 grid(tiles, splits), grid8(gx8, splits), block(256), block8(512);
   if (ozgemm_direct() && (g.S == 4 || g.S == 5)) {
-    if (g.S == 4) hipLaunchKernelGGL((ozgemm8d_kernel<4>), grid8, block8, 0, s, g);
-    else hipLaunchKernelGGL((ozgemm8d_kernel<5>), grid8, block8, 0, s, g);
+    if (oz_small_tile(g)) {
+      // one 64-row tile of A rows: half the MFMA work of a 128-row tile
+      const int t64 = ((g.M + 63) / 64) * ((g.N + 127) / 128) * g.nb;
+      int gx = (splits == 1 && pb > 0 && t64 > pb && g.Kp <= 8 * OZ_BK) ? pb : t64;
+      if (g.tune_pb >= 0) gx = (g.tune_pb > 0 && splits == 1 && g.tune_pb < t64) ? g.tune_pb : t64;
+      const dim3 grid64(gx, splits);
+      if (g.S == 4) hipLaunchKernelGGL((ozgemm8d_kernel<4, 64>), grid64, block8, 0, s, g);
+      else hipLaunchKernelGGL((ozgemm8d_kernel<5, 64>), grid64, block8, 0, s, g);
+      return;
+    }
+    if (g.S == 4) hipLaunchKernelGGL((ozgemm8d_kernel<4, 128>), grid8, block8, 0, s, g);
+    else hipLaunchKernelGGL((ozgemm8d_kernel<5, 128>), grid8, block8, 0, s, g);
     return;
   }
   switch (g.S) {

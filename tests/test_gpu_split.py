@@ -45,13 +45,16 @@ def bound(S):
 # pass per chunk (at large sizes only for rows longer than 16384).  "m3"
 # computes every complex x complex digit GEMM from three real products
 # (re re, im im, (re + im)(re + im); at large sizes only the single-split
-# ones above 512 x 512 complex outputs and 1e9 multiply-adds).
+# ones above 512 x 512 complex outputs and 1e9 multiply-adds).  "tile64"
+# runs every LDS-direct digit GEMM on 64-row tiles (at large sizes only those
+# with at most 64 A rows).
 MODES = {"default": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_NPAIR_MIN_K": "0"},
          "multipass": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_SPLIT_MULTIPASS": "1"},
          "stream": {"TDCC_STATIC_STREAM_MIN": "0"},
          "dot": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_DOT_MIN_K": "0"},
          "chunked": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_GEMM_SPLITS": "3", "TDCC_SPLIT_CHUNKED": "1"},
-         "m3": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_M3": "2"}}
+         "m3": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_M3": "2"},
+         "tile64": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_OZ_TILE64": "2"}}
 
 
 @pytest.fixture(scope="module", params=[(n, m) for m in MODES for n in ("o4v6", "c1_o10v4")],
@@ -61,7 +64,7 @@ def case(request):
     g = dict(np.load(os.path.join(HERE, "golden", f"golden_{name}.npz")))
     print(f"\n[split case {name} {mode}]", flush=True)
     keys = ("TDCC_GEMM_MIN_FLOPS", "TDCC_SPLIT_MULTIPASS", "TDCC_STATIC_STREAM_MIN", "TDCC_NPAIR_MIN_K",
-            "TDCC_DOT_MIN_K", "TDCC_GEMM_SPLITS", "TDCC_SPLIT_CHUNKED", "TDCC_M3")
+            "TDCC_DOT_MIN_K", "TDCC_GEMM_SPLITS", "TDCC_SPLIT_CHUNKED", "TDCC_M3", "TDCC_OZ_TILE64")
     saved = {k: os.environ.get(k) for k in keys}
     for k in keys:
         os.environ.pop(k, None)
@@ -184,6 +187,55 @@ def test_persistent_digit_gemm_is_bitwise_neutral():
             else:
                 os.environ[k] = val
     for (name, _), x, y in zip(RHS, out["8"], out["0"]):
+        assert np.array_equal(x, y), name
+
+
+@pytest.mark.parametrize("digits", [4, 5])
+def test_small_tile_digit_gemm_is_bitwise_neutral(digits):
+    """The LDS-direct digit GEMM takes 64-row tiles for A operands of at most
+    64 rows (TDCC_OZ_TILE64; 2: every launch, 0: always 128 rows).  The
+    integer sums and their FP64 assembly do not depend on the tile shape, so
+    at o = 8, v = 64 (every contraction on the GEMM path) both tilings give
+    bitwise equal residuals, for 4 and 5 digits, and with three real products
+    for the complex ones."""
+    from python_code_to_check_c_amd import synthetic
+    o, v = 8, 64
+    sysm = synthetic.SyntheticSystem(3, o, v, with_tei=False)
+    rng = np.random.default_rng(6)
+
+    def amp(shape):
+        x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+        return 1e-2 * x
+
+    t1, l1 = amp((o, v)), amp((o, v))
+    t2, l2 = amp((o, o, v, v)), amp((o, o, v, v))
+    for x in (t2, l2):
+        x -= x.transpose(1, 0, 2, 3)
+        x -= x.transpose(0, 1, 3, 2)
+    Fc = sysm.F + 0j
+    out = {}
+    keys = ("TDCC_OZ_TILE64", "TDCC_GEMM_MIN_FLOPS", "TDCC_M3")
+    saved = {k: os.environ.get(k) for k in keys}
+    try:
+        os.environ["TDCC_GEMM_MIN_FLOPS"] = "0"
+        os.environ["TDCC_M3"] = "2"
+        for mode in ("2", "0"):
+            os.environ["TDCC_OZ_TILE64"] = mode
+            c = _lib.Context(o, v)
+            try:
+                c.set_system(None, np.asarray(sysm.F), np.asarray(sysm.mu))
+                c.synth_tei(3, 0.05)
+                c.set_precision(digits)
+                out[mode] = [c.rhs(kind, Fc, t1, t2, l1, l2) for _, kind in RHS]
+            finally:
+                c.close()
+    finally:
+        for k, val in saved.items():
+            if val is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = val
+    for (name, _), x, y in zip(RHS, out["2"], out["0"]):
         assert np.array_equal(x, y), name
 
 
