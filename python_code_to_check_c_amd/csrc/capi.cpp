@@ -420,6 +420,7 @@ Program& step_program(tdcc_ctx* c, int method, double dt) {
     io.base[k] = mark_amplitude(*g, ext_planes(*g, c->base[k], shape_of_block(k), "base"), k, c->pairs);
   }
   for (int k = 0; k < 4; ++k) io.ynew[k] = ext_planes(*g, c->state[k], shape_of_block(k), "ynew");
+  g->set_lambda_externals({io.y[2], io.y[3], io.base[2], io.base[3]});
   build_rk4_step(*g, io, method, dt);
   return finish_program(c, key, std::move(g), true);
 }
@@ -461,8 +462,10 @@ Program& batch_step_program(tdcc_ctx* c, int method, double dt) {
       io.base[k] = mark_amplitude(*g, ext(c->bbase[0][k], k, "base"), k, pairs);
     }
     for (int k = 0; k < 4; ++k) io.ynew[k] = ext(c->bstate[0][k], k, "ynew");
+    g->set_lambda_externals({io.y[2], io.y[3], io.base[2], io.base[3]});
     build_rk4_step(*g, io, method, dt);
   } else {
+    std::vector<int> lam;
     for (int b = 0; b < c->nb; ++b) {
       g->set_traj(b);
       StepIO io;
@@ -472,9 +475,11 @@ Program& batch_step_program(tdcc_ctx* c, int method, double dt) {
         io.base[k] = mark_amplitude(*g, ext_planes(*g, c->bbase[b][k], shape_of_block(k), "base"), k, pairs);
       }
       for (int k = 0; k < 4; ++k) io.ynew[k] = ext_planes(*g, c->bstate[b][k], shape_of_block(k), "ynew");
+      for (int k = 2; k < 4; ++k) lam.insert(lam.end(), {io.y[k], io.base[k]});
       build_rk4_step(*g, io, method, dt);
     }
     g->set_traj(-1);
+    g->set_lambda_externals(lam);
   }
   Program& p = finish_program(c, key, std::move(g), true);
   p.batched = true;

@@ -128,6 +128,10 @@ Graph::~Graph() {
   if (dev_tables_) (void)hipFree(dev_tables_);
   if (ones_buf_) (void)hipFree(ones_buf_);
   if (cache_buf_) (void)hipFree(cache_buf_);
+  for (hipEvent_t e : events_) (void)hipEventDestroy(e);
+  if (fork_ev_) (void)hipEventDestroy(fork_ev_);
+  if (join_ev_) (void)hipEventDestroy(join_ev_);
+  if (sb_) (void)hipStreamDestroy(sb_);
 }
 
 int64_t Graph::size(const Shape& s) const {
@@ -872,6 +876,77 @@ bool Graph::is_static(int id) const {
   return n.op == Op::External && (n.key.rfind("X|tei|", 0) == 0 || n.key.rfind("X|ones|", 0) == 0);
 }
 
+static bool two_streams_on() {
+  const char* e = std::getenv("TDCC_STREAMS");
+  return !e || std::atoi(e) != 1;
+}
+
+static bool is_post_op(Op op) {
+  return op == Op::Finalize || op == Op::Trace || op == Op::Assign || op == Op::FieldVjp || op == Op::TraceVjp;
+}
+
+// Sides of a two-stream step (see set_lambda_externals): a node depending on
+// a Lambda external is on the Lambda side; a node that does not is on the
+// Lambda side when the Lambda side reads it and the T side does not (the
+// finalize and what follows it do not count), else on the T side.  A T side
+// node the Lambda side reads is a crossing.
+void Graph::compute_sides() {
+  side_.clear();
+  cross_.clear();
+  bool have_fin = false;
+  for (int i = 0; i < (int)nodes_.size(); ++i) have_fin = have_fin || (live_[i] && nodes_[i].op == Op::Finalize);
+  if (lam_ext_.empty() || !have_fin || !two_streams_on()) return;
+  const int N = (int)nodes_.size();
+  auto root = [&](int x) {
+    while (nodes_[x].op == Op::View) x = nodes_[x].base;
+    return x;
+  };
+  std::vector<char> lam(N, 0);
+  for (int x : lam_ext_) lam[x] = 1;
+  std::vector<int> in;
+  for (int i = 0; i < N; ++i) {
+    const Node& n = nodes_[i];
+    if (n.op == Op::External) continue;
+    inputs_of(n, in);
+    for (int x : in) lam[i] = lam[i] || lam[x];
+  }
+  side_.assign(N, 0);
+  cross_.assign(N, 0);
+  std::vector<char> has_user(N, 0), all_lam(N, 1);
+  for (int i = N - 1; i >= 0; --i) {
+    const Node& n = nodes_[i];
+    if (!live_[i] || n.op == Op::View || n.op == Op::External) continue;
+    if (is_post_op(n.op)) {
+      side_[i] = 2;
+      continue;
+    }
+    side_[i] = lam[i] ? 1 : ((has_user[i] && all_lam[i]) ? 1 : 0);
+    inputs_of(n, in);
+    for (int x : in) {
+      const int r = root(x);
+      has_user[r] = 1;
+      if (side_[i] != 1) all_lam[r] = 0;
+    }
+  }
+  // nodes after the first finalize run on the first stream after the join
+  int first_fin = N;
+  for (int i = 0; i < N; ++i)
+    if (live_[i] && side_[i] == 2) {
+      first_fin = i;
+      break;
+    }
+  for (int i = first_fin; i < N; ++i)
+    if (side_[i] == 1) side_[i] = 0;
+  for (int i = 0; i < N; ++i) {
+    if (!live_[i] || side_[i] != 1) continue;
+    inputs_of(nodes_[i], in);
+    for (int x : in) {
+      const int r = root(x);
+      if (nodes_[r].op != Op::External && side_[r] == 0) cross_[r] = 1;
+    }
+  }
+}
+
 void Graph::plan() {
   const int N = (int)nodes_.size();
   cache_.clear();
@@ -892,6 +967,7 @@ void Graph::plan() {
   inline_lincombs(live);
   live_ = live;
   mark_stack();
+  compute_sides();
   // liveness: uses of views count as uses of their base
   auto root = [&](int x) {
     while (nodes_[x].op == Op::View) x = nodes_[x].base;
@@ -906,10 +982,15 @@ void Graph::plan() {
       nodes_[r].last_use = std::max(nodes_[r].last_use, i);
     }
   }
-  // arena layout: first fit over a coalescing free list
-  std::map<int64_t, int64_t> free_blocks;  // offset -> bytes
-  int64_t top = 0;
+  // arena layout: first fit over a coalescing free list per region (one
+  // region in a one-stream program; in a two-stream step one per stream, each
+  // planned over that stream's launch order, and one never reused within the
+  // step for the T side buffers the Lambda side reads)
+  std::map<int64_t, int64_t> free_lists[3];  // offset -> bytes
+  int64_t tops[3] = {0, 0, 0};
+  int cur_region = 0;
   auto alloc = [&](int64_t bytes) -> int64_t {
+    std::map<int64_t, int64_t>& free_blocks = free_lists[cur_region];
     bytes = (bytes + 255) & ~int64_t(255);
     for (auto it = free_blocks.begin(); it != free_blocks.end(); ++it) {
       if (it->second >= bytes) {
@@ -920,11 +1001,13 @@ void Graph::plan() {
         return off;
       }
     }
-    const int64_t off = top;
-    top += bytes;
+    const int64_t off = tops[cur_region];
+    tops[cur_region] += bytes;
     return off;
   };
   auto release = [&](int64_t off, int64_t bytes) {
+    if (cur_region == 2) return;  // crossing buffers live through the step
+    std::map<int64_t, int64_t>& free_blocks = free_lists[cur_region];
     bytes = (bytes + 255) & ~int64_t(255);
     auto it = free_blocks.emplace(off, bytes).first;
     auto nx = std::next(it);
@@ -960,19 +1043,46 @@ void Graph::plan() {
     share_at[shares_[k].first].push_back(k);
     share_end[shares_[k].last].push_back(k);
   }
+  const bool two = !side_.empty();
+  auto side_region = [&](int i) { return two && side_[i] == 1 ? 1 : 0; };
+  auto buf_region = [&](int i) { return two && cross_[i] ? 2 : side_region(i); };
+  std::vector<int> node_region(N, 0), share_region(shares_.size(), 0);
   for (int i = 0; i < N; ++i) {
     if (!live[i]) continue;
     Node& n = nodes_[i];
-    if (owns_buffer(n.op)) n.arena_off = alloc(zbytes(n) * (n.zdep ? nz_ : 1));
-    for (int k : share_at[i]) shares_[k].off = alloc(shares_[k].bytes);
+    if (owns_buffer(n.op)) {
+      cur_region = node_region[i] = buf_region(i);
+      n.arena_off = alloc(zbytes(n) * (n.zdep ? nz_ : 1));
+    }
+    cur_region = side_region(i);
+    for (int k : share_at[i]) {
+      share_region[k] = cur_region;
+      shares_[k].off = alloc(shares_[k].bytes);
+    }
     if ((n.op == Op::Contract || n.op == Op::Ladder) && n.scratch_len > 0) {
       n.scratch_off = alloc(n.scratch_len);
       release(n.scratch_off, n.scratch_len);
+      node_region[i] |= cur_region << 4;  // scratch region
     }
-    for (int k : share_end[i]) release(shares_[k].off, shares_[k].bytes);
-    for (int x : ends[i]) release(nodes_[x].arena_off, zbytes(nodes_[x]) * (nodes_[x].zdep ? nz_ : 1));
+    for (int k : share_end[i]) {
+      cur_region = share_region[k];
+      release(shares_[k].off, shares_[k].bytes);
+    }
+    for (int x : ends[i]) {
+      cur_region = node_region[x] & 15;
+      release(nodes_[x].arena_off, zbytes(nodes_[x]) * (nodes_[x].zdep ? nz_ : 1));
+    }
   }
-  arena_bytes_ = top;
+  // regions laid out one after another
+  const int64_t base_of[3] = {0, tops[0], tops[0] + tops[1]};
+  for (int i = 0; i < N; ++i) {
+    if (!live[i]) continue;
+    Node& n = nodes_[i];
+    if (owns_buffer(n.op)) n.arena_off += base_of[node_region[i] & 15];
+    if ((n.op == Op::Contract || n.op == Op::Ladder) && n.scratch_len > 0) n.scratch_off += base_of[node_region[i] >> 4];
+  }
+  for (size_t k = 0; k < shares_.size(); ++k) shares_[k].off += base_of[share_region[k]];
+  arena_bytes_ = tops[0] + tops[1] + tops[2];
   if (const char* d = std::getenv("TDCC_DUMP_NODES"); d && d[0] == '1') dump_nodes();
   if (dev_tables_) {
     (void)hipFree(dev_tables_);
@@ -1181,7 +1291,12 @@ void Graph::plan_shares() {
       if (w > lo && w < hi) return true;
     return false;
   };
-  for (auto& [key, uses] : dyn_uses_) {
+  for (auto& [key, all_uses] : dyn_uses_) {
+   // a two-stream step shares digit planes only within one side
+   for (int sd = 0; sd < (side_.empty() ? 1 : 2); ++sd) {
+    std::vector<DynUse> uses;
+    for (const DynUse& u : all_uses)
+      if (side_.empty() || (side_[u.node] == 1 ? 1 : 0) == sd) uses.push_back(u);
     size_t g0 = 0;
     while (g0 < uses.size()) {
       size_t g1 = g0 + 1;
@@ -1201,6 +1316,7 @@ void Graph::plan_shares() {
       }
       g0 = g1;
     }
+   }
   }
 }
 
@@ -1826,6 +1942,7 @@ void Graph::bind(double* arena) {
   for (int i = 0; i < N; ++i) {
     if (!live_[i]) continue;
     const Node& n = nodes_[i];
+    const size_t nl0 = launches_.size();
     switch (n.op) {
       case Op::Contract: {
         auto L0 = std::move(gett.at(gi++));
@@ -2406,6 +2523,7 @@ void Graph::bind(double* arena) {
       }
       default: break;
     }
+    for (size_t j = nl0; j < launches_.size(); ++j) launches_[j]->node = i;
   }
   if (cache_launched) {
     TDCC_HIP(hipGetLastError());
@@ -2417,6 +2535,57 @@ void Graph::bind(double* arena) {
   const char* at = std::getenv("TDCC_AUTOTUNE");
   const int mode = at ? std::atoi(at) : (nz_ > 1 ? 1 : 0);
   if (mode > 0) autotune(mode >= 2);
+  plan_streams();
+}
+
+// Two-stream step: the stream of every launch, the crossing events each
+// Lambda side launch waits for first, and the event a T side launch records
+// once the last launch of a crossing node is queued.
+void Graph::plan_streams() {
+  const size_t NL = launches_.size();
+  lstream_.assign(NL, 0);
+  lrecord_.assign(NL, -1);
+  lwait_.assign(NL, {});
+  first_post_ = -1;
+  if (side_.empty()) return;
+  auto root = [&](int x) {
+    while (nodes_[x].op == Op::View) x = nodes_[x].base;
+    return x;
+  };
+  std::map<int, int> event_of;  // crossing node -> event
+  std::map<int, size_t> last_launch;
+  for (size_t j = 0; j < NL; ++j) {
+    const int nd = launches_[j]->node;
+    if (nd < 0) throw Error("internal: launch without node");
+    lstream_[j] = side_[nd] == 1 ? 1 : 0;
+    if (first_post_ < 0 && side_[nd] == 2) first_post_ = (int)j;
+    if (cross_[nd]) {
+      if (!event_of.count(nd)) event_of.emplace(nd, (int)event_of.size());
+      last_launch[nd] = j;
+    }
+  }
+  for (auto& [nd, j] : last_launch) lrecord_[j] = event_of.at(nd);
+  std::vector<char> waited(event_of.size(), 0);
+  std::vector<int> in;
+  for (size_t j = 0; j < NL; ++j) {
+    if (!lstream_[j]) continue;
+    inputs_of(nodes_[launches_[j]->node], in);
+    for (int x : in) {
+      auto it = event_of.find(root(x));
+      if (it == event_of.end() || waited[it->second]) continue;
+      if (last_launch.at(it->first) > j) throw Error("internal: crossing read before it is written");
+      waited[it->second] = 1;
+      lwait_[j].push_back(it->second);
+    }
+  }
+  if (!sb_) TDCC_HIP(hipStreamCreateWithFlags(&sb_, hipStreamNonBlocking));
+  if (!fork_ev_) TDCC_HIP(hipEventCreateWithFlags(&fork_ev_, hipEventDisableTiming));
+  if (!join_ev_) TDCC_HIP(hipEventCreateWithFlags(&join_ev_, hipEventDisableTiming));
+  while (events_.size() < event_of.size()) {
+    hipEvent_t e = nullptr;
+    TDCC_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    events_.push_back(e);
+  }
 }
 
 namespace {
@@ -2584,8 +2753,27 @@ static void run_ladder(const LadderArgs& a, hipStream_t s) {
   ladder_unpack(a, s);
 }
 
-void Graph::run(hipStream_t s) const {
-  for (const auto& L : launches_) {
+void Graph::run(hipStream_t s0) const {
+  // two-stream step: fork the second stream off the caller's, and join it
+  // back before the first finalize (or at the end)
+  bool joined = side_.empty() || !sb_;
+  if (!joined) {
+    TDCC_HIP(hipEventRecord(fork_ev_, s0));
+    TDCC_HIP(hipStreamWaitEvent(sb_, fork_ev_, 0));
+  }
+  auto join = [&] {
+    TDCC_HIP(hipEventRecord(join_ev_, sb_));
+    TDCC_HIP(hipStreamWaitEvent(s0, join_ev_, 0));
+    joined = true;
+  };
+  for (size_t j = 0; j < launches_.size(); ++j) {
+    const auto& L = launches_[j];
+    if (!joined && (int)j == first_post_) join();
+    hipStream_t s = s0;
+    if (!joined) {
+      for (int e : lwait_[j]) TDCC_HIP(hipStreamWaitEvent(sb_, events_[e], 0));
+      if (lstream_[j]) s = sb_;
+    }
     switch (L->kind) {
       case Launch::GETT:
         if (L->g.M == 0 || L->g.N == 0) break;
@@ -2624,7 +2812,9 @@ void Graph::run(hipStream_t s) const {
       case Launch::SEED: launch_seed(L->sd, s); break;
       case Launch::TVJP: launch_trace_vjp(L->tv, s); break;
     }
+    if (!joined && lrecord_[j] >= 0) TDCC_HIP(hipEventRecord(events_[lrecord_[j]], s0));
   }
+  if (!joined) join();
   TDCC_HIP(hipGetLastError());
 }
 

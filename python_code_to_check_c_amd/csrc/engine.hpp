@@ -260,6 +260,12 @@ class Graph {
   // file TDCC_AUTOTUNE_FILE names.
   void autotune(bool k_splits);
   void run(hipStream_t s) const;
+  // Step programs: the externals holding the Lambda amplitudes.  The nodes
+  // that depend on them, and those only they use, run on a second stream
+  // concurrently with the T side (the Lambda chains read the T amplitudes of
+  // the step's start, which nothing writes before the finalize), joined before
+  // the finalize; TDCC_STREAMS=1 keeps one stream.
+  void set_lambda_externals(std::vector<int> ext) { lam_ext_ = std::move(ext); }
 
   // One eager pass with a HIP event pair around every kernel launch.
   struct LaunchRecord {
@@ -329,6 +335,24 @@ class Graph {
   int64_t cache_bytes_ = 0;
   char* cache_buf_ = nullptr;
   double* ones_buf_ = nullptr;
+  // two-stream step (set_lambda_externals): side of every live node (0: T
+  // side, stream of run(); 1: Lambda side, second stream; 2: finalize and
+  // after, on the first stream once the second has joined), and whether a T
+  // side node is read by the Lambda side (its buffer is then never reused
+  // within the step)
+  std::vector<int> lam_ext_;
+  std::vector<int> side_;
+  std::vector<char> cross_;
+  void compute_sides();
+  void plan_streams();  // at bind, from the sides
+  // launch schedule: stream of each launch, events to wait on before it,
+  // event to record after it, index of the first launch after the join
+  std::vector<int> lstream_, lrecord_;
+  std::vector<std::vector<int>> lwait_;
+  int first_post_ = -1;
+  hipStream_t sb_ = nullptr;
+  std::vector<hipEvent_t> events_;
+  hipEvent_t fork_ev_ = nullptr, join_ev_ = nullptr;
 };
 
 constexpr uint8_t anti_bit(int d1, int d2) {

@@ -254,3 +254,65 @@ def test_rejects_bad_digit_count(case):
     for bad in (1, 7, -1):
         with pytest.raises(_lib.TdccError):
             ctx.set_precision(bad)
+
+
+@pytest.mark.parametrize("digits", [0, 5])
+@pytest.mark.parametrize("batch", ["single", "stacked", "side_by_side"])
+def test_two_stream_step_is_bitwise_neutral(digits, batch):
+    """An RK4 step runs its T side and its Lambda side (the Lambda chains read
+    the step-start t) on two streams joined before the update, with the
+    buffers the Lambda side reads from the T side held through the step;
+    TDCC_STREAMS=1 keeps one stream.  The same kernels compute the same
+    values in either order, so three steps of CCSD give bitwise equal
+    amplitudes and dipoles, in FP64 and with 5 digits, for one trajectory and
+    for a batch of two (stacked and side by side)."""
+    from python_code_to_check_c_amd import synthetic
+    o, v = 8, 64
+    sysm = synthetic.SyntheticSystem(3, o, v, with_tei=False)
+    rng = np.random.default_rng(11)
+
+    def amp(shape):
+        return 1e-2 * (rng.standard_normal(shape) + 1j * rng.standard_normal(shape))
+
+    states = []
+    for _ in range(2):
+        t1, l1 = amp((o, v)), amp((o, v))
+        t2, l2 = amp((o, o, v, v)), amp((o, o, v, v))
+        for x in (t2, l2):
+            x -= x.transpose(1, 0, 2, 3)
+            x -= x.transpose(0, 1, 3, 2)
+        states.append((t1, t2, l1, l2))
+    out = {}
+    keys = ("TDCC_STREAMS", "TDCC_GEMM_MIN_FLOPS", "TDCC_STACK")
+    saved = {k: os.environ.get(k) for k in keys}
+    try:
+        os.environ["TDCC_GEMM_MIN_FLOPS"] = "0"
+        os.environ["TDCC_STACK"] = "0" if batch == "side_by_side" else "1"
+        for mode in ("2", "1"):
+            os.environ["TDCC_STREAMS"] = mode
+            c = _lib.Context(o, v)
+            try:
+                c.set_system(None, np.asarray(sysm.F), np.asarray(sysm.mu))
+                c.synth_tei(3, 0.05)
+                c.set_precision(digits)
+                if batch == "single":
+                    c.set_state(*states[0])
+                    mu, _ = c.rk4_run("CCSD", 3, 0.0, 0.05, 0.3, 0.02)
+                    out[mode] = [mu, *c.get_state()]
+                else:
+                    c.set_batch(2)
+                    for b in range(2):
+                        c.set_state_slot(b, *states[b])
+                    mu, _ = c.rk4_run_batch("CCSD", 3, 0.0, 0.05, [2, 0], [0.3, 0.25], [0.02, 0.03])
+                    out[mode] = [mu] + [x for b in range(2) for x in c.get_state_slot(b)]
+            finally:
+                c.close()
+    finally:
+        for k, val in saved.items():
+            if val is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = val
+    assert np.abs(out["2"][0]).max() > 0
+    for x, y in zip(out["2"], out["1"]):
+        assert np.array_equal(x, y)
