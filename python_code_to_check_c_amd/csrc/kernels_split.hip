@@ -355,6 +355,13 @@ int64_t ozaki_split_workspace(int64_t R, int64_t Kp, bool by_columns) {
   return ((Kp + tk - 1) / tk) * R * (int64_t)sizeof(double);
 }
 
+// Column-oriented short-K operands up to this many rows (planes counted)
+// take the one-wave-per-row kernel; TDCC_WAVE_COLS_MAX_ROWS overrides 2048.
+static int64_t wave_cols_max_rows() {
+  const char* e = std::getenv("TDCC_WAVE_COLS_MAX_ROWS");
+  return e ? std::atoll(e) : 2048;
+}
+
 template <int S>
 static void split_launch(const SplitArgs& a, bool by_columns, hipStream_t s) {
   const int64_t R = (int64_t)a.planes * a.Rb;
@@ -363,8 +370,7 @@ static void split_launch(const SplitArgs& a, bool by_columns, hipStream_t s) {
   // here too: 64-row column tiles would give them a handful of blocks whose
   // serial loads dominate, while the whole operand is small enough that
   // uncoalesced loads cost little.
-  constexpr int64_t kWaveColsMaxRows = 2048;
-  if (a.sk_chunk == 0 && !a.multipass && a.Kp <= 256 && (!by_columns || R <= kWaveColsMaxRows)) {
+  if (a.sk_chunk == 0 && !a.multipass && a.Kp <= 256 && (!by_columns || R <= wave_cols_max_rows())) {
     hipLaunchKernelGGL(split_rows_wave_kernel<S>, dim3((unsigned)((R + 3) / 4)), block, 0, s, a);
     return;
   }

@@ -47,14 +47,17 @@ def bound(S):
 # (re re, im im, (re + im)(re + im); at large sizes only the single-split
 # ones above 512 x 512 complex outputs and 1e9 multiply-adds).  "tile64"
 # runs every LDS-direct digit GEMM on 64-row tiles (at large sizes only those
-# with at most 64 A rows).
+# with at most 64 A rows).  "cols_resident" sends short-K column-oriented
+# splits to the 64-row register-resident kernel whatever their row count (at
+# large sizes only those above 2048 rows; below, one wave per row).
 MODES = {"default": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_NPAIR_MIN_K": "0"},
          "multipass": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_SPLIT_MULTIPASS": "1"},
          "stream": {"TDCC_STATIC_STREAM_MIN": "0"},
          "dot": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_DOT_MIN_K": "0"},
          "chunked": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_GEMM_SPLITS": "3", "TDCC_SPLIT_CHUNKED": "1"},
          "m3": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_M3": "2"},
-         "tile64": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_OZ_TILE64": "2"}}
+         "tile64": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_OZ_TILE64": "2"},
+         "cols_resident": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_WAVE_COLS_MAX_ROWS": "0"}}
 
 
 @pytest.fixture(scope="module", params=[(n, m) for m in MODES for n in ("o4v6", "c1_o10v4")],
@@ -64,7 +67,8 @@ def case(request):
     g = dict(np.load(os.path.join(HERE, "golden", f"golden_{name}.npz")))
     print(f"\n[split case {name} {mode}]", flush=True)
     keys = ("TDCC_GEMM_MIN_FLOPS", "TDCC_SPLIT_MULTIPASS", "TDCC_STATIC_STREAM_MIN", "TDCC_NPAIR_MIN_K",
-            "TDCC_DOT_MIN_K", "TDCC_GEMM_SPLITS", "TDCC_SPLIT_CHUNKED", "TDCC_M3", "TDCC_OZ_TILE64")
+            "TDCC_DOT_MIN_K", "TDCC_GEMM_SPLITS", "TDCC_SPLIT_CHUNKED", "TDCC_M3", "TDCC_OZ_TILE64",
+            "TDCC_WAVE_COLS_MAX_ROWS")
     saved = {k: os.environ.get(k) for k in keys}
     for k in keys:
         os.environ.pop(k, None)
