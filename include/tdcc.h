@@ -86,11 +86,32 @@ int tdcc_set_state(tdcc_ctx* ctx, const double* t1, const double* t2, const doub
  * CCSD_Helper.py:1207-1214, :1243). */
 int tdcc_get_state(tdcc_ctx* ctx, int which, double* t1, double* t2, double* l1, double* l2);
 
+/* Device-memory variants of tdcc_set_state / tdcc_get_state: t1, t2, lam1,
+ * lam2 are DEVICE pointers (interleaved complex in C order, the memory of a
+ * complex128 tensor) on the context's GPU; there is no host copy.  The
+ * (i, j) antisymmetry test that picks the i < j shortcuts runs on the GPU with
+ * the host test's bound.  Inputs must be complete when the call is made (the
+ * caller synchronizes its own stream first); the call returns after its
+ * results are written.  A host pointer is rejected with an error.  Same
+ * state as the reference's Runge_Kutta_solver arguments
+ * (CCSD_Helper.py:1191) and its saved t1min/t2min/L1min/L2min (:1207-1214). */
+int tdcc_set_state_device(tdcc_ctx* ctx, const double* t1, const double* t2, const double* l1,
+                          const double* l2);
+int tdcc_get_state_device(tdcc_ctx* ctx, int which, double* t1, double* t2, double* l1, double* l2);
+
 /* One right-hand side at the given complex F (n,n); out has the shape of the
  * residual (o,v) or (o,o,v,v).  Unused amplitude arguments may be NULL only
  * for the kinds that do not read them (T1/T2 kinds ignore lam1/lam2). */
 int tdcc_rhs(tdcc_ctx* ctx, int kind, const double* F, const double* t1, const double* t2,
              const double* l1, const double* l2, double* out);
+/* tdcc_rhs on DEVICE pointers (F, amplitudes and out as for
+ * tdcc_set_state_device): the residual functions T1eq_rhs, T2eq_rhs,
+ * lam_1eq_rhs, lam2eq_rhs (CCSD_Helper.py:177, :268, :555, :610) and the CC2
+ * kinds T2eq_rhs_CC2, L1eq_rhs_CC2, L2eq_rhs_CC2 (CC2_Helper.py:12, :50,
+ * :128) with their arguments and result in GPU memory; bit for bit the result
+ * of tdcc_rhs. */
+int tdcc_rhs_device(tdcc_ctx* ctx, int kind, const double* F, const double* t1, const double* t2,
+                    const double* l1, const double* l2, double* out);
 
 /* One named intermediate of the reference's residuals, evaluated with the
  * same graph builders the residuals use (parity checking): "Fae", "Fmi",

@@ -21,6 +21,7 @@ EXPORTS = (
     "tdcc_dipole_gradient", "tdcc_write_restart_block", "tdcc_io_error",
     "tdcc_dipole_device_noalloc", "tdcc_read_restart_block", "tdcc_write_state_file",
     "tdcc_read_state_file_header", "tdcc_read_state_file",
+    "tdcc_set_state_device", "tdcc_get_state_device", "tdcc_rhs_device",
 )
 
 RHS_T1, RHS_T2, RHS_L1, RHS_L2, RHS_T2_CC2, RHS_L1_CC2, RHS_L2_CC2 = range(7)
@@ -83,6 +84,9 @@ def load():
                                                        ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                                        ctypes.POINTER(ctypes.c_int), _dp]),
         "tdcc_read_state_file": (ctypes.c_int, [ctypes.c_char_p, _dp, _dp, _dp, _dp, _dp]),
+        "tdcc_set_state_device": (ctypes.c_int, [vp, vp, vp, vp, vp]),
+        "tdcc_get_state_device": (ctypes.c_int, [vp, ctypes.c_int, vp, vp, vp, vp]),
+        "tdcc_rhs_device": (ctypes.c_int, [vp, ctypes.c_int, vp, vp, vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -90,6 +94,67 @@ def load():
         f.argtypes = args
     _lib = lib
     return lib
+
+
+def _require_one_hip_runtime():
+    """GPU torch tensors are usable only when the library runs on torch's HIP
+    runtime: torch (with its GPU initialized) must be loaded before the first
+    Context, so that the library binds to torch's runtime instead of loading
+    the system one beside it."""
+    with open("/proc/self/maps") as f:
+        runtimes = {ln.split()[-1] for ln in f if "libamdhip64" in ln}
+    if len(runtimes) != 1:
+        raise TdccError("GPU torch tensors need the library on torch's HIP runtime; two are loaded "
+                        f"({sorted(runtimes)}): import torch and initialize its GPU before the first "
+                        "Context, or pass DeviceArray buffers")
+
+
+class DeviceArray:
+    """Device memory from the library's own HIP runtime holding an array
+    (complex128 amplitudes and F for the *_device entry points), for callers
+    without torch."""
+
+    def __init__(self, shape, dtype=np.complex128):
+        lib = load()
+        lib.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        lib.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        lib.hipFree.argtypes = [ctypes.c_void_p]
+        self._lib = lib
+        self.shape = tuple(int(n) for n in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self._p = ctypes.c_void_p()
+        if lib.hipMalloc(ctypes.byref(self._p), max(self.nbytes, 1)) != 0:
+            raise TdccError(f"hipMalloc of {self.nbytes} bytes failed")
+
+    @classmethod
+    def from_numpy(cls, a, dtype=np.complex128):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        d = cls(a.shape, dtype)
+        if d._lib.hipMemcpy(d._p, a.ctypes.data, d.nbytes, 1) != 0:  # host to device
+            raise TdccError("hipMemcpy to the device failed")
+        return d
+
+    def to_numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        if self._lib.hipMemcpy(out.ctypes.data, self._p, self.nbytes, 2) != 0:  # device to host
+            raise TdccError("hipMemcpy from the device failed")
+        return out
+
+    @property
+    def ptr(self):
+        return self._p.value
+
+    def free(self):
+        if self._p.value:
+            self._lib.hipFree(self._p)
+            self._p = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def _c(x):
@@ -113,6 +178,7 @@ class Context:
 
     def __init__(self, nocc, nvir, device=0):
         self.lib = load()
+        self.device = int(device)
         self.o, self.v, self.n = int(nocc), int(nvir), int(nocc) + int(nvir)
         h = ctypes.c_void_p()
         if self.lib.tdcc_create(self.o, self.v, int(device), ctypes.byref(h)) != 0 or not h.value:
@@ -172,6 +238,84 @@ class Context:
         self._shape_ok(a, b)
         self._shape_ok(c, d)
         self._check(self.lib.tdcc_set_state(self._h, pa, pb, pc, pd), "tdcc_set_state")
+
+    # ------------------------------------------------- device-resident arrays
+    def _dev(self, x, shape, what):
+        """Device address of `x`: a DeviceArray, or a contiguous complex128 GPU
+        torch tensor on the context's GPU (torch's stream synchronized first),
+        of the given shape."""
+        if isinstance(x, DeviceArray):
+            if x.shape != tuple(shape) or x.dtype != np.complex128:
+                raise ValueError(f"{what} is a {x.dtype} DeviceArray of shape {x.shape}, "
+                                 f"expected complex128 {tuple(shape)}")
+            return ctypes.c_void_p(x.ptr)
+        import torch
+        if not isinstance(x, torch.Tensor) or not x.is_cuda:
+            raise TypeError(f"{what} must be a DeviceArray or a GPU torch tensor")
+        if x.dtype != torch.complex128:
+            raise TypeError(f"{what} must be complex128, got {x.dtype}")
+        if tuple(x.shape) != tuple(shape):
+            raise ValueError(f"{what} has shape {tuple(x.shape)}, expected {tuple(shape)}")
+        if not x.is_contiguous():
+            raise ValueError(f"{what} must be contiguous")
+        if x.device.index != self.device:
+            raise ValueError(f"{what} is on cuda:{x.device.index}, the context on cuda:{self.device}")
+        _require_one_hip_runtime()
+        torch.cuda.current_stream(x.device).synchronize()
+        return ctypes.c_void_p(x.data_ptr())
+
+    def _empty_like(self, ref, shape):
+        if isinstance(ref, DeviceArray):
+            return DeviceArray(shape), None
+        import torch
+        t = torch.empty(shape, dtype=torch.complex128, device=ref.device)
+        return t, ctypes.c_void_p(t.data_ptr())
+
+    def _shapes(self):
+        o, v = self.o, self.v
+        return (o, v), (o, o, v, v)
+
+    def set_state_device(self, t1, t2, lam1, lam2):
+        """set_state from device memory (DeviceArray or GPU torch tensors,
+        complex128): no host copy."""
+        s1, s2 = self._shapes()
+        ptrs = [self._dev(x, sh, nm) for x, sh, nm in
+                ((t1, s1, "t1"), (t2, s2, "t2"), (lam1, s1, "lam1"), (lam2, s2, "lam2"))]
+        self._check(self.lib.tdcc_set_state_device(self._h, *ptrs), "tdcc_set_state_device")
+
+    def get_state_device(self, which=0, like=None):
+        """get_state into new device arrays: GPU torch tensors on `like`'s
+        device when `like` is a torch tensor, else DeviceArrays."""
+        s1, s2 = self._shapes()
+        shapes = (s1, s2, s1, s2)
+        if like is not None and not isinstance(like, DeviceArray):
+            import torch
+            _require_one_hip_runtime()
+            out = [torch.empty(sh, dtype=torch.complex128, device=like.device) for sh in shapes]
+            ptrs = [ctypes.c_void_p(x.data_ptr()) for x in out]
+            torch.cuda.current_stream(like.device).synchronize()
+        else:
+            out = [DeviceArray(sh) for sh in shapes]
+            ptrs = [ctypes.c_void_p(x.ptr) for x in out]
+        self._check(self.lib.tdcc_get_state_device(self._h, int(which), *ptrs), "tdcc_get_state_device")
+        return tuple(out)
+
+    def rhs_device(self, kind, F, t1, t2, lam1=None, lam2=None):
+        """rhs on device memory (DeviceArray or GPU torch tensors, complex128,
+        result of t1's kind): no host copies."""
+        s1, s2 = self._shapes()
+        lam = kind in (RHS_L1, RHS_L2, RHS_L1_CC2, RHS_L2_CC2)
+        if lam and (lam1 is None or lam2 is None):
+            raise ValueError("lam1 and lam2 are required for a Lambda right-hand side")
+        pF = self._dev(F, (self.n, self.n), "F")
+        p1, p2 = self._dev(t1, s1, "t1"), self._dev(t2, s2, "t2")
+        pl1 = self._dev(lam1, s1, "lam1") if lam else None
+        pl2 = self._dev(lam2, s2, "lam2") if lam else None
+        out, po = self._empty_like(t1, s1 if kind in (RHS_T1, RHS_L1, RHS_L1_CC2) else s2)
+        if po is None:
+            po = ctypes.c_void_p(out.ptr)
+        self._check(self.lib.tdcc_rhs_device(self._h, int(kind), pF, p1, p2, pl1, pl2, po), "tdcc_rhs_device")
+        return out
 
     def get_state(self, which=0):
         o, v = self.o, self.v

@@ -68,6 +68,7 @@ struct tdcc_ctx {
   int8_t* vpack8[2] = {nullptr, nullptr};  // their digit planes in split precision (alias alike)
   double* vpack8_scale[2] = {nullptr, nullptr};
   unsigned long long* counts = nullptr;    // kCountBlocks + 1 partial counts
+  double* amp_check = nullptr;             // 2 kCountBlocks + 2 partial maxima (device amplitude checks)
   Runtime rt;
   std::map<std::string, Program> progs;
   // t2 and lam2 of the state (of the checkpoint) are antisymmetric in
@@ -157,6 +158,41 @@ bool antisym_ij(const double* x, int o, int v) {
           }
     }
   return dev <= 1e-12 * mx;
+}
+
+// antisym_ij for an amplitude in device memory (same bound, on the GPU).
+bool antisym_ij_device(tdcc_ctx* c, const double* x) {
+  const char* e = std::getenv("TDCC_IJ_PAIRS");
+  if (e && e[0] == '0') return false;
+  if (!x) return true;
+  if (!c->amp_check) TDCC_HIP(hipMalloc(&c->amp_check, (2 * kCountBlocks + 2) * sizeof(double)));
+  launch_amp_antisym_check(x, c->o, c->v, c->amp_check, c->stream);
+  TDCC_HIP(hipGetLastError());
+  double r[2];
+  TDCC_HIP(hipMemcpyAsync(r, c->amp_check + 2 * kCountBlocks, sizeof r, hipMemcpyDeviceToHost, c->stream));
+  TDCC_HIP(hipStreamSynchronize(c->stream));
+  return r[1] <= 1e-12 * r[0];
+}
+
+// A pointer handed to a *_device entry point must be device (or managed)
+// memory of the context's GPU.
+void require_device(const tdcc_ctx* c, const void* p, const char* what) {
+  hipPointerAttribute_t at{};
+  if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+    (void)hipGetLastError();
+    throw Error(std::string(what) + " is not device memory");
+  }
+  if (at.type != hipMemoryTypeDevice && at.type != hipMemoryTypeManaged)
+    throw Error(std::string(what) + " is not device memory");
+  if (at.device != c->device)
+    throw Error(std::string(what) + " is on device " + std::to_string(at.device) + ", the context on device " +
+                std::to_string(c->device));
+}
+
+// Interleaved complex device array -> the context's planes (no host copy).
+void upload_complex_device(tdcc_ctx* c, const double* dev, const Planes& p) {
+  launch_split_complex(dev, p.re(), p.im(), p.n, c->stream);
+  TDCC_HIP(hipGetLastError());
 }
 
 void upload_complex(tdcc_ctx* c, const double* host, const Planes& p) {
@@ -893,6 +929,7 @@ void tdcc_destroy(tdcc_ctx* c) {
     if (c->vpack8_scale[k]) (void)hipFree(c->vpack8_scale[k]);
   }
   if (c->counts) (void)hipFree(c->counts);
+  if (c->amp_check) (void)hipFree(c->amp_check);
   if (c->counter) (void)hipFree(c->counter);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -948,6 +985,36 @@ int tdcc_set_state(tdcc_ctx* c, const double* t1, const double* t2, const double
       upload_complex(c, h[k], c->state[k]);
     }
     c->pairs = antisym_ij(t2, c->o, c->v) && antisym_ij(l2, c->o, c->v);
+  });
+}
+
+int tdcc_set_state_device(tdcc_ctx* c, const double* t1, const double* t2, const double* l1, const double* l2) {
+  return guarded(c, [&] {
+    const double* d[4] = {t1, t2, l1, l2};
+    for (int k = 0; k < 4; ++k) {
+      if (!d[k]) throw Error("all four amplitude arrays are required");
+      require_device(c, d[k], "amplitude");
+    }
+    const bool pairs = antisym_ij_device(c, t2) && antisym_ij_device(c, l2);
+    for (int k = 0; k < 4; ++k) upload_complex_device(c, d[k], c->state[k]);
+    TDCC_HIP(hipStreamSynchronize(c->stream));
+    c->pairs = pairs;
+  });
+}
+
+int tdcc_get_state_device(tdcc_ctx* c, int which, double* t1, double* t2, double* l1, double* l2) {
+  return guarded(c, [&] {
+    if (which != 0 && which != 1) throw Error("which must be 0 (state) or 1 (rounded base)");
+    double* d[4] = {t1, t2, l1, l2};
+    for (int k = 0; k < 4; ++k)
+      if (d[k]) require_device(c, d[k], "amplitude");
+    for (int k = 0; k < 4; ++k) {
+      if (!d[k]) continue;
+      const Planes& p = which == 0 ? c->state[k] : c->base[k];
+      launch_join_complex(p.re(), p.im(), d[k], p.n, c->stream);
+    }
+    TDCC_HIP(hipGetLastError());
+    TDCC_HIP(hipStreamSynchronize(c->stream));
   });
 }
 
@@ -1220,6 +1287,33 @@ int tdcc_rhs(tdcc_ctx* c, int kind, const double* F, const double* t1, const dou
     Planes o = c->io_out;
     o.n = block_size(c, out_k);
     download_complex(c, o, out);
+  });
+}
+
+int tdcc_rhs_device(tdcc_ctx* c, int kind, const double* F, const double* t1, const double* t2, const double* l1,
+                    const double* l2, double* out) {
+  return guarded(c, [&] {
+    require_system(c);
+    if (!F || !t1 || !t2 || !out) throw Error("F, t1, t2 and out are required");
+    const bool lam = kind == TDCC_RHS_L1 || kind == TDCC_RHS_L2 || kind == TDCC_RHS_L1_CC2 || kind == TDCC_RHS_L2_CC2;
+    if (lam && (!l1 || !l2)) throw Error("lam1 and lam2 are required for a Lambda right-hand side");
+    const double* d[4] = {t1, t2, lam ? l1 : nullptr, lam ? l2 : nullptr};
+    require_device(c, F, "F");
+    require_device(c, out, "out");
+    for (int k = 0; k < 4; ++k)
+      if (d[k]) require_device(c, d[k], "amplitude");
+    const bool pairs = antisym_ij_device(c, t2) && (!lam || antisym_ij_device(c, l2));
+    Program& p = rhs_program(c, kind, pairs);
+    upload_complex_device(c, F, c->io_F);
+    for (int k = 0; k < 4; ++k)
+      if (d[k]) upload_complex_device(c, d[k], c->io_in[k]);
+    run_program(c, p, false);
+    const int out_k = (kind == TDCC_RHS_T1 || kind == TDCC_RHS_L1 || kind == TDCC_RHS_L1_CC2) ? 0 : 1;
+    Planes o = c->io_out;
+    o.n = block_size(c, out_k);
+    launch_join_complex(o.re(), o.im(), out, o.n, c->stream);
+    TDCC_HIP(hipGetLastError());
+    TDCC_HIP(hipStreamSynchronize(c->stream));
   });
 }
 

@@ -1401,6 +1401,56 @@ void launch_tei_antisym_check(const double* tei, int n, unsigned long long* coun
   hipLaunchKernelGGL(sum_counts_kernel, dim3(1), dim3(64), 0, s, count, kCountBlocks, count + kCountBlocks);
 }
 
+// (i, j)- and (a, b)-antisymmetry of an interleaved complex (o, o, v, v)
+// amplitude in device memory: per-block maxima of |x| and of the deviations
+// |x_ijab + x_jiab|, |x_ijab + x_ijba| (NaN entries ignored, as std::max
+// ignores them on the host), then one block takes the maxima of the partials.
+__global__ void amp_antisym_kernel(const double* x, int o, int v, double* partial) {
+  __shared__ double rm[256], rd[256];
+  const int64_t vv = (int64_t)v * v, ovv = (int64_t)o * vv, total = 2 * o * ovv;
+  double mx = 0.0, dev = 0.0;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t e = idx >> 1, c = idx & 1;
+    const int64_t i = e / ovv, r1 = e - i * ovv, j = r1 / vv, ab = r1 - j * vv, a = ab / v, b = ab - a * v;
+    const double xa = x[idx];
+    mx = fmax(mx, fabs(xa));
+    dev = fmax(dev, fabs(xa + x[2 * ((j * o + i) * vv + ab) + c]));
+    dev = fmax(dev, fabs(xa + x[2 * ((i * o + j) * vv + b * v + a) + c]));
+  }
+  rm[threadIdx.x] = mx;
+  rd[threadIdx.x] = dev;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      rm[threadIdx.x] = fmax(rm[threadIdx.x], rm[threadIdx.x + w]);
+      rd[threadIdx.x] = fmax(rd[threadIdx.x], rd[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = rm[0];
+    partial[2 * blockIdx.x + 1] = rd[0];
+  }
+}
+
+__global__ void max2_reduce_kernel(const double* partial, int nb, double* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double m = 0.0, d = 0.0;
+    for (int i = 0; i < nb; ++i) {
+      m = fmax(m, partial[2 * i]);
+      d = fmax(d, partial[2 * i + 1]);
+    }
+    out[0] = m;
+    out[1] = d;
+  }
+}
+
+void launch_amp_antisym_check(const double* x, int o, int v, double* work, hipStream_t s) {
+  hipLaunchKernelGGL(amp_antisym_kernel, dim3(kCountBlocks), dim3(256), 0, s, x, o, v, work);
+  hipLaunchKernelGGL(max2_reduce_kernel, dim3(1), dim3(64), 0, s, work, kCountBlocks, work + 2 * kCountBlocks);
+}
+
 void launch_count_mismatch(const double* x, const double* y, int64_t n, unsigned long long* count, hipStream_t s) {
   hipLaunchKernelGGL(mismatch_kernel, dim3(kCountBlocks), dim3(256), 0, s, x, y, n, count);
   hipLaunchKernelGGL(sum_counts_kernel, dim3(1), dim3(64), 0, s, count, kCountBlocks, count + kCountBlocks);

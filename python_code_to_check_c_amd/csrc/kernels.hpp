@@ -192,6 +192,10 @@ void launch_build_vpack(const double* tei, int o, int v, int ladder_ket, double*
 // Counts entries of the virtual block violating <ab||ef> = -<ba||ef> = -<ab||fe>.
 // `count` holds kCountBlocks + 1 entries; the total lands in count[kCountBlocks].
 constexpr int kCountBlocks = 1024;
+// Antisymmetry of a device (o, o, v, v) interleaved complex amplitude in (i, j)
+// and (a, b): `work` holds 2 kCountBlocks + 2 doubles; the maximum of |x| and
+// the largest deviation land in work[2 kCountBlocks] and work[2 kCountBlocks + 1].
+void launch_amp_antisym_check(const double* x, int o, int v, double* work, hipStream_t s);
 void launch_vvvv_antisym_check(const double* tei, int o, int v, unsigned long long* count, hipStream_t s);
 // The same count over the whole (n,n,n,n) array.
 void launch_tei_antisym_check(const double* tei, int n, unsigned long long* count, hipStream_t s);
