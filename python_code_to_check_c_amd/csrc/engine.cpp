@@ -1100,8 +1100,9 @@ void Graph::plan() {
 }
 
 // A term of a linear combination that reads an outer product of two rank-2
-// tensors (a contraction 'ia,jb->ijab' with nothing summed, such as t1 t1) is
-// replaced by a product term reading the two factors, so the outer product is
+// tensors (a contraction 'ia,jb->ijab' with nothing summed, such as t1 t1), or
+// a rank-4 tensor scaled element-wise by a small one ('ijab,b->ijab'), is
+// replaced by a product term reading the two factors, so the product is
 // computed where it is added instead of being written and read back; the
 // contraction is dropped when nothing else reads it.  TDCC_FOLD_OUTER=0
 // disables it.
@@ -1119,19 +1120,26 @@ void Graph::fold_outer_products(std::vector<char>& live) {
     unique_inputs(nodes_[i], in);
     for (int x : in) ++users[x];
   }
+  // a rank-4 contraction with no summed letter: the outer product of two
+  // matrices, or a rank-4 tensor scaled element-wise by a vector or matrix
+  // over some of its letters (e.g. x_ijab f_b after the sum over e of
+  // x_ijab F_eb), read as x * y at every element
   auto outer = [&](int c, std::string& la, std::string& lb, std::string& lc) {
     const Node& C = nodes_[c];
     if (C.op != Op::Contract || C.shape.rank != 4 || !C.kpair.empty()) return false;
-    if (nodes_[C.a].shape.rank != 2 || nodes_[C.b].shape.rank != 2) return false;
     std::vector<std::string> ins;
     parse_spec(C.spec, ins, lc);
     if (ins.size() != 2) return false;
     la = ins[0];
     lb = ins[1];
-    if (la.size() != 2 || lb.size() != 2 || lc.size() != 4) return false;
-    for (char x : la)
-      if (lb.find(x) != std::string::npos) return false;
-    return true;
+    if (lc.size() != 4) return false;
+    bool shared = false;
+    for (char x : la) shared = shared || lb.find(x) != std::string::npos;
+    for (const std::string* l : {&la, &lb})
+      for (char x : *l)
+        if (lc.find(x) == std::string::npos) return false;
+    if (!shared) return la.size() == 2 && lb.size() == 2;
+    return (la.size() == 4 && lb.size() <= 2) || (lb.size() == 4 && la.size() <= 2);
   };
   std::vector<int> before, after;
   for (int i = 0; i < N; ++i) {
