@@ -287,7 +287,6 @@ def main():
                    "speedup_vs_fp64": sps / sps64,
                    "F_ref_flop_per_step": f_ref, "F_alg_flop_per_step": f_alg,
                    "F_exec_contraction_flop_per_step": flops_exec,
-                   "ref_equiv_TFLOPs": f_alg * sps / 1e12,
                    "exec_TFLOPs": flops_exec * sps / 1e12},
     }
     if split and mu64 is not None:
@@ -405,8 +404,7 @@ def batched(args):
                    "digits": args.digits, "per_molecule": per_mol,
                    "sum_of_alone_seconds": sum(p["seconds_alone"] for p in per_mol),
                    "concurrent_seconds": wall,
-                   "F_alg_flop_per_batched_step": flops,
-                   "ref_equiv_TFLOPs": flops * args.steps / wall / 1e12}}), flush=True)
+                   "F_alg_flop_per_batched_step": flops}}), flush=True)
 
 
 def gradient(args, o=10, v=38, nsteps=500, ckpt=50):
