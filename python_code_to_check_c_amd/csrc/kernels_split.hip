@@ -350,6 +350,137 @@ __global__ __launch_bounds__(256) void split_cols_resident_kernel(SplitArgs a) {
   }
 }
 
+// Three-plane operands (re, im, re + im of a 3-product complex GEMM) split
+// from one read of each base row: the kernels below take base row rb of both
+// source planes once and write rows rb, Rb + rb and 2 Rb + rb (re, im and
+// re + im, the sum formed as row_at forms it), with the digits, scales and
+// maxima the per-plane kernels give.
+template <int S, int NJ>
+__global__ __launch_bounds__(256) void split_rows_resident3_kernel(SplitArgs a) {
+  __shared__ double red[3][4];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int64_t rb = blockIdx.x;
+  const RowSrc b0 = split_row_base(a, rb), b1 = split_row_base(a, a.Rb + rb);
+  double x[NJ][4], y[NJ][4];
+  double m[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const int64_t k = 1024 * (int64_t)j + 4 * tid;
+    load_row4(a, b0, k, x[j]);
+    load_row4(a, b1, k, y[j]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      m[0] = fmax(m[0], abs_or_inf(x[j][q]));
+      m[1] = fmax(m[1], abs_or_inf(y[j][q]));
+      m[2] = fmax(m[2], abs_or_inf(x[j][q] + y[j][q]));
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < 3; ++p) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m[p] = fmax(m[p], __shfl_xor(m[p], off));
+    if (lane == 0) red[p][w] = m[p];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < 3; ++p) {
+    const double mp = fmax(fmax(red[p][0], red[p][1]), fmax(red[p][2], red[p][3]));
+    const bool finite = std::isfinite(mp);
+    const int e = finite ? (mp > 0.0 ? ilogb(mp) + 2 : 0) : 0;
+    const int64_t r = p * a.Rb + rb;
+    if (tid == 0) a.scale[r] = finite ? ldexp(1.0, e) : NAN;
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      const int64_t k = 1024 * (int64_t)j + 4 * tid;
+      if (k >= a.Kp) break;
+      double v[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const double z = p == 0 ? x[j][q] : (p == 1 ? y[j][q] : x[j][q] + y[j][q]);
+        v[q] = finite ? ldexp(z, -e) : 0.0;
+      }
+#pragma unroll
+      for (int s = 0; s < S; ++s) *reinterpret_cast<uint32_t*>(digit_ptr<S>(a, r, k, s)) = next_digit4(v);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void split_max_cols3_kernel(SplitArgs a) {
+  __shared__ double red[3][4][SC_TR];
+  const int64_t R = 3 * a.Rb;
+  const int rl = threadIdx.x & 63, kg = threadIdx.x >> 6;
+  const int64_t rb = (int64_t)blockIdx.x * SC_TR + rl;
+  const int64_t k0 = (int64_t)blockIdx.y * SC_TK + 16 * kg;
+  double m[3] = {0.0, 0.0, 0.0};
+  if (rb < a.Rb) {
+    const RowSrc b0 = split_row_base(a, rb), b1 = split_row_base(a, a.Rb + rb);
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+      if (k0 + j < a.K) {
+        const int64_t o = split_koff(a, k0 + j);
+        const double xr = row_at(b0, o), xi = row_at(b1, o);
+        m[0] = fmax(m[0], abs_or_inf(xr));
+        m[1] = fmax(m[1], abs_or_inf(xi));
+        m[2] = fmax(m[2], abs_or_inf(xr + xi));
+      }
+  }
+#pragma unroll
+  for (int p = 0; p < 3; ++p) red[p][kg][rl] = m[p];
+  __syncthreads();
+  if (kg == 0 && rb < a.Rb)
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+      a.pmax[(int64_t)blockIdx.y * R + p * a.Rb + rb] =
+          fmax(fmax(red[p][0][rl], red[p][1][rl]), fmax(red[p][2][rl], red[p][3][rl]));
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void split_digits_cols3_kernel(SplitArgs a) {
+  const int rl = threadIdx.x & 63, kg = threadIdx.x >> 6;
+  const int64_t rb = (int64_t)blockIdx.x * SC_TR + rl;
+  const int64_t k0 = (int64_t)blockIdx.y * SC_TK + 16 * kg;
+  if (rb >= a.Rb) return;
+  const RowSrc b0 = split_row_base(a, rb), b1 = split_row_base(a, a.Rb + rb);
+  double x[16], y[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int64_t k = k0 + j;
+    const int64_t o = k < a.K ? split_koff(a, k) : 0;
+    x[j] = k < a.K ? row_at(b0, o) : 0.0;
+    y[j] = k < a.K ? row_at(b1, o) : 0.0;
+  }
+#pragma unroll
+  for (int p = 0; p < 3; ++p) {
+    const int64_t r = p * a.Rb + rb;
+    const double sc = a.scale[r];
+    const bool finite = std::isfinite(sc);
+    const int e = finite ? ilogb(sc) : 0;
+    uint32_t out[4][S];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      double v[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int jj = 4 * q + j;
+        const double z = p == 0 ? x[jj] : (p == 1 ? y[jj] : x[jj] + y[jj]);
+        v[j] = (finite && k0 + jj < a.K) ? ldexp(z, -e) : 0.0;
+      }
+#pragma unroll
+      for (int s = 0; s < S; ++s) out[q][s] = next_digit4(v);
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+      *reinterpret_cast<uint4*>(digit_ptr<S>(a, r, k0, s)) = make_uint4(out[0][s], out[1][s], out[2][s], out[3][s]);
+  }
+}
+
+// TDCC_SPLIT3_FUSED=0: three-plane operands split plane by plane (the third
+// plane reading both source planes again).
+static bool split3_fused_on() {
+  const char* e = std::getenv("TDCC_SPLIT3_FUSED");
+  return !e || std::atoi(e) != 0;
+}
+
 int64_t ozaki_split_workspace(int64_t R, int64_t Kp, bool by_columns) {
   const int64_t tk = by_columns ? SC_TK : SR_TK;
   return ((Kp + tk - 1) / tk) * R * (int64_t)sizeof(double);
@@ -372,6 +503,25 @@ static void split_launch(const SplitArgs& a, bool by_columns, hipStream_t s) {
   // uncoalesced loads cost little.
   if (a.sk_chunk == 0 && !a.multipass && a.Kp <= 256 && (!by_columns || R <= wave_cols_max_rows())) {
     hipLaunchKernelGGL(split_rows_wave_kernel<S>, dim3((unsigned)((R + 3) / 4)), block, 0, s, a);
+    return;
+  }
+  const bool three = a.planes == 3 && !a.interleave && a.sk_chunk == 0 && !a.multipass && split3_fused_on();
+  if (three && !by_columns && a.Kp <= 16384) {
+    const dim3 grid((unsigned)a.Rb);
+    if (a.Kp <= 1024) hipLaunchKernelGGL((split_rows_resident3_kernel<S, 1>), grid, block, 0, s, a);
+    else if (a.Kp <= 2048) hipLaunchKernelGGL((split_rows_resident3_kernel<S, 2>), grid, block, 0, s, a);
+    else if (a.Kp <= 4096) hipLaunchKernelGGL((split_rows_resident3_kernel<S, 4>), grid, block, 0, s, a);
+    else if (a.Kp <= 8192) hipLaunchKernelGGL((split_rows_resident3_kernel<S, 8>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((split_rows_resident3_kernel<S, 16>), grid, block, 0, s, a);
+    return;
+  }
+  if (three && by_columns && a.Kp > 256) {
+    const int64_t kt = (a.Kp + SC_TK - 1) / SC_TK;
+    const dim3 grid((unsigned)((a.Rb + SC_TR - 1) / SC_TR), (unsigned)kt);
+    hipLaunchKernelGGL(split_max_cols3_kernel, grid, block, 0, s, a);
+    if (kt > 64) hipLaunchKernelGGL(split_scale_long_kernel, dim3((unsigned)R), block, 0, s, a, kt);
+    else hipLaunchKernelGGL(split_scale_kernel, dim3((unsigned)((R + 255) / 256)), block, 0, s, a, kt);
+    hipLaunchKernelGGL(split_digits_cols3_kernel<S>, grid, block, 0, s, a);
     return;
   }
   // rows in one pass: whole rows up to 16384, or K chunks with their own

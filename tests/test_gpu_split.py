@@ -260,6 +260,54 @@ def test_rejects_bad_digit_count(case):
             ctx.set_precision(bad)
 
 
+@pytest.mark.parametrize("digits", [4, 5])
+def test_fused_three_plane_split_is_bitwise_neutral(digits):
+    """Three-plane operands of the 3-product complex GEMMs (re, im, re + im)
+    are split from one read of each base row (TDCC_SPLIT3_FUSED=0: plane by
+    plane).  The digits and scales are the same numbers, so at o = 8, v = 64
+    with every complex x complex GEMM on three products (summed length 512:
+    the row-resident and the three-pass column splits) the residuals are
+    bitwise equal."""
+    from python_code_to_check_c_amd import synthetic
+    o, v = 8, 64
+    sysm = synthetic.SyntheticSystem(3, o, v, with_tei=False)
+    rng = np.random.default_rng(8)
+
+    def amp(shape):
+        return 1e-2 * (rng.standard_normal(shape) + 1j * rng.standard_normal(shape))
+
+    t1, l1 = amp((o, v)), amp((o, v))
+    t2, l2 = amp((o, o, v, v)), amp((o, o, v, v))
+    for x in (t2, l2):
+        x -= x.transpose(1, 0, 2, 3)
+        x -= x.transpose(0, 1, 3, 2)
+    Fc = sysm.F + 0j
+    out = {}
+    keys = ("TDCC_SPLIT3_FUSED", "TDCC_GEMM_MIN_FLOPS", "TDCC_M3")
+    saved = {k: os.environ.get(k) for k in keys}
+    try:
+        os.environ["TDCC_GEMM_MIN_FLOPS"] = "0"
+        os.environ["TDCC_M3"] = "2"
+        for mode in ("1", "0"):
+            os.environ["TDCC_SPLIT3_FUSED"] = mode
+            c = _lib.Context(o, v)
+            try:
+                c.set_system(None, np.asarray(sysm.F), np.asarray(sysm.mu))
+                c.synth_tei(3, 0.05)
+                c.set_precision(digits)
+                out[mode] = [c.rhs(kind, Fc, t1, t2, l1, l2) for _, kind in RHS]
+            finally:
+                c.close()
+    finally:
+        for k, val in saved.items():
+            if val is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = val
+    for (name, _), x, y in zip(RHS, out["1"], out["0"]):
+        assert np.array_equal(x, y), name
+
+
 @pytest.mark.parametrize("digits", [0, 5])
 @pytest.mark.parametrize("batch", ["single", "stacked", "side_by_side"])
 def test_two_stream_step_is_bitwise_neutral(digits, batch):
