@@ -890,9 +890,17 @@ static bool is_post_op(Op op) {
 // Lambda side when the Lambda side reads it and the T side does not (the
 // finalize and what follows it do not count), else on the T side.  A T side
 // node the Lambda side reads is a crossing.
+static bool cross_first_on() {
+  const char* e = std::getenv("TDCC_CROSS_FIRST");
+  return !e || std::atoi(e) != 0;
+}
+
 void Graph::compute_sides() {
   side_.clear();
   cross_.clear();
+  order_.resize(nodes_.size());
+  pos_.resize(nodes_.size());
+  for (int i = 0; i < (int)nodes_.size(); ++i) order_[i] = pos_[i] = i;
   bool have_fin = false;
   for (int i = 0; i < (int)nodes_.size(); ++i) have_fin = have_fin || (live_[i] && nodes_[i].op == Op::Finalize);
   if (lam_ext_.empty() || !have_fin || !two_streams_on()) return;
@@ -945,6 +953,27 @@ void Graph::compute_sides() {
       if (nodes_[r].op != Op::External && side_[r] == 0) cross_[r] = 1;
     }
   }
+  // The T side computes what the Lambda side reads from it (the crossing
+  // nodes and everything they are computed from) before its own remaining
+  // work, so that the Lambda side is not held up by T-side work it does not
+  // need (e.g. a field-dependent intermediate of the step-start t first built
+  // for a late T stage); TDCC_CROSS_FIRST=0 keeps node order.
+  if (!cross_first_on()) return;
+  std::vector<char> cl(N, 0);
+  for (int i = N - 1; i >= 0; --i) {
+    if (!live_[i]) continue;
+    if (cross_[i]) cl[i] = 1;
+    if (!cl[i]) continue;
+    inputs_of(nodes_[i], in);
+    for (int x : in) cl[x] = 1;
+  }
+  order_.clear();
+  for (int pass = 0; pass < 3; ++pass)
+    for (int i = 0; i < N; ++i) {
+      const int cls = i >= first_fin ? 2 : (cl[i] ? 0 : 1);
+      if (cls == pass) order_.push_back(i);
+    }
+  for (int p = 0; p < N; ++p) pos_[order_[p]] = p;
 }
 
 void Graph::plan() {
@@ -973,13 +1002,15 @@ void Graph::plan() {
     while (nodes_[x].op == Op::View) x = nodes_[x].base;
     return x;
   };
+  // (positions in the execution order)
   for (auto& n : nodes_) n.last_use = -1;
-  for (int i = 0; i < N; ++i) {
+  for (int p = 0; p < N; ++p) {
+    const int i = order_[p];
     if (!live[i] || nodes_[i].op == Op::View) continue;
     inputs_of(nodes_[i], in);
     for (int x : in) {
       const int r = root(x);
-      nodes_[r].last_use = std::max(nodes_[r].last_use, i);
+      nodes_[r].last_use = std::max(nodes_[r].last_use, p);
     }
   }
   // arena layout: first fit over a coalescing free list per region (one
@@ -1023,14 +1054,15 @@ void Graph::plan() {
       }
     }
   };
-  std::vector<std::vector<int>> ends(N);
+  std::vector<std::vector<int>> ends(N);  // by position
   for (int i = 0; i < N; ++i)
-    if (live[i] && owns_buffer(nodes_[i].op)) ends[std::max(nodes_[i].last_use, i)].push_back(i);
+    if (live[i] && owns_buffer(nodes_[i].op)) ends[std::max(nodes_[i].last_use, pos_[i])].push_back(i);
   host_tables_.clear();
   launches_.clear();
   dyn_uses_.clear();
   shares_.clear();
-  for (int i = 0; i < N; ++i) {
+  for (int p = 0; p < N; ++p) {
+    const int i = order_[p];
     if (!live[i]) continue;
     Node& n = nodes_[i];
     if (n.op == Op::Ladder)
@@ -1047,7 +1079,8 @@ void Graph::plan() {
   auto side_region = [&](int i) { return two && side_[i] == 1 ? 1 : 0; };
   auto buf_region = [&](int i) { return two && cross_[i] ? 2 : side_region(i); };
   std::vector<int> node_region(N, 0), share_region(shares_.size(), 0);
-  for (int i = 0; i < N; ++i) {
+  for (int p = 0; p < N; ++p) {
+    const int i = order_[p];
     if (!live[i]) continue;
     Node& n = nodes_[i];
     if (owns_buffer(n.op)) {
@@ -1068,7 +1101,7 @@ void Graph::plan() {
       cur_region = share_region[k];
       release(shares_[k].off, shares_[k].bytes);
     }
-    for (int x : ends[i]) {
+    for (int x : ends[p]) {
       cur_region = node_region[x] & 15;
       release(nodes_[x].arena_off, zbytes(nodes_[x]) * (nodes_[x].zdep ? nz_ : 1));
     }
@@ -1292,11 +1325,11 @@ void Graph::plan_shares() {
     while (nodes_[x].op == Op::View) x = nodes_[x].base;
     return x;
   };
-  auto written_between = [&](int src, int lo, int hi) {
+  auto written_between = [&](int src, int lo, int hi) {  // nodes, compared by position
     auto it = writes.find(root(src));
     if (it == writes.end()) return false;
     for (int w : it->second)
-      if (w > lo && w < hi) return true;
+      if (pos_[w] > pos_[lo] && pos_[w] < pos_[hi]) return true;
     return false;
   };
   for (auto& [key, all_uses] : dyn_uses_) {
@@ -1305,6 +1338,8 @@ void Graph::plan_shares() {
     std::vector<DynUse> uses;
     for (const DynUse& u : all_uses)
       if (side_.empty() || (side_[u.node] == 1 ? 1 : 0) == sd) uses.push_back(u);
+    std::stable_sort(uses.begin(), uses.end(),
+                     [&](const DynUse& x, const DynUse& y) { return pos_[x.node] < pos_[y.node]; });
     size_t g0 = 0;
     while (g0 < uses.size()) {
       size_t g1 = g0 + 1;
@@ -1947,7 +1982,8 @@ void Graph::bind(double* arena) {
   size_t gi = 0;
   const int N = (int)nodes_.size();
   const int64_t n2 = (int64_t)(o_ + v_) * (o_ + v_);
-  for (int i = 0; i < N; ++i) {
+  for (int p = 0; p < N; ++p) {
+    const int i = order_[p];
     if (!live_[i]) continue;
     const Node& n = nodes_[i];
     const size_t nl0 = launches_.size();

@@ -344,6 +344,11 @@ class Graph {
   std::vector<int> side_;
   std::vector<char> cross_;
   void compute_sides();
+  // execution order of the nodes (a topological order; node order in a
+  // one-stream program; in a two-stream step the crossing nodes and their
+  // ancestors first, then the rest up to the first finalize, then the rest)
+  // and each node's position in it
+  std::vector<int> order_, pos_;
   void plan_streams();  // at bind, from the sides
   // launch schedule: stream of each launch, events to wait on before it,
   // event to record after it, index of the first launch after the join
