@@ -186,9 +186,11 @@ def main():
                     help="only sum a rocprofv3 --pmc counter csv per kernel (with the MFMA-busy fraction "
                          "SQ_VALU_MFMA_BUSY_CYCLES / (4 SIMDs x SQ_BUSY_CU_CYCLES) and LDS bank-conflict rate "
                          "where those counters are present) and exit")
-    ap.add_argument("--trace-summary", nargs=2, default=None, metavar=("KERNEL_TRACE_CSV", "STEPS"),
-                    help="only summarise a rocprofv3 --kernel-trace csv of a bench.py leg per step (STEPS = the timed "
-                         "steps, the dispatches after the last state upload) and exit")
+    ap.add_argument("--trace-summary", nargs="+", default=None, metavar="ARG",
+                    help="KERNEL_TRACE_CSV STEPS [BLOCK]: only summarise a rocprofv3 --kernel-trace csv of a bench.py "
+                         "leg per step and exit (BLOCK, default 1: the last block of dispatches between the runtime's "
+                         "copy kernels, the timed run with --no-profile or the profiled step without it; 2: the one "
+                         "before)")
     ap.add_argument("--batched", action="store_true",
                     help="the batched workload instead: 4 molecule shapes x 3 polarisations x 8 frequencies, "
                          "one context and one captured step per molecule (prints its own JSON line)")
@@ -220,7 +222,8 @@ def main():
         counter_summary(args.counter_summary)
         return
     if args.trace_summary:
-        trace_summary(args.trace_summary[0], int(args.trace_summary[1]))
+        trace_summary(args.trace_summary[0], int(args.trace_summary[1]),
+                      int(args.trace_summary[2]) if len(args.trace_summary) > 2 else 1)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding of this "
@@ -505,30 +508,48 @@ def counter_summary(path):
                         [f"{e[1](v):.4f}" for e in extra]))
 
 
-def trace_summary(path, steps):
+def trace_summary(path, steps, block=1):
     """Per-kernel statistics per RK4 step of a rocprofv3 kernel trace of a
-    bench.py leg: the dispatches after the last runtime copy kernel that
-    precedes the trailing ones (the state upload before the timed run) are the
-    timed steps, `steps` graph replays; set-up, autotuner trials and the
-    warm-up come before it.  The replays run a multi-stream graph, so the order of
-    dispatches differs between steps; the counts per step do not."""
+    bench.py leg.  The dispatches fall into blocks separated by the runtime's
+    copy kernels; `block` counts them from the end.  With --no-profile the
+    last block is the timed run (`steps` graph replays after the state
+    upload; set-up, autotuner trials and the warm-up come before it); with the
+    profiled step it is that eager step (steps = 1, launches one after another
+    on one stream, each between two HIP events: the durations behind the
+    roofline figure) and the timed run is block 2.  The replays run a
+    two-stream graph, so the order of dispatches differs between steps; the
+    counts per step do not."""
     import csv
     with open(path) as f:
         rows = sorted(csv.DictReader(f), key=lambda r: int(r["Dispatch_Id"]))
-    rt = [r["Kernel_Name"].startswith("__amd_rocclr") for r in rows]
-    end = len(rows)
-    while end > 0 and rt[end - 1]:
-        end -= 1
-    beg = end
-    while beg > 0 and not rt[beg - 1]:
-        beg -= 1
-    sel = rows[beg:end]
+    blocks, cur = [], []
+    for r in rows:
+        if r["Kernel_Name"].startswith("__amd_rocclr"):
+            if cur:
+                blocks.append(cur)
+            cur = []
+        else:
+            cur.append(r)
+    if cur:
+        blocks.append(cur)
+    sel = blocks[-int(block)]
     acc = {}
     for r in sel:
         acc.setdefault(r["Kernel_Name"], []).append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3)
     total = sum(sum(v) for v in acc.values())
+    # timeline: wall clock, time with at least one and with two or more kernels running
+    ev = sorted([(int(r["Start_Timestamp"]), 1) for r in sel] + [(int(r["End_Timestamp"]), -1) for r in sel])
+    level, last, busy, multi = 0, ev[0][0], 0, 0
+    for t, d in ev:
+        busy += (t - last) if level > 0 else 0
+        multi += (t - last) if level > 1 else 0
+        level, last = level + d, t
+    wall = ev[-1][0] - ev[0][0]
     print(f"# {len(sel) // steps} dispatches per step over {steps} steps ({total * 1e-3:.2f} ms of kernel time, "
           f"{total * 1e-3 / steps:.2f} ms per step)")
+    print(f"# timeline per step: {wall * 1e-6 / steps:.2f} ms wall (first start to last end), {busy * 1e-6 / steps:.2f} ms "
+          f"with a kernel running, {multi * 1e-6 / steps:.2f} ms with two or more (kernels that share the GPU "
+          "run longer than alone, so their durations add up to more than the wall time)")
     print("| kernel | calls per step | ms per step | avg us | min us | max us | % |")
     print("|---|---|---|---|---|---|---|")
     for n, v in sorted(acc.items(), key=lambda kv: -sum(kv[1])):
@@ -544,14 +565,19 @@ def pmc_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, copy_mib):
     cf = [v for n, v in _counter_rows(copy_fetch_csv) if "MulFunctor" in n]
     cw = [v for n, v in _counter_rows(copy_write_csv) if "MulFunctor" in n]
     f_scale, w_scale = true_kib / np.mean(cf), true_kib / np.mean(cw)
-    acc = {}
-    for (n, fv), (n2, wv) in zip(_counter_rows(fetch_csv), _counter_rows(write_csv)):
-        if n != n2:
-            raise RuntimeError("the FETCH_SIZE and WRITE_SIZE runs dispatched different kernels")
+    acc, wacc = {}, {}
+    for n, fv in _counter_rows(fetch_csv):
         a = acc.setdefault(n, [0, 0.0, 0.0])
         a[0] += 1
         a[1] += fv * f_scale * 1024.0
-        a[2] += wv * w_scale * 1024.0
+    for n, wv in _counter_rows(write_csv):
+        w = wacc.setdefault(n, [0, 0.0])
+        w[0] += 1
+        w[1] += wv * w_scale * 1024.0
+    if {n: a[0] for n, a in acc.items()} != {n: w[0] for n, w in wacc.items()}:
+        raise RuntimeError("the FETCH_SIZE and WRITE_SIZE runs dispatched different kernels")
+    for n, w in wacc.items():
+        acc[n][2] = w[1]
     print(f"# FETCH_SIZE scale {f_scale:.4f}, WRITE_SIZE scale {w_scale:.4f} (copy kernel, {copy_mib} MiB each way)")
     print("# kernel\tlaunches\tfetch_bytes_per_launch\twrite_bytes_per_launch")
     for n, (c, fb, wb) in sorted(acc.items(), key=lambda kv: -(kv[1][1] + kv[1][2])):
