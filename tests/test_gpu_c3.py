@@ -95,3 +95,41 @@ def test_split_residuals(c3):
             assert r <= 300.0 * 128.0 ** -S, (S, name, r)
     for name in rel[4]:
         assert rel[5][name] * 32.0 <= rel[4][name], (name, rel[4][name], rel[5][name])
+
+
+@pytest.mark.parametrize("digits", [0, 5])
+def test_two_stream_steps_at_workload_shape(digits):
+    """Two RK4 steps of CCSD at o=20, v=160 from the seeded state: the step
+    with its T side and Lambda side on two streams (the default) gives
+    bitwise the dipoles and amplitudes of the one-stream step
+    (TDCC_STREAMS=1), in FP64 and with 5 digits.  At this shape the step holds
+    the packed ladders, the three-product rings and the T-side buffers the
+    Lambda side reads across the streams."""
+    g = dict(np.load(os.path.join(HERE, "golden", "c3_samples.npz")))
+    o, v, cid, sigma = int(g["o"]), int(g["v"]), int(g["config_id"]), float(g["sigma"])
+    s = synthetic.SyntheticSystem(cid, o, v, tei_sigma=sigma, tei_on_device=True)
+    state = synthetic.perturbed_state(cid, o, v, sigma)
+    out = {}
+    saved = os.environ.get("TDCC_STREAMS")
+    try:
+        for mode in ("2", "1"):
+            os.environ["TDCC_STREAMS"] = mode
+            print(f"\n[c3 steps digits={digits} streams={mode}]", flush=True)
+            ctx = _lib.Context(o, v)
+            try:
+                ctx.set_system(None, s.F, s.mu)
+                ctx.synth_tei(cid, sigma)
+                ctx.set_precision(digits)
+                ctx.set_state(*state)
+                mu, _ = ctx.rk4_run("CCSD", 2, 0.0, 0.001, 0.923, 0.002)
+                out[mode] = [mu, *ctx.get_state(0)]
+            finally:
+                ctx.close()
+    finally:
+        if saved is None:
+            os.environ.pop("TDCC_STREAMS", None)
+        else:
+            os.environ["TDCC_STREAMS"] = saved
+    assert np.isfinite(out["2"][0]).all()
+    for x, y in zip(out["2"], out["1"]):
+        assert np.array_equal(x, y)
