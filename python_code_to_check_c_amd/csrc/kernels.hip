@@ -62,7 +62,9 @@ __global__ __launch_bounds__(NT) void gett_kernel(GettArgs g) {
       a_lk[p] = kk;
     }
   } else {
-    const int mm = tid & 63, kr = tid >> 6;
+    // kr is the wave index, uniform across the wave: this operand's k
+    // offsets are then read with scalar loads
+    const int mm = tid & 63, kr = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int m = m0 + mm;
     const int64_t mo = (m < g.M) ? g.am[m] : -1;
 #pragma unroll
@@ -84,7 +86,7 @@ __global__ __launch_bounds__(NT) void gett_kernel(GettArgs g) {
       b_lk[p] = kk;
     }
   } else {
-    const int nn = tid & 63, kr = tid >> 6;
+    const int nn = tid & 63, kr = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int n = n0 + nn;
     const int64_t no = (n < g.N) ? g.bn[n] : -1;
 #pragma unroll
@@ -102,7 +104,13 @@ __global__ __launch_bounds__(NT) void gett_kernel(GettArgs g) {
     for (int p = 0; p < 4; ++p) {
       const int k = k0 + a_lk[p];
       const bool ok = (a_mo[p] >= 0) && (k < ke);
-      const int64_t off = ok ? a_mo[p] + g.ak[k] : 0;
+      int64_t off;
+      if constexpr (AKC) {
+        off = ok ? a_mo[p] + g.ak[k] : 0;
+      } else {
+        const int64_t kofs = k < ke ? g.ak[k] : 0;  // uniform across the wave: a scalar load
+        off = ok ? a_mo[p] + kofs : 0;
+      }
       ra[0][p] = ok ? a_re[off] : 0.0;
       if (AC) ra[PA - 1][p] = ok ? a_im[off] : 0.0;
     }
@@ -110,7 +118,13 @@ __global__ __launch_bounds__(NT) void gett_kernel(GettArgs g) {
     for (int p = 0; p < 4; ++p) {
       const int k = k0 + b_lk[p];
       const bool ok = (b_no[p] >= 0) && (k < ke);
-      const int64_t off = ok ? b_no[p] + g.bk[k] : 0;
+      int64_t off;
+      if constexpr (BKC) {
+        off = ok ? b_no[p] + g.bk[k] : 0;
+      } else {
+        const int64_t kofs = k < ke ? g.bk[k] : 0;  // uniform across the wave: a scalar load
+        off = ok ? b_no[p] + kofs : 0;
+      }
       rb[0][p] = ok ? b_re[off] : 0.0;
       if (BC) rb[PB - 1][p] = ok ? b_im[off] : 0.0;
     }

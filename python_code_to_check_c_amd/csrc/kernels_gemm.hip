@@ -108,10 +108,14 @@ __global__ __launch_bounds__(256) void dot_kernel(DotArgs g) {
   const int64_t ke = kb + g.k_chunk < g.K ? kb + g.k_chunk : g.K;
   const int64_t nsteps = (ke - kb + DK_BK - 1) / DK_BK;
   const int tid = threadIdx.x;
+  // the wave index, known to the compiler as uniform across the wave, so that
+  // the k offsets of an operand contiguous along its rows (one k per wave)
+  // are read with scalar loads
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   // loader element p of an operand contiguous along k: row tid/32 + 8p, k
   // tid%32; of one contiguous along its rows: row tid%64, k tid/64 + 4p
   auto lrow = [&](bool kc, int p) { return kc ? (tid >> 5) + 8 * p : (tid & 63); };
-  auto lk = [&](bool kc, int p) { return kc ? (tid & 31) : (tid >> 6) + 4 * p; };
+  auto lk = [&](bool kc, int p) { return kc ? (tid & 31) : wv + 4 * p; };
   const double* arow[8];
   const double* brow[8];
 #pragma unroll
