@@ -296,6 +296,12 @@ static int64_t dot_min_k() {
   return e ? std::atoll(e) : 8192;
 }
 
+// TDCC_GETT_LINEAR_K=0: GETT launches read every K offset from its table.
+static bool gett_linear_k_on() {
+  const char* e = std::getenv("TDCC_GETT_LINEAR_K");
+  return !e || std::atoi(e) != 0;
+}
+
 // Skinny contractions with at least this much big-operand work (rows x K) take
 // the VALU streaming kernel; TDCC_SKINNY_MIN overrides it (0: all skinny
 // shapes, a negative value: none).
@@ -2289,6 +2295,31 @@ void Graph::bind(double* arena) {
           sc.alpha_re = n.ar * n.kfac;
           sc.alpha_im = n.ai * n.kfac;
           sc.accumulate = 0;
+        }
+        // a plain GETT launch whose K offsets step evenly reads them as
+        // k * step, the first offset folded into the operand pointers
+        g.ak_step = g.bk_step = -1;
+        if (!L->skinny && !L->gp.on && gett_linear_k_on()) {
+          auto linear = [&](int64_t tab, int64_t& step) {
+            const int64_t* t = host_tables_.data() + tab;
+            step = g.K > 1 ? t[1] - t[0] : 0;
+            for (int64_t k = 0; k < g.K; ++k)
+              if (t[k] != t[0] + k * step) return false;
+            return g.K > 0 && step >= 0;
+          };
+          int64_t st = 0;
+          if (linear(L->tab[1], st)) {
+            const int64_t k0 = host_tables_[L->tab[1]];
+            g.ak_step = st;
+            g.a_re += k0;
+            if (g.a_im) g.a_im += k0;
+          }
+          if (linear(L->tab[3], st)) {
+            const int64_t k0 = host_tables_[L->tab[3]];
+            g.bk_step = st;
+            g.b_re += k0;
+            if (g.b_im) g.b_im += k0;
+          }
         }
         launches_.push_back(std::move(L));
         }

@@ -106,9 +106,10 @@ __global__ __launch_bounds__(NT) void gett_kernel(GettArgs g) {
       const bool ok = (a_mo[p] >= 0) && (k < ke);
       int64_t off;
       if constexpr (AKC) {
-        off = ok ? a_mo[p] + g.ak[k] : 0;
+        off = ok ? a_mo[p] + (g.ak_step >= 0 ? (int64_t)k * g.ak_step : g.ak[k]) : 0;
       } else {
-        const int64_t kofs = k < ke ? g.ak[k] : 0;  // uniform across the wave: a scalar load
+        // uniform across the wave: a scalar load
+        const int64_t kofs = k < ke ? (g.ak_step >= 0 ? (int64_t)k * g.ak_step : g.ak[k]) : 0;
         off = ok ? a_mo[p] + kofs : 0;
       }
       ra[0][p] = ok ? a_re[off] : 0.0;
@@ -120,9 +121,9 @@ __global__ __launch_bounds__(NT) void gett_kernel(GettArgs g) {
       const bool ok = (b_no[p] >= 0) && (k < ke);
       int64_t off;
       if constexpr (BKC) {
-        off = ok ? b_no[p] + g.bk[k] : 0;
+        off = ok ? b_no[p] + (g.bk_step >= 0 ? (int64_t)k * g.bk_step : g.bk[k]) : 0;
       } else {
-        const int64_t kofs = k < ke ? g.bk[k] : 0;  // uniform across the wave: a scalar load
+        const int64_t kofs = k < ke ? (g.bk_step >= 0 ? (int64_t)k * g.bk_step : g.bk[k]) : 0;
         off = ok ? b_no[p] + kofs : 0;
       }
       rb[0][p] = ok ? b_re[off] : 0.0;

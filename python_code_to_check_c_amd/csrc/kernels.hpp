@@ -49,6 +49,9 @@ struct GettArgs {
   double alpha_re, alpha_im;
   int accumulate;
   double* partial;              // [splits][batch][M][N] x planes, when splits > 1
+  // >= 0: the K offsets are k * step (ak / bk unused; their first entry is
+  // folded into the operand's pointers)
+  int64_t ak_step = -1, bk_step = -1;
 };
 
 // Loader mapping: whether A's / B's memory-contiguous index lies in K.
