@@ -1707,6 +1707,9 @@ void Graph::lower_contract(int id) {
   gk = k_by_b ? by_stride(gk, sb) : by_stride(gk, sa);
 
   auto push = [&](const std::vector<int64_t>& t) {
+    // every table starts on a 32-byte boundary, so that kernels can read four
+    // consecutive offsets with two 16-byte loads
+    while (host_tables_.size() % 4) host_tables_.push_back(0);
     const int64_t off = (int64_t)host_tables_.size();
     host_tables_.insert(host_tables_.end(), t.begin(), t.end());
     return off;

@@ -89,6 +89,14 @@ __device__ __forceinline__ void load_row4(const SplitArgs& a, const RowSrc& base
       v[2] += y2.x;
       v[3] += y2.y;
     }
+  } else if (a.ktab && k + 3 < kend && (reinterpret_cast<uintptr_t>(a.ktab + k) & 15) == 0) {
+    // the four offsets with two 16-byte loads
+    const longlong2 o01 = *reinterpret_cast<const longlong2*>(a.ktab + k);
+    const longlong2 o23 = *reinterpret_cast<const longlong2*>(a.ktab + k + 2);
+    v[0] = row_at(base, o01.x);
+    v[1] = row_at(base, o01.y);
+    v[2] = row_at(base, o23.x);
+    v[3] = row_at(base, o23.y);
   } else {
 #pragma unroll
     for (int q = 0; q < 4; ++q) v[q] = k + q < kend ? row_at(base, split_koff(a, k + q)) : 0.0;
