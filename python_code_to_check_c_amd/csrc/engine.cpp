@@ -1979,6 +1979,19 @@ void Graph::lower_contract(int id) {
   launches_.push_back(std::move(L));
 }
 
+// Whether the K offsets at `tab` (K entries) step evenly: offset k is
+// first + k * step, step >= 0.
+bool Graph::linear_table(int64_t tab, int64_t K, int64_t& step, int64_t& first) const {
+  if (K < 1) return false;
+  const int64_t* t = host_tables_.data() + tab;
+  first = t[0];
+  step = K > 1 ? t[1] - t[0] : 0;
+  if (step < 0) return false;
+  for (int64_t k = 0; k < K; ++k)
+    if (t[k] != first + k * step) return false;
+  return true;
+}
+
 void Graph::bind(double* arena) {
   arena_ = arena;
   for (auto& e : cache_) e.filled = false;
@@ -2303,22 +2316,13 @@ void Graph::bind(double* arena) {
         // k * step, the first offset folded into the operand pointers
         g.ak_step = g.bk_step = -1;
         if (!L->skinny && !L->gp.on && gett_linear_k_on()) {
-          auto linear = [&](int64_t tab, int64_t& step) {
-            const int64_t* t = host_tables_.data() + tab;
-            step = g.K > 1 ? t[1] - t[0] : 0;
-            for (int64_t k = 0; k < g.K; ++k)
-              if (t[k] != t[0] + k * step) return false;
-            return g.K > 0 && step >= 0;
-          };
-          int64_t st = 0;
-          if (linear(L->tab[1], st)) {
-            const int64_t k0 = host_tables_[L->tab[1]];
+          int64_t st = 0, k0 = 0;
+          if (linear_table(L->tab[1], g.K, st, k0)) {
             g.ak_step = st;
             g.a_re += k0;
             if (g.a_im) g.a_im += k0;
           }
-          if (linear(L->tab[3], st)) {
-            const int64_t k0 = host_tables_[L->tab[3]];
+          if (linear_table(L->tab[3], g.K, st, k0)) {
             g.bk_step = st;
             g.b_re += k0;
             if (g.b_im) g.b_im += k0;

@@ -310,6 +310,7 @@ class Graph {
   // digit planes of static operands (split precision), filled at bind
   bool is_static(int id) const;
   void simplify_split_tables(SplitArgs& a, int64_t rtab_off, int64_t ktab_off) const;
+  bool linear_table(int64_t tab, int64_t K, int64_t& step, int64_t& first) const;
   struct DigitCache {
     int64_t digits_off = 0, scale_off = 0;
     bool filled = false;
