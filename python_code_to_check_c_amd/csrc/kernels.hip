@@ -2,6 +2,7 @@
 #include "kernels.hpp"
 
 #include <cstdio>
+#include <utility>
 #include <cstdlib>
 
 namespace tdcc {
@@ -808,6 +809,40 @@ static bool lincomb_use_tiled(const LincombArgs& a) {
   return false;
 }
 
+// The arguments with the index axes reordered so that the tiled kernel
+// applies: a term read along a leading axis d (unit stride there, not along
+// the output's innermost axis) has axis d exchanged with axis rank - 2.  The
+// sum is taken over the same index tuples, term by term in the same order, so
+// every element gets the same value; only the order in which blocks visit the
+// elements changes.  False when no term is read that way.
+// TDCC_LINCOMB_SWAP=0: no reordering (such sums on the element-wise kernel, a
+// parity-test mode).
+static bool lincomb_swap_on() {
+  const char* e = std::getenv("TDCC_LINCOMB_SWAP");
+  return !e || std::atoi(e) != 0;
+}
+static bool lincomb_tiled_view(const LincombArgs& a, LincombArgs& b) {
+  b = a;
+  if (lincomb_use_tiled(a)) return true;
+  const int R = a.rank;
+  if (R < 3 || a.ost[R - 1] != 1 || !lincomb_swap_on()) return false;
+  for (int t = 0; t < a.nterms; ++t) {
+    const LincombTerm& T = a.t[t];
+    if (T.re2 || T.st[R - 1] == 1) continue;
+    for (int d = 0; d < R - 2; ++d) {
+      if (T.st[d] != 1) continue;
+      std::swap(b.dims[d], b.dims[R - 2]);
+      std::swap(b.ost[d], b.ost[R - 2]);
+      for (int u = 0; u < b.nterms; ++u) {
+        std::swap(b.t[u].st[d], b.t[u].st[R - 2]);
+        std::swap(b.t[u].st2[d], b.t[u].st2[R - 2]);
+      }
+      return true;
+    }
+  }
+  return false;
+}
+
 void launch_lincomb(const LincombArgs& a, hipStream_t s) {
   if (a.total <= 0) return;
   const unsigned nz = (unsigned)(a.nz < 1 ? 1 : a.nz);
@@ -831,14 +866,15 @@ void launch_lincomb(const LincombArgs& a, hipStream_t s) {
     }
     return;
   }
-  if (lincomb_use_tiled(a)) {
-    const int R = a.rank;
-    const int64_t n2 = (a.dims[R - 2] + kLinTile - 1) / kLinTile, n3 = (a.dims[R - 1] + kLinTile - 1) / kLinTile;
+  LincombArgs v;
+  if (lincomb_tiled_view(a, v)) {
+    const int R = v.rank;
+    const int64_t n2 = (v.dims[R - 2] + kLinTile - 1) / kLinTile, n3 = (v.dims[R - 1] + kLinTile - 1) / kLinTile;
     int64_t outer = 1;
-    for (int d = 0; d < R - 2; ++d) outer *= a.dims[d];
+    for (int d = 0; d < R - 2; ++d) outer *= v.dims[d];
     const int64_t blocks = outer * n2 * n3;
     if (blocks < (int64_t)1 << 31) {
-      hipLaunchKernelGGL(lincomb_tiled_kernel, dim3((unsigned)blocks, nz), dim3(256), 0, s, a);
+      hipLaunchKernelGGL(lincomb_tiled_kernel, dim3((unsigned)blocks, nz), dim3(256), 0, s, v);
       return;
     }
   }
@@ -849,7 +885,8 @@ void launch_lincomb(const LincombArgs& a, hipStream_t s) {
 const char* lincomb_kernel_name(const LincombArgs& a) {
   if (a.total > 0 && lincomb_use_anti(a)) return "lincomb_anti_kernel";
   if (a.total > 0 && lincomb_is_flat(a)) return lincomb_has_prod(a) ? "lincomb_flat_kernel<prod>" : "lincomb_flat_kernel";
-  if (a.total > 0 && lincomb_use_tiled(a)) return "lincomb_tiled_kernel";
+  LincombArgs v;
+  if (a.total > 0 && lincomb_tiled_view(a, v)) return "lincomb_tiled_kernel";
   return "lincomb_kernel";
 }
 

@@ -43,7 +43,9 @@ RHS = [("T1eq_rhs", _lib.RHS_T1), ("T2eq_rhs", _lib.RHS_T2), ("lam_1eq_rhs", _li
 # their plain terms are laid out like the output); every RK4 step on one
 # stream (by default the T side and the Lambda side run on two); the GETT
 # kernel for every skinny shape with each K offset read from its table (by
-# default evenly stepped offsets are computed).
+# default evenly stepped offsets are computed); linear combinations over a term
+# contiguous along a leading axis on the element-wise kernel (by default that
+# axis is exchanged with the second-to-last and the LDS-tiled kernel runs).
 MODES = {"default": {}, "skinny": {"TDCC_SKINNY_MIN": "0"}, "gett": {"TDCC_SKINNY_MIN": "-1"},
          "skinny_runs": {"TDCC_SKINNY_MIN": "0", "TDCC_SKINNY_RUNS_MIN": "0"},
          "gemm": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_NPAIR_MIN_K": "0"},
@@ -52,11 +54,12 @@ MODES = {"default": {}, "skinny": {"TDCC_SKINNY_MIN": "0"}, "gett": {"TDCC_SKINN
          "dot": {"TDCC_GEMM_MIN_FLOPS": "0", "TDCC_DOT_MIN_K": "0"},
          "no_inline": {"TDCC_INLINE_LINCOMB": "0", "TDCC_FOLD_OUTER": "0"},
          "lincomb_elementwise": {"TDCC_LINCOMB_FLAT_PROD": "0"}, "one_stream": {"TDCC_STREAMS": "1"},
-         "gett_tables": {"TDCC_SKINNY_MIN": "-1", "TDCC_GETT_LINEAR_K": "0"}}
+         "gett_tables": {"TDCC_SKINNY_MIN": "-1", "TDCC_GETT_LINEAR_K": "0"},
+         "lincomb_no_swap": {"TDCC_LINCOMB_SWAP": "0"}}
 ENV_KEYS = ("TDCC_GEMM_MIN_FLOPS", "TDCC_GEMM_SPLITS", "TDCC_SKINNY_MIN", "TDCC_SKINNY_RUNS_MIN", "TDCC_IJ_PAIRS",
             "TDCC_NPAIR_MIN_K",
             "TDCC_LINCOMB_ANTI", "TDCC_DOT_MIN_K", "TDCC_INLINE_LINCOMB", "TDCC_FOLD_OUTER", "TDCC_LINCOMB_FLAT_PROD",
-            "TDCC_STREAMS", "TDCC_GETT_LINEAR_K")
+            "TDCC_STREAMS", "TDCC_GETT_LINEAR_K", "TDCC_LINCOMB_SWAP")
 
 
 @pytest.fixture(scope="module", params=[(n, m) for m in MODES for n in ("o4v6", "c1_o10v4")],

@@ -309,6 +309,57 @@ def test_fused_three_plane_split_is_bitwise_neutral(digits):
 
 
 @pytest.mark.parametrize("digits", [0, 5])
+def test_lincomb_axis_swap_is_bitwise_neutral(digits):
+    """A linear combination whose term is contiguous along a leading axis (a
+    permuted integral block) runs on the LDS-tiled kernel with that axis
+    exchanged for the second-to-last one (TDCC_LINCOMB_SWAP=0: element-wise
+    kernel).  Each element is the same sum in the same term order, so at
+    o = 8, v = 64 the residuals and one profiled step are bitwise equal, and
+    the swap moves at least one launch onto the tiled kernel."""
+    from python_code_to_check_c_amd import synthetic
+    o, v = 8, 64
+    sysm = synthetic.SyntheticSystem(3, o, v, with_tei=False)
+    rng = np.random.default_rng(12)
+
+    def amp(shape):
+        return 1e-2 * (rng.standard_normal(shape) + 1j * rng.standard_normal(shape))
+
+    t1, l1 = amp((o, v)), amp((o, v))
+    t2, l2 = amp((o, o, v, v)), amp((o, o, v, v))
+    for x in (t2, l2):
+        x -= x.transpose(1, 0, 2, 3)
+        x -= x.transpose(0, 1, 3, 2)
+    Fc = sysm.F + 0j
+    out, state, tiled = {}, {}, {}
+    saved = os.environ.get("TDCC_LINCOMB_SWAP")
+    try:
+        for mode in ("1", "0"):
+            os.environ["TDCC_LINCOMB_SWAP"] = mode
+            c = _lib.Context(o, v)
+            try:
+                c.set_system(None, np.asarray(sysm.F), np.asarray(sysm.mu))
+                c.synth_tei(3, 0.05)
+                c.set_precision(digits)
+                out[mode] = [c.rhs(kind, Fc, t1, t2, l1, l2) for _, kind in RHS]
+                c.set_state(t1, t2, l1, l2)
+                names = [r[0] for r in c.profile_step("CCSD", 0.0, 1e-3, 0.923, 2e-3)]
+                tiled[mode] = sum(n == "lincomb_tiled_kernel" for n in names)
+                state[mode] = c.get_state(0)
+            finally:
+                c.close()
+    finally:
+        if saved is None:
+            os.environ.pop("TDCC_LINCOMB_SWAP", None)
+        else:
+            os.environ["TDCC_LINCOMB_SWAP"] = saved
+    for (name, _), x, y in zip(RHS, out["1"], out["0"]):
+        assert np.array_equal(x, y), name
+    for x, y in zip(state["1"], state["0"]):
+        assert np.array_equal(x, y)
+    assert tiled["1"] > tiled["0"], tiled
+
+
+@pytest.mark.parametrize("digits", [0, 5])
 @pytest.mark.parametrize("batch", ["single", "stacked", "side_by_side"])
 def test_two_stream_step_is_bitwise_neutral(digits, batch):
     """An RK4 step runs its T side and its Lambda side (the Lambda chains read
